@@ -1,0 +1,550 @@
+// g2g_apply.hip -- the IE operator matvec on one MI355X.
+//
+// Reference path replaced: APPLY_PRECOND -> MULT_IE_OP -> {distributed FFT,
+// VERTICAL_MULT_GENERAL(_ZERO), inverse FFT} (apply_ie_operator_module.F90:
+// 39-374, Distributed_FFT.F90:220-238,437-505,639-788).  Five kernels:
+//   K1 fwd_x   : pre-scale by gsig, zero-pad x to 2Nx, FFT along x (LDS),
+//                write X tiled [chunk][y][kx][row]
+//   K2 fwd_y   : per kx column, zero-pad y to 2Ny, FFT along y, write the
+//                spectrum S[kx][ky][c*Nz+z] (Nyquist row/column skipped)
+//   K3 contract: per (kx,ky) quadrant block, the 3Nz x 3Nz block product for
+//                the four points (+-kx,+-ky) sharing one stored tensor block,
+//                staged through LDS; Nyquist points written as zero
+//   K4 inv_y   : inverse FFT along y, keep y < Ny
+//   K5 inv_x   : inverse FFT along x, keep x < Nx, post-scale
+//                out = sqrt(Re sb) * (asiga*v - F/(dz*4NxNy))
+// Each kernel is templated on the real type (double: the FP64 drop-in;
+// float: the complex64 inner-solver operator).
+#include "g2g_fft.hpp"
+#include "g2g_op.hpp"
+
+#include <cmath>
+
+namespace g2g {
+
+constexpr int NT = 256;  // threads per block for every apply kernel
+
+template <typename R> constexpr int nr_for(int L) {
+  if constexpr (sizeof(R) == 8)
+    return L <= 64 ? 32 : (L <= 128 ? 16 : (L <= 512 ? 8 : 4));
+  else
+    return L <= 64 ? 64 : (L <= 128 ? 32 : (L <= 512 ? 16 : 8));
+}
+template <typename R> int nr_rt(int L) {
+  if (sizeof(R) == 8) return L <= 64 ? 32 : (L <= 128 ? 16 : (L <= 512 ? 8 : 4));
+  return L <= 64 ? 64 : (L <= 128 ? 32 : (L <= 512 ? 16 : 8));
+}
+
+struct Geo {
+  int Nx, Ny, Nz, Nz3;
+  int nrx_log;  // X tile rows = 1 << nrx_log
+  int64_t nc;   // cells
+  int slot[3];
+};
+
+__device__ inline int64_t xidx(const Geo& g, int cz, int y, int kx) {
+  const int nrx = 1 << g.nrx_log;
+  return ((((int64_t)(cz >> g.nrx_log) * g.Ny + y) * (2 * g.Nx) + kx) << g.nrx_log) + (cz & (nrx - 1));
+}
+
+template <typename T2, int L>
+__device__ inline void load_tw(T2* tw, const T2* g) {
+  for (int i = threadIdx.x; i < L; i += NT) tw[i] = g[i];
+}
+
+// ---------------------------------------------------------------- K1 ----
+template <typename R, int L>
+__global__ __launch_bounds__(NT) void k_fwd_x(const cx_t<R>* __restrict__ v,
+                                              const cx_t<R>* __restrict__ gsig,
+                                              cx_t<R>* __restrict__ X,
+                                              const cx_t<R>* __restrict__ twg, Geo g) {
+  using T2 = cx_t<R>;
+  constexpr int NR = nr_for<R>(L);
+  constexpr int LS = L + 1;
+  __shared__ T2 buf[NR * LS];
+  __shared__ T2 tw[L];
+  const int y = blockIdx.x, chunk = blockIdx.y, Nx = g.Nx;
+  load_tw<T2, L>(tw, twg);
+  for (int i = threadIdx.x; i < NR * Nx; i += NT) {
+    const int s = i / Nx, x = i - s * Nx, cz = chunk * NR + s;
+    T2 val = mk<T2>(0, 0);
+    if (cz < g.Nz3) {
+      const int c = cz / g.Nz, z = cz - c * g.Nz;
+      const int64_t ci = ((int64_t)z * g.Ny + y) * Nx + x;
+      val = v[g.slot[c] * g.nc + ci] * gsig[ci];
+    }
+    buf[s * LS + x] = val;
+    buf[s * LS + x + Nx] = mk<T2>(0, 0);
+  }
+  __syncthreads();
+  lds_fft<T2, L, NR, LS, NT, -1>(buf, tw);
+  // X tile for this chunk is contiguous: [y][kx][s]
+  T2* dst = X + (((int64_t)chunk * g.Ny + y) * L) * NR;
+  for (int i = threadIdx.x; i < NR * L; i += NT) {
+    const int kx = i / NR, s = i - kx * NR;
+    dst[i] = buf[s * LS + kx];
+  }
+}
+
+// ---------------------------------------------------------------- K2 ----
+template <typename R, int L>
+__global__ __launch_bounds__(NT) void k_fwd_y(const cx_t<R>* __restrict__ X,
+                                              cx_t<R>* __restrict__ S,
+                                              const cx_t<R>* __restrict__ twg, Geo g) {
+  using T2 = cx_t<R>;
+  constexpr int NR = nr_for<R>(L);
+  constexpr int LS = L + 1;
+  __shared__ T2 buf[NR * LS];
+  __shared__ T2 tw[L];
+  const int kx = blockIdx.x, chunk = blockIdx.y, Ny = g.Ny;
+  if (kx == g.Nx) return;  // x-Nyquist column is never used
+  load_tw<T2, L>(tw, twg);
+  for (int i = threadIdx.x; i < NR * Ny; i += NT) {
+    const int yy = i / NR, s = i - yy * NR, cz = chunk * NR + s;
+    buf[s * LS + yy] = (cz < g.Nz3) ? X[xidx(g, cz, yy, kx)] : mk<T2>(0, 0);
+    buf[s * LS + yy + Ny] = mk<T2>(0, 0);
+  }
+  __syncthreads();
+  lds_fft<T2, L, NR, LS, NT, -1>(buf, tw);
+  for (int i = threadIdx.x; i < NR * L; i += NT) {
+    const int ky = i / NR, s = i - ky * NR, cz = chunk * NR + s;
+    if (ky != Ny && cz < g.Nz3) S[((int64_t)kx * L + ky) * g.Nz3 + cz] = buf[s * LS + ky];
+  }
+}
+
+// ---------------------------------------------------------------- K4 ----
+template <typename R, int L>
+__global__ __launch_bounds__(NT) void k_inv_y(const cx_t<R>* __restrict__ S,
+                                              cx_t<R>* __restrict__ X,
+                                              const cx_t<R>* __restrict__ twg, Geo g) {
+  using T2 = cx_t<R>;
+  constexpr int NR = nr_for<R>(L);
+  constexpr int LS = L + 1;
+  __shared__ T2 buf[NR * LS];
+  __shared__ T2 tw[L];
+  const int kx = blockIdx.x, chunk = blockIdx.y, Ny = g.Ny;
+  if (kx == g.Nx) {  // x-Nyquist: zero (InitialTransposeRepack, Distributed_FFT.F90:723-724)
+    for (int i = threadIdx.x; i < NR * Ny; i += NT) {
+      const int yy = i / NR, s = i - yy * NR, cz = chunk * NR + s;
+      if (cz < g.Nz3) X[xidx(g, cz, yy, kx)] = mk<T2>(0, 0);
+    }
+    return;
+  }
+  load_tw<T2, L>(tw, twg);
+  for (int i = threadIdx.x; i < NR * L; i += NT) {
+    const int ky = i / NR, s = i - ky * NR, cz = chunk * NR + s;
+    buf[s * LS + ky] = (cz < g.Nz3) ? S[((int64_t)kx * L + ky) * g.Nz3 + cz] : mk<T2>(0, 0);
+  }
+  __syncthreads();
+  lds_fft<T2, L, NR, LS, NT, +1>(buf, tw);
+  for (int i = threadIdx.x; i < NR * Ny; i += NT) {
+    const int yy = i / NR, s = i - yy * NR, cz = chunk * NR + s;
+    if (cz < g.Nz3) X[xidx(g, cz, yy, kx)] = buf[s * LS + yy];
+  }
+}
+
+// ---------------------------------------------------------------- K5 ----
+template <typename R, int L>
+__global__ __launch_bounds__(NT) void k_inv_x(const cx_t<R>* __restrict__ X,
+                                              const cx_t<R>* __restrict__ vin,
+                                              const cx_t<R>* __restrict__ asiga,
+                                              const R* __restrict__ sq,
+                                              const R* __restrict__ dzn,
+                                              cx_t<R>* __restrict__ vout,
+                                              const cx_t<R>* __restrict__ twg, Geo g) {
+  using T2 = cx_t<R>;
+  constexpr int NR = nr_for<R>(L);
+  constexpr int LS = L + 1;
+  __shared__ T2 buf[NR * LS];
+  __shared__ T2 tw[L];
+  const int y = blockIdx.x, chunk = blockIdx.y, Nx = g.Nx;
+  load_tw<T2, L>(tw, twg);
+  const T2* src = X + (((int64_t)chunk * g.Ny + y) * L) * NR;
+  for (int i = threadIdx.x; i < NR * L; i += NT) {
+    const int kx = i / NR, s = i - kx * NR;
+    buf[s * LS + kx] = src[i];
+  }
+  __syncthreads();
+  lds_fft<T2, L, NR, LS, NT, +1>(buf, tw);
+  for (int i = threadIdx.x; i < NR * Nx; i += NT) {
+    const int s = i / Nx, x = i - s * Nx, cz = chunk * NR + s;
+    if (cz < g.Nz3) {
+      const int c = cz / g.Nz, z = cz - c * g.Nz;
+      const int64_t ci = ((int64_t)z * g.Ny + y) * Nx + x;
+      const int64_t vi = g.slot[c] * g.nc + ci;
+      T2 d1 = vin[vi] * asiga[ci];
+      T2 d2 = d1 - buf[s * LS + x] * dzn[z];
+      vout[vi] = d2 * sq[z];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- K3 ----
+// One block per stored (kx,ky) quadrant block, serving the four spectrum
+// points p = 0:(kx,ky) 1:(-kx,ky) 2:(kx,-ky) 3:(-kx,-ky) with
+//   out_x = Sxx x + s m Sxy y + s Axz z
+//   out_y = s m Sxy x + Syy y + m Ayz z
+//   out_z = -s Axz^T x - m Ayz^T y + Szz z       (s = +-1 for +-kx, m for +-ky)
+// (VERTICAL_MULT_GENERAL, apply_ie_operator_module.F90:276-292).  The six
+// components (xx,xy,yy,zz packed symmetric; xz,yz full) stream through two
+// LDS buffers: while the block works on component k, the coalesced global
+// loads of component k+1 are in flight in registers.  Thread (row i, part)
+// owns receiver layer i for all four points and sums source layers
+// j = part, part+split, ...; partial sums are combined with lane shuffles.
+// Register arrays are spelled out as named variables (no private arrays, so
+// nothing is placed in scratch memory).
+#define G2G_FOR4(X) X(0) X(1) X(2) X(3)
+#define G2G_FOR16(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15)
+
+template <typename R, int PF>  // PF prefetch registers per thread: Nz*Nz <= PF*NT
+__global__ __launch_bounds__(NT) void k_contract(cx_t<R>* __restrict__ S,
+                                                 const cx_t<R>* __restrict__ G, Geo g,
+                                                 int64_t BS, int split_log) {
+  using T2 = cx_t<R>;
+  static_assert(PF == 4 || PF == 16, "PF must be 4 or 16");
+  extern __shared__ unsigned char smem_raw[];
+  const int Nz = g.Nz, Nz3 = g.Nz3, NZP = Nz + 1;
+  const int P = Nz * (Nz + 1) / 2;
+  const int GB = Nz * NZP;                    // one LDS component buffer
+  T2* vin = reinterpret_cast<T2*>(smem_raw);  // [4][3Nz]
+  T2* gbuf = vin + 4 * Nz3;                   // [2][GB]
+  const int b = blockIdx.x, kx = b % g.Nx, ky = b / g.Nx;
+  const int Nx2 = 2 * g.Nx, Ny2 = 2 * g.Ny;
+  // point p: bit 0 = -kx (x-Nyquist column when kx == 0), bit 1 = -ky
+  const int gNx = g.Nx, gNy = g.Ny;
+#define PX(p) (((p) & 1) ? (kx == 0 ? gNx : Nx2 - kx) : kx)
+#define PY(p) (((p) & 2) ? (ky == 0 ? gNy : Ny2 - ky) : ky)
+#define VAL(p) ((!((p) & 1) || kx > 0) && (!((p) & 2) || ky > 0))
+  const T2* blk = G + (int64_t)b * BS;
+  const T2 zero = mk<T2>(0, 0);
+#define G2G_PFDECL(q) T2 pf##q = zero;
+  G2G_FOR16(G2G_PFDECL)
+#undef G2G_PFDECL
+#define G2G_CLEN(c) ((c) < 4 ? P : Nz * Nz)
+#define G2G_CSRC(c) ((c) < 4 ? blk + (c) * P : blk + 4 * P + ((c) - 4) * Nz * Nz)
+#define G2G_LD(q) if ((q) < PF && threadIdx.x + (q) * NT < n_) pf##q = src_[threadIdx.x + (q) * NT];
+#define G2G_PREFETCH(c)                  \
+  {                                      \
+    const T2* src_ = G2G_CSRC(c);        \
+    const int n_ = G2G_CLEN(c);          \
+    G2G_FOR16(G2G_LD)                    \
+  }
+#define G2G_ST(q)                                          \
+  if ((q) < PF && threadIdx.x + (q) * NT < n_) {           \
+    const int e = threadIdx.x + (q) * NT;                  \
+    if (c_ < 4) dst_[e] = pf##q;                           \
+    else dst_[(e / Nz) * NZP + e % Nz] = pf##q;            \
+  }
+#define G2G_STASH(c, dst)            \
+  {                                  \
+    const int c_ = (c);              \
+    T2* dst_ = (dst);                \
+    const int n_ = G2G_CLEN(c_);     \
+    G2G_FOR16(G2G_ST)                \
+  }
+  G2G_PREFETCH(0)
+  for (int i = threadIdx.x; i < 4 * Nz3; i += NT) {
+    const int p = i / Nz3, e = i - p * Nz3;
+    T2 vv = zero;
+    if (VAL(p)) vv = S[((int64_t)PX(p) * Ny2 + PY(p)) * Nz3 + e];
+    vin[i] = vv;
+  }
+  G2G_STASH(0, gbuf)
+  __syncthreads();
+  const int split = 1 << split_log;
+  const int part = threadIdx.x & (split - 1);
+  const int i = threadIdx.x >> split_log;
+  const bool active = i < Nz;
+  // sign of point p: s = -1 for -kx (bit 0), m = -1 for -ky (bit 1)
+#define G2G_SG(p) (((p) & 1) ? R(-1) : R(1))
+#define G2G_MR(p) (((p) & 2) ? R(-1) : R(1))
+#define G2G_ACC(p) T2 ax##p = zero, ay##p = zero, az##p = zero;
+  G2G_FOR4(G2G_ACC)
+#undef G2G_ACC
+  for (int comp = 0; comp < 6; comp++) {
+    const T2* cur = gbuf + (comp & 1) * GB;
+    if (comp < 5) G2G_PREFETCH(comp + 1)
+    if (active) {
+#define G2G_T(p) T2 t1##p = zero, t2##p = zero;
+      G2G_FOR4(G2G_T)
+#undef G2G_T
+      if (comp < 4) {
+        const int ii = i * (i + 1) / 2;
+        const int off1 = comp == 0 ? 0 : (comp == 3 ? 2 * Nz : Nz);  // x for xx, z for zz, y otherwise
+        for (int j = part; j < Nz; j += split) {
+          const T2 sv = cur[j <= i ? j + ii : i + j * (j + 1) / 2];
+#define G2G_SYM(p)                                      \
+  {                                                     \
+    const T2* v = vin + (p) * Nz3;                      \
+    cmac(t1##p, sv, v[off1 + j]);                       \
+    if (comp == 1) cmac(t2##p, sv, v[j]);               \
+  }
+          G2G_FOR4(G2G_SYM)
+#undef G2G_SYM
+        }
+#define G2G_FOLD(p)                                                  \
+  if (comp == 0) ax##p = ax##p + t1##p;                              \
+  else if (comp == 1) {                                              \
+    ax##p = ax##p + t1##p * (G2G_SG(p) * G2G_MR(p));                 \
+    ay##p = ay##p + t2##p * (G2G_SG(p) * G2G_MR(p));                 \
+  } else if (comp == 2) ay##p = ay##p + t1##p;                       \
+  else az##p = az##p + t1##p;
+        G2G_FOR4(G2G_FOLD)
+#undef G2G_FOLD
+      } else {
+        const int offT = comp == 4 ? 0 : Nz;  // A^T acts on x (xz) or y (yz)
+        for (int j = part; j < Nz; j += split) {
+          const T2 a = cur[j * NZP + i];   // A(i,j)
+          const T2 at = cur[i * NZP + j];  // A(j,i)
+#define G2G_ASY(p)                          \
+  {                                         \
+    const T2* v = vin + (p) * Nz3;          \
+    cmac(t1##p, a, v[2 * Nz + j]);          \
+    cmac(t2##p, at, v[offT + j]);           \
+  }
+          G2G_FOR4(G2G_ASY)
+#undef G2G_ASY
+        }
+#define G2G_FOLD(p)                                   \
+  if (comp == 4) {                                    \
+    ax##p = ax##p + t1##p * G2G_SG(p);                \
+    az##p = az##p - t2##p * G2G_SG(p);                \
+  } else {                                            \
+    ay##p = ay##p + t1##p * G2G_MR(p);                \
+    az##p = az##p - t2##p * G2G_MR(p);                \
+  }
+        G2G_FOR4(G2G_FOLD)
+#undef G2G_FOLD
+      }
+    }
+    if (comp < 5) G2G_STASH(comp + 1, gbuf + ((comp + 1) & 1) * GB)
+    __syncthreads();
+  }
+  for (int o = 1; o < split; o <<= 1) {
+#define G2G_RED(p)                                                       \
+  ax##p.x += __shfl_xor(ax##p.x, o); ax##p.y += __shfl_xor(ax##p.y, o);  \
+  ay##p.x += __shfl_xor(ay##p.x, o); ay##p.y += __shfl_xor(ay##p.y, o);  \
+  az##p.x += __shfl_xor(az##p.x, o); az##p.y += __shfl_xor(az##p.y, o);
+    G2G_FOR4(G2G_RED)
+#undef G2G_RED
+  }
+  if (active && part == 0) {
+#define G2G_OUT(p)                                                  \
+  {                                                                 \
+    T2* dst = S + ((int64_t)PX(p) * Ny2 + PY(p)) * Nz3;             \
+    if (!VAL(p)) ax##p = ay##p = az##p = zero;                      \
+    dst[i] = ax##p;                                                 \
+    dst[Nz + i] = ay##p;                                            \
+    dst[2 * Nz + i] = az##p;                                        \
+  }
+    G2G_FOR4(G2G_OUT)
+#undef G2G_OUT
+  }
+#undef G2G_SG
+#undef G2G_MR
+#undef PX
+#undef PY
+#undef VAL
+#undef G2G_CLEN
+#undef G2G_CSRC
+#undef G2G_LD
+#undef G2G_PREFETCH
+#undef G2G_ST
+#undef G2G_STASH
+}
+
+// ------------------------------------------------------------ dispatch ---
+template <typename R, template <typename, int> class KF> struct Dummy {};
+
+static Geo make_geo(const Operator& op, int nrx) {
+  Geo g;
+  g.Nx = op.Nx;
+  g.Ny = op.Ny;
+  g.Nz = op.Nz;
+  g.Nz3 = 3 * op.Nz;
+  g.nrx_log = ilog2(nrx);
+  g.nc = op.ncell();
+  for (int c = 0; c < 3; c++) g.slot[c] = op.comp_slot[c];
+  return g;
+}
+
+template <typename R> static void make_twiddles(int L, cx_t<R>** dptr) {
+  std::vector<cx_t<R>> h(L);
+  for (int t = 0; t < L; t++) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * t / L;
+    h[t] = mk<cx_t<R>>((R)cosl(a), (R)sinl(a));
+  }
+  G2G_HIP(hipMalloc(dptr, sizeof(cx_t<R>) * L));
+  G2G_HIP(hipMemcpy(*dptr, h.data(), sizeof(cx_t<R>) * L, hipMemcpyHostToDevice));
+}
+
+template <typename R> void op_alloc_buffers(Operator& op, OpBuffers<R>& b, bool with_tensor) {
+  using T2 = cx_t<R>;
+  const int Nx2 = 2 * op.Nx, Ny2 = 2 * op.Ny, Nz3 = 3 * op.Nz;
+  b.NR = nr_rt<R>(Nx2);
+  b.nchunk = (Nz3 + b.NR - 1) / b.NR;
+  const size_t nX = (size_t)b.nchunk * b.NR * op.Ny * Nx2;
+  const size_t nS = (size_t)Nx2 * Ny2 * Nz3;
+  const size_t nc = op.ncell();
+  b.bytes = 0;
+  auto A = [&](auto** p, size_t n, size_t es) {
+    G2G_HIP(hipMalloc((void**)p, n * es));
+    b.bytes += n * es;
+  };
+  if (with_tensor) A(&b.G, (size_t)op.Nx * op.Ny * op.BS, sizeof(T2));
+  A(&b.X, nX, sizeof(T2));
+  A(&b.S, nS, sizeof(T2));
+  A(&b.gsig, nc, sizeof(T2));
+  A(&b.asiga, nc, sizeof(T2));
+  A(&b.sq, (size_t)op.Nz, sizeof(R));
+  A(&b.dzn, (size_t)op.Nz, sizeof(R));
+  make_twiddles<R>(Nx2, &b.twx);
+  make_twiddles<R>(Ny2, &b.twy);
+  std::vector<R> dzn(op.Nz);
+  for (int z = 0; z < op.Nz; z++) dzn[z] = (R)(1.0 / (op.dz[z] * 4.0 * op.Nx * op.Ny));
+  G2G_HIP(hipMemcpy(b.dzn, dzn.data(), sizeof(R) * op.Nz, hipMemcpyHostToDevice));
+}
+
+template <typename R> void op_free_buffers(OpBuffers<R>& b) {
+  auto F = [](auto*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+  };
+  F(b.G); F(b.X); F(b.S); F(b.gsig); F(b.asiga); F(b.sq); F(b.dzn); F(b.twx); F(b.twy);
+}
+
+template <typename R>
+void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, hipStream_t s,
+              bool timed) {
+  G2G_CHECK(op.have_tensor && op.have_sigma, -4, "operator not ready: tensor or conductivity missing");
+  const int Nx2 = 2 * op.Nx, Ny2 = 2 * op.Ny, Nz3 = 3 * op.Nz;
+  Geo g = make_geo(op, b.NR);
+  const int nry = nr_rt<R>(Ny2);
+  const int chy = (Nz3 + nry - 1) / nry;
+  if (timed) G2G_HIP(hipEventRecord(op.ev[0], s));
+  pow2_dispatch(Nx2, [&](auto LC) {
+    constexpr int LL = decltype(LC)::value;
+    hipLaunchKernelGGL((k_fwd_x<R, LL>), dim3(op.Ny, b.nchunk), dim3(NT), 0, s,
+                                         vin, b.gsig, b.X, b.twx, g);
+  });
+  pow2_dispatch(Ny2, [&](auto LC) {
+    constexpr int LL = decltype(LC)::value;
+    hipLaunchKernelGGL((k_fwd_y<R, LL>), dim3(Nx2, chy), dim3(NT), 0, s, b.X,
+                                         b.S, b.twy, g);
+  });
+  if (timed) G2G_HIP(hipEventRecord(op.ev[1], s));
+  {
+    // rows = Nz receiver layers, split lanes per row (power of two)
+    int split_log = 0;
+    while ((op.Nz << (split_log + 1)) <= NT) split_log++;
+    const size_t sm = sizeof(cx_t<R>) * (4 * (size_t)Nz3 + 2 * (size_t)op.Nz * (op.Nz + 1));
+    auto launch = [&](auto kern) {
+      if (sm > 65536) {
+        G2G_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    160 * 1024));
+      }
+      hipLaunchKernelGGL(kern, dim3(op.Nx * op.Ny), dim3(NT), sm, s, b.S, b.G, g, op.BS, split_log);
+    };
+    if (op.Nz * op.Nz <= 4 * NT) launch(k_contract<R, 4>);
+    else launch(k_contract<R, 16>);
+  }
+  if (timed) G2G_HIP(hipEventRecord(op.ev[2], s));
+  pow2_dispatch(Ny2, [&](auto LC) {
+    constexpr int LL = decltype(LC)::value;
+    hipLaunchKernelGGL((k_inv_y<R, LL>), dim3(Nx2, chy), dim3(NT), 0, s, b.S,
+                                         b.X, b.twy, g);
+  });
+  pow2_dispatch(Nx2, [&](auto LC) {
+    constexpr int LL = decltype(LC)::value;
+    hipLaunchKernelGGL((k_inv_x<R, LL>), dim3(op.Ny, b.nchunk), dim3(NT), 0, s,
+                                         b.X, vin, b.asiga, b.sq, b.dzn, vout, b.twx, g);
+  });
+  if (timed) G2G_HIP(hipEventRecord(op.ev[3], s));
+  G2G_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------ conductivity ---
+template <typename R>
+__global__ void k_sigma(const zd* __restrict__ csiga, const zd* __restrict__ csigb,
+                        const double* __restrict__ sq, cx_t<R>* gsig, cx_t<R>* asiga,
+                        int64_t ncxy, int Nz) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncxy * Nz) return;
+  const int z = (int)(i / ncxy);
+  const zd sa = csiga[i], sb = csigb[z];
+  // a = 2 sqrt(Re sb) / (sa + conj(sb)); g = (sa - sb) * a
+  const zd a = cdiv(mkz(2.0 * sq[z], 0.0), mkz(sa.x + sb.x, sa.y - sb.y));
+  const zd gg = mkz(sa.x - sb.x, sa.y - sb.y) * a;
+  gsig[i] = mk<cx_t<R>>((R)gg.x, (R)gg.y);
+  asiga[i] = mk<cx_t<R>>((R)a.x, (R)a.y);
+}
+
+template <typename R> __global__ void k_to_real(const double* a, R* b, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (R)a[i];
+}
+
+__global__ void k_z_to_f(const zd* __restrict__ a, zf* __restrict__ b, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = zf{(float)a[i].x, (float)a[i].y};
+}
+
+// complex64 copies of the per-cell scalings for the complex64 operator
+void op_sigma_to_f32(Operator& op) {
+  const int64_t n = op.ncell();
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_z_to_f, dim3(blocks), dim3(256), 0, op.stream, op.d64.gsig, op.d32.gsig, n);
+  hipLaunchKernelGGL(k_z_to_f, dim3(blocks), dim3(256), 0, op.stream, op.d64.asiga, op.d32.asiga, n);
+  hipLaunchKernelGGL((k_to_real<float>), dim3(1), dim3(256), 0, op.stream, op.d64.sq, op.d32.sq, op.Nz);
+  G2G_HIP(hipGetLastError());
+}
+
+void op_set_sigma(Operator& op, const zd* d_csiga) {
+  G2G_CHECK(op.have_bkg, -4, "background not set");
+  const int Nz = op.Nz;
+  zd* d_csigb;
+  double* d_sq;
+  G2G_HIP(hipMalloc(&d_csigb, sizeof(zd) * Nz));
+  G2G_HIP(hipMalloc(&d_sq, sizeof(double) * Nz));
+  G2G_HIP(hipMemcpy(d_csigb, op.csigb.data(), sizeof(zd) * Nz, hipMemcpyHostToDevice));
+  G2G_HIP(hipMemcpy(d_sq, op.sqsigb.data(), sizeof(double) * Nz, hipMemcpyHostToDevice));
+  const int64_t n = op.ncell(), ncxy = (int64_t)op.Nx * op.Ny;
+  const int blocks = (int)((n + 255) / 256);
+  hipLaunchKernelGGL((k_sigma<double>), dim3(blocks), dim3(256), 0, op.stream, d_csiga, d_csigb,
+                     d_sq, op.d64.gsig, op.d64.asiga, ncxy, Nz);
+  G2G_HIP(hipMemcpyAsync(op.d64.sq, d_sq, sizeof(double) * Nz, hipMemcpyDeviceToDevice, op.stream));
+  op.have_sigma = true;
+  if (op.d32.gsig) op_sigma_to_f32(op);
+  G2G_HIP(hipGetLastError());
+  G2G_HIP(hipStreamSynchronize(op.stream));
+  (void)hipFree(d_csigb);
+  (void)hipFree(d_sq);
+}
+
+__global__ void k_g_to_f32(const zd* __restrict__ a, zf* __restrict__ b, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = zf{(float)a[i].x, (float)a[i].y};
+}
+
+void op_make_g32(Operator& op) {
+  G2G_CHECK(op.have_tensor, -4, "tensor not computed");
+  if (!op.d32.X) op_alloc_buffers<float>(op, op.d32, true);
+  const int64_t n = (int64_t)op.Nx * op.Ny * op.BS;
+  hipLaunchKernelGGL(k_g_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, op.stream,
+                     op.d64.G, op.d32.G, n);
+  G2G_HIP(hipGetLastError());
+  if (op.have_sigma) op_sigma_to_f32(op);
+  G2G_HIP(hipStreamSynchronize(op.stream));
+  op.have_g32 = true;
+}
+
+template void op_alloc_buffers<double>(Operator&, OpBuffers<double>&, bool);
+template void op_alloc_buffers<float>(Operator&, OpBuffers<float>&, bool);
+template void op_free_buffers<double>(OpBuffers<double>&);
+template void op_free_buffers<float>(OpBuffers<float>&);
+template void op_apply<double>(Operator&, OpBuffers<double>&, const zd*, zd*,
+                               hipStream_t, bool);
+template void op_apply<float>(Operator&, OpBuffers<float>&, const zf*, zf*, hipStream_t,
+                              bool);
+
+}  // namespace g2g

@@ -1,0 +1,595 @@
+// g2g_capi.hip -- extern "C" entry points (include/giem2g_amd.h).
+#include <complex>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/giem2g_amd.h"
+#include "g2g_op.hpp"
+#include "g2g_solver.hpp"
+
+using namespace g2g;
+using zc = std::complex<double>;
+
+namespace {
+
+thread_local std::string g_last_error;
+void (*g_logger)(const char*) = nullptr;
+
+constexpr double PI_D = 3.1415926535897932384626433832795028841971693993751;
+constexpr double MU0 = 4.0 * PI_D * 1e-7;
+
+void log_msg(const std::string& m) {
+  if (g_logger) g_logger(m.c_str());
+  else std::fprintf(stdout, "giem2g: %s\n", m.c_str());
+}
+
+template <typename F> int guard(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.code < 0 ? e.code : -1;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return -1;
+  }
+}
+
+struct Handle {
+  Operator op;
+  std::vector<double> hist;
+  PrecondHook learned;
+};
+
+Handle* H(void* p) {
+  G2G_CHECK(p != nullptr, -2, "null operator handle");
+  return static_cast<Handle*>(p);
+}
+
+// GetLayerSingle (data_types_module.F90:75-103)
+int get_layer(const Operator& op, double z) {
+  if (std::fabs(z) < 1e-15) return 0;
+  if (z < 0) return -1;
+  if (op.Nl == 1) return 1;
+  std::vector<double> depth(op.Nl + 1, 0.0);
+  for (int i = 1; i <= op.Nl - 1; i++) depth[i] = depth[i - 1] + op.thick[i];
+  if (z > depth[op.Nl - 1]) return op.Nl;
+  int l = 1;
+  while (z > depth[l]) l++;
+  return l;
+}
+
+void op_init(Operator& op, int Nx, int Ny, int Nz, double dx, double dy, const double* z) {
+  G2G_CHECK(Nx >= 8 && Ny >= 8 && is_pow2(Nx) && is_pow2(Ny) && Nx <= 512 && Ny <= 512, -3,
+            "Nx and Ny must be powers of two in [8,512]");
+  G2G_CHECK(Nz >= 1 && Nz <= 64, -3, "Nz must be in [1,64]");
+  G2G_CHECK(dx > 0 && dy > 0, -3, "dx, dy must be positive");
+  op.Nx = Nx;
+  op.Ny = Ny;
+  op.Nz = Nz;
+  op.dx = dx;
+  op.dy = dy;
+  op.z.assign(z, z + Nz + 1);
+  op.dz.resize(Nz);
+  for (int i = 0; i < Nz; i++) {
+    op.dz[i] = op.z[i + 1] - op.z[i];
+    G2G_CHECK(op.dz[i] > 0, -3, "z borders must increase");
+  }
+  op.BS = 4 * (int64_t)Nz * (Nz + 1) / 2 + 2 * (int64_t)Nz * Nz;
+  op.stream = nullptr;  // the default stream orders with torch's default stream
+  for (auto& e : op.ev) G2G_HIP(hipEventCreate(&e));
+  op_alloc_buffers<double>(op, op.d64, true);
+}
+
+void op_fini(Operator& op) {
+  op_free_buffers<double>(op.d64);
+  op_free_buffers<float>(op.d32);
+  op_free_tensor_scratch(op);
+  for (auto& e : op.ev)
+    if (e) (void)hipEventDestroy(e);
+}
+
+void set_background(Operator& op, int Nl, const double* csigb, const double* thick, double omega) {
+  G2G_CHECK(Nl >= 1 && Nl <= 64, -3, "Nl must be in [1,64]");
+  op.Nl = Nl;
+  op.omega = omega;
+  op.csigma.resize(Nl + 1);
+  for (int i = 0; i <= Nl; i++) op.csigma[i] = mkz(csigb[2 * i], csigb[2 * i + 1]);
+  op.thick.assign(Nl + 1, 0.0);
+  for (int i = 1; i <= Nl - 1; i++) op.thick[i] = thick[i - 1];
+  // Lnumber (mpi_saveload_module.F90:101-102)
+  op.lnum.resize(op.Nz + 1);
+  for (int i = 1; i <= op.Nz; i++) op.lnum[i] = get_layer(op, (op.z[i - 1] + op.z[i]) / 2.0);
+  op.lnum[0] = get_layer(op, op.z[0] * (1.0 - 1e-7));
+  op.csigb.resize(op.Nz);
+  op.sqsigb.resize(op.Nz);
+  for (int iz = 0; iz < op.Nz; iz++) {
+    const int l = op.lnum[iz + 1];
+    G2G_CHECK(l >= 1, -3, "anomaly layers must lie below the surface");
+    op.csigb[iz] = op.csigma[l];
+    op.sqsigb[iz] = std::sqrt(op.csigma[l].x);  // PreparePreconditioner (calc_ie_tensor_module.F90:303)
+  }
+  op.have_bkg = true;
+  op.have_tensor = false;
+  op.have_g32 = false;
+}
+
+void build_tensor(Operator& op, int stage) {
+  if (stage <= 0) {
+    op_build_tensor(op);
+    op.have_tensor = false;
+    op.have_g32 = false;
+  }
+  if (stage != 0) {
+    hipEvent_t a, b;
+    G2G_HIP(hipEventCreate(&a));
+    G2G_HIP(hipEventCreate(&b));
+    G2G_HIP(hipEventRecord(a, op.stream));
+    op_tensor_fft(op, op.d64.G);
+    G2G_HIP(hipEventRecord(b, op.stream));
+    G2G_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    G2G_HIP(hipEventElapsedTime(&ms, a, b));
+    op.cnt.t_tensor_fft = ms * 1e-3;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    op.have_tensor = true;
+    if (op.d32.G) op_make_g32(op);
+  }
+}
+
+void set_sigma_dev(Operator& op, const void* d) {
+  op_set_sigma(op, static_cast<const zd*>(d));
+}
+
+void set_sigma_host(Operator& op, const double* h) {
+  zd* d;
+  const size_t n = op.ncell();
+  G2G_HIP(hipMalloc(&d, sizeof(zd) * n));
+  G2G_HIP(hipMemcpy(d, h, sizeof(zd) * n, hipMemcpyHostToDevice));
+  try {
+    op_set_sigma(op, d);
+  } catch (...) {
+    (void)hipFree(d);
+    throw;
+  }
+  (void)hipFree(d);
+}
+
+void apply_host(Operator& op, const double* in, double* out) {
+  const size_t n = op.nvec();
+  zd *a, *b;
+  G2G_HIP(hipMalloc(&a, sizeof(zd) * n));
+  G2G_HIP(hipMalloc(&b, sizeof(zd) * n));
+  G2G_HIP(hipMemcpy(a, in, sizeof(zd) * n, hipMemcpyHostToDevice));
+  op_apply<double>(op, op.d64, a, b, op.stream, false);
+  op.cnt.matvecs++;
+  G2G_HIP(hipMemcpy(out, b, sizeof(zd) * n, hipMemcpyDeviceToHost));
+  (void)hipFree(a);
+  (void)hipFree(b);
+}
+
+// Calc_Apq with the A matrix at lambda = 0 (apq_module.F90:230-266) and
+// PlaneWaveIntegral (sources_module.F90:67-101).
+void plane_wave(const Operator& op, int pol, double* Eh) {
+  G2G_CHECK(op.have_bkg, -4, "background not set");
+  G2G_CHECK(pol == 0 || pol == 1, -3, "pol must be 0 (EX) or 1 (EY)");
+  const int Nl = op.Nl;
+  std::vector<zc> cs(Nl + 1), k2(Nl + 1), eta(Nl + 1), el(Nl + 1);
+  std::vector<double> depth(Nl + 1, 0.0);
+  for (int i = 0; i <= Nl; i++) {
+    cs[i] = zc(op.csigma[i].x, op.csigma[i].y);
+    k2[i] = zc(0, 1) * cs[i] * MU0 * op.omega;
+  }
+  for (int i = 1; i <= Nl - 1; i++) depth[i] = depth[i - 1] + op.thick[i];
+  const double lm = 0.0;
+  for (int i = 1; i <= Nl; i++) eta[i] = std::sqrt(lm * lm - k2[i]);
+  for (int i = 1; i <= Nl - 1; i++) el[i] = std::exp(-2.0 * eta[i] * op.thick[i]);
+  el[Nl] = 0;
+  std::vector<std::array<zc, 2>> p(Nl + 1), q(Nl + 1), Arr(Nl + 1), w(Nl + 1);
+  {
+    const zc eta0 = std::sqrt(lm * lm - k2[0]), ceta = eta0 / eta[1];
+    p[1][0] = (1.0 - ceta) / (1.0 + ceta);
+    const zc cg = cs[0] / cs[1];
+    p[1][1] = (cg - ceta) / (cg + ceta);
+    for (int i = 2; i <= Nl; i++) {
+      zc a[2];
+      a[0] = eta[i - 1] / eta[i];
+      a[1] = a[0] * cs[i] / cs[i - 1];
+      for (int k = 0; k < 2; k++) {
+        zc ww = el[i - 1] * p[i - 1][k];
+        ww = (ww - 1.0) / (ww + 1.0);
+        ww = ww * a[k];
+        p[i][k] = (1.0 + ww) / (1.0 - ww);
+      }
+    }
+  }
+  q[Nl][0] = q[Nl][1] = 0;
+  if (Nl > 1) {
+    zc a[2];
+    a[0] = eta[Nl] / eta[Nl - 1];
+    a[1] = a[0] * cs[Nl - 1] / cs[Nl];
+    for (int k = 0; k < 2; k++) q[Nl - 1][k] = (1.0 - a[k]) / (1.0 + a[k]);
+    for (int i = Nl - 2; i >= 1; i--) {
+      a[0] = eta[i + 1] / eta[i];
+      a[1] = a[0] * cs[i] / cs[i + 1];
+      for (int k = 0; k < 2; k++) {
+        zc ww = el[i + 1] * q[i + 1][k];
+        ww = (ww - 1.0) / (ww + 1.0);
+        ww = ww * a[k];
+        q[i][k] = (1.0 + ww) / (1.0 - ww);
+      }
+    }
+  }
+  for (int i = 1; i <= Nl; i++)
+    for (int k = 0; k < 2; k++) Arr[i][k] = 1.0 / eta[i] / (1.0 - p[i][k] * q[i][k] * el[i]);
+  // A(I,1,1) column of the layer-coupling matrix, TE part only
+  for (int i = 1; i <= Nl - 1; i++) {
+    zc tmp = (k2[i] - k2[i + 1]);
+    tmp = depth[i] * tmp;
+    tmp = tmp / (eta[i + 1] + eta[i]);
+    for (int k = 0; k < 2; k++) w[i][k] = (1.0 + p[i + 1][k]) / (1.0 + p[i][k] * el[i]) * std::exp(tmp);
+  }
+  std::vector<zc> A1(Nl + 1);  // A(l,1,1) = A(1,l,1)
+  for (int l = 1; l <= Nl; l++) {
+    zc a = Arr[l][0];
+    for (int j = l - 1; j >= 1; j--) a = a * w[j][0];
+    A1[l] = a;
+  }
+  zc t1 = Nl > 1 ? 1.0 + q[1][0] * std::exp(-depth[1] * eta[1] * 2.0) : zc(1.0);
+  t1 = A1[1] * t1;
+  const int64_t nxy = (int64_t)op.Nx * op.Ny, nc = op.ncell();
+  std::memset(Eh, 0, sizeof(double) * 2 * 3 * nc);
+  const int slot = op.comp_slot[pol];
+  for (int Iz = 1; Iz <= op.Nz; Iz++) {
+    const double dz = op.z[Iz] - op.z[Iz - 1];
+    const int l = op.lnum[Iz];
+    const zc e1 = std::exp(-eta[l] * op.z[Iz - 1]), e2 = std::exp(-eta[l] * op.z[Iz]);
+    zc e3 = 0, e4 = 0;
+    if (l < Nl) {
+      e3 = q[l][0] * std::exp(-eta[l] * (2.0 * depth[l] - op.z[Iz - 1]));
+      e4 = q[l][0] * std::exp(-eta[l] * (2.0 * depth[l] - op.z[Iz]));
+    }
+    zc G = (e1 - e2 + e4 - e3) * A1[l] / t1;
+    G = G / dz / eta[l];
+    double* dst = Eh + 2 * (slot * nc + (int64_t)(Iz - 1) * nxy);
+    for (int64_t i = 0; i < nxy; i++) {
+      dst[2 * i] = G.real();
+      dst[2 * i + 1] = G.imag();
+    }
+  }
+}
+
+SolveParams params_from(const giem2g_solver_params* p) {
+  SolveParams s;
+  if (p) {
+    s.outer_m = p->outer_m;
+    s.inner_m = p->inner_m;
+    s.max_restart = p->max_restart;
+    s.tol = p->tol;
+    s.inner_precision = p->inner_precision;
+    s.precond = p->precond;
+  }
+  G2G_CHECK(s.outer_m >= 1 && s.outer_m <= 63 && s.inner_m >= 1 && s.inner_m <= 63, -3,
+            "buffer sizes must be in [1,63]");
+  G2G_CHECK(s.max_restart >= 1 && s.tol > 0, -3, "bad max_restart or tol");
+  return s;
+}
+
+void solve_dev(Handle& h, const zd* Ebkg, const zd* E0, const giem2g_solver_params* pp,
+               zd* Esol, giem2g_solve_info* info) {
+  Operator& op = h.op;
+  G2G_CHECK(op.have_tensor && op.have_sigma, -4, "operator not ready");
+  const SolveParams p = params_from(pp);
+  const int64_t N = op.nvec();
+  zd *rhs, *guess;
+  G2G_HIP(hipMalloc(&rhs, sizeof(zd) * N));
+  G2G_HIP(hipMalloc(&guess, sizeof(zd) * N));
+  hipEvent_t a, b;
+  G2G_HIP(hipEventCreate(&a));
+  G2G_HIP(hipEventCreate(&b));
+  const int64_t m0 = op.cnt.matvecs, m32 = op.cnt.matvecs32;
+  h.hist.clear();
+  SolveResult r;
+  try {
+    G2G_HIP(hipEventRecord(a, op.stream));
+    solve_scale(op, rhs, Ebkg, 0);  // rhs = sqrt(Re sb) E_bkg (ie_solver_module.F90:64-66)
+    if (E0) solve_scale(op, guess, E0, 1);
+    else G2G_HIP(hipMemcpyAsync(guess, rhs, sizeof(zd) * N, hipMemcpyDeviceToDevice, op.stream));
+    r = nested_solve(op, rhs, guess, Esol, p, &h.hist, h.learned.apply ? &h.learned : nullptr);
+    solve_scale(op, Esol, Esol, 2);  // E = asiga x (:85-97)
+    G2G_HIP(hipEventRecord(b, op.stream));
+    G2G_HIP(hipEventSynchronize(b));
+  } catch (...) {
+    (void)hipFree(rhs);
+    (void)hipFree(guess);
+    throw;
+  }
+  float ms = 0;
+  G2G_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipFree(rhs);
+  (void)hipFree(guess);
+  if (info) {
+    info->outer_iterations = r.iterations;
+    info->status = r.stat;
+    info->arnoldi_residual = r.bea;
+    info->true_residual = r.be;
+    info->matvecs_fp64 = op.cnt.matvecs - m0;
+    info->matvecs_fp32 = op.cnt.matvecs32 - m32;
+    info->precond_calls = r.precond_calls;
+    info->precond_fallbacks = r.precond_fallbacks;
+    info->seconds = ms * 1e-3;
+  }
+  if (pp && pp->verbose) {
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "GFGMRES status %d after %d outer iterations, Arnoldi %.3e, true %.3e",
+                  r.stat, r.iterations, r.bea, r.be);
+    log_msg(buf);
+  }
+}
+
+}  // namespace
+
+// ====================================================================== //
+extern "C" {
+
+const char* giem2g_last_error(void) { return g_last_error.c_str(); }
+
+int giem2g_op_create(int Nx, int Ny, int Nz, double dx, double dy, const double* z, void** out) {
+  return guard([&] {
+    G2G_CHECK(out && z, -2, "null argument");
+    auto h = std::make_unique<Handle>();
+    op_init(h->op, Nx, Ny, Nz, dx, dy, z);
+    *out = h.release();
+  });
+}
+
+int giem2g_op_destroy(void* p) {
+  return guard([&] {
+    Handle* h = H(p);
+    op_fini(h->op);
+    delete h;
+  });
+}
+
+int giem2g_set_component_order(void* p, int order) {
+  return guard([&] {
+    Operator& op = H(p)->op;
+    G2G_CHECK(order == GIEM2G_ORDER_XYZ || order == GIEM2G_ORDER_YXZ, -3, "bad component order");
+    op.comp_slot[0] = order == GIEM2G_ORDER_XYZ ? 0 : 1;
+    op.comp_slot[1] = order == GIEM2G_ORDER_XYZ ? 1 : 0;
+    op.comp_slot[2] = 2;
+  });
+}
+
+int giem2g_op_set_background(void* p, int Nl, const double* csigb, const double* thick, double omega) {
+  return guard([&] { set_background(H(p)->op, Nl, csigb, thick, omega); });
+}
+
+int giem2g_op_build_tensor(void* p) {
+  return guard([&] { build_tensor(H(p)->op, -1); });
+}
+
+int giem2g_op_tensor_stage(void* p, int kind) {
+  return guard([&] {
+    G2G_CHECK(kind == 0 || kind == 1, -3, "kind must be 0 or 1");
+    build_tensor(H(p)->op, kind);
+  });
+}
+
+int giem2g_op_get_tensor(void* p, double* G) {
+  return guard([&] {
+    Operator& op = H(p)->op;
+    G2G_HIP(hipMemcpy(G, op.d64.G, sizeof(zd) * op.Nx * op.Ny * op.BS, hipMemcpyDeviceToHost));
+  });
+}
+
+int giem2g_op_put_tensor(void* p, const double* G, int spectral) {
+  return guard([&] {
+    Operator& op = H(p)->op;
+    G2G_HIP(hipMemcpy(op.d64.G, G, sizeof(zd) * op.Nx * op.Ny * op.BS, hipMemcpyHostToDevice));
+    op.have_tensor = spectral != 0;
+    op.have_g32 = false;
+    if (op.have_tensor && op.d32.G) op_make_g32(op);
+  });
+}
+
+int giem2g_op_set_sigma(void* p, const double* csiga) {
+  return guard([&] { set_sigma_host(H(p)->op, csiga); });
+}
+
+int giem2g_op_set_sigma_dev(void* p, const void* csiga) {
+  return guard([&] { set_sigma_dev(H(p)->op, csiga); });
+}
+
+int giem2g_op_apply_dev(void* p, const void* in, void* out) {
+  return guard([&] {
+    Operator& op = H(p)->op;
+    op_apply<double>(op, op.d64, static_cast<const zd*>(in), static_cast<zd*>(out), op.stream,
+                     false);
+    op.cnt.matvecs++;
+  });
+}
+
+int giem2g_op_apply_dev_f32(void* p, const void* in, void* out) {
+  return guard([&] {
+    Operator& op = H(p)->op;
+    if (!op.have_g32) op_make_g32(op);
+    op_apply<float>(op, op.d32, static_cast<const zf*>(in), static_cast<zf*>(out), op.stream,
+                    false);
+    op.cnt.matvecs32++;
+  });
+}
+
+int giem2g_op_apply_host(void* p, const double* in, double* out) {
+  return guard([&] { apply_host(H(p)->op, in, out); });
+}
+
+int giem2g_op_stream(void* p, void** s) {
+  return guard([&] { *s = (void*)H(p)->op.stream; });
+}
+
+int giem2g_op_sync(void* p) {
+  return guard([&] { G2G_HIP(hipStreamSynchronize(H(p)->op.stream)); });
+}
+
+int giem2g_op_stats(void* p, double* o) {
+  return guard([&] {
+    Operator& op = H(p)->op;
+    const Counters& c = op.cnt;
+    const double v[11] = {(double)c.matvecs, (double)c.matvecs32, (double)c.dotprods, c.t_apply,
+                          c.t_fwd, c.t_contr, c.t_bwd, c.t_dot, c.t_tensor, c.t_tensor_fft,
+                          (double)(op.d64.bytes + op.d32.bytes)};
+    std::memcpy(o, v, sizeof v);
+  });
+}
+
+int giem2g_op_time_apply(void* p, int precision, const void* in, void* out, int reps, double* ms3) {
+  return guard([&] {
+    Operator& op = H(p)->op;
+    G2G_CHECK(reps >= 1, -3, "reps must be >= 1");
+    if (precision == 1 && !op.have_g32) op_make_g32(op);
+    double acc[3] = {0, 0, 0};
+    for (int r = 0; r < reps; r++) {
+      if (precision == 0)
+        op_apply<double>(op, op.d64, (const zd*)in, (zd*)out, op.stream, true);
+      else
+        op_apply<float>(op, op.d32, (const zf*)in, (zf*)out, op.stream, true);
+      G2G_HIP(hipEventSynchronize(op.ev[3]));
+      float t[3];
+      for (int k = 0; k < 3; k++) G2G_HIP(hipEventElapsedTime(&t[k], op.ev[k], op.ev[k + 1]));
+      for (int k = 0; k < 3; k++) acc[k] += t[k];
+    }
+    for (int k = 0; k < 3; k++) ms3[k] = acc[k] / reps;
+    op.cnt.t_fwd += acc[0] * 1e-3;
+    op.cnt.t_contr += acc[1] * 1e-3;
+    op.cnt.t_bwd += acc[2] * 1e-3;
+    op.cnt.t_apply += (acc[0] + acc[1] + acc[2]) * 1e-3;
+    if (precision == 0) op.cnt.matvecs += reps;
+    else op.cnt.matvecs32 += reps;
+  });
+}
+
+int giem2g_plane_wave(void* p, int pol, double* E) {
+  return guard([&] { plane_wave(H(p)->op, pol, E); });
+}
+
+int giem2g_solve_dev(void* p, const void* Ebkg, const void* E0, const giem2g_solver_params* prm,
+                     void* Esol, giem2g_solve_info* info) {
+  return guard([&] {
+    solve_dev(*H(p), (const zd*)Ebkg, (const zd*)E0, prm, (zd*)Esol, info);
+  });
+}
+
+int giem2g_solve(void* p, const double* Ebkg, const double* E0, const giem2g_solver_params* prm,
+                 double* Esol, giem2g_solve_info* info) {
+  return guard([&] {
+    Handle* h = H(p);
+    const size_t n = h->op.nvec();
+    zd *a = nullptr, *b = nullptr, *c = nullptr;
+    G2G_HIP(hipMalloc(&a, sizeof(zd) * n));
+    G2G_HIP(hipMalloc(&c, sizeof(zd) * n));
+    if (E0) G2G_HIP(hipMalloc(&b, sizeof(zd) * n));
+    G2G_HIP(hipMemcpy(a, Ebkg, sizeof(zd) * n, hipMemcpyHostToDevice));
+    if (E0) G2G_HIP(hipMemcpy(b, E0, sizeof(zd) * n, hipMemcpyHostToDevice));
+    try {
+      solve_dev(*h, a, b, prm, c, info);
+      G2G_HIP(hipMemcpy(Esol, c, sizeof(zd) * n, hipMemcpyDeviceToHost));
+    } catch (...) {
+      (void)hipFree(a); (void)hipFree(b); (void)hipFree(c);
+      throw;
+    }
+    (void)hipFree(a);
+    (void)hipFree(b);
+    (void)hipFree(c);
+  });
+}
+
+int giem2g_last_history(void* p, double* out, int maxlen, int* len) {
+  return guard([&] {
+    Handle* h = H(p);
+    const int n = (int)h->hist.size();
+    for (int i = 0; i < n && i < maxlen; i++) out[i] = h->hist[i];
+    *len = n;
+  });
+}
+
+// ------------------------------------------------- reference-compatible ---
+void giem2g_calc_data_sizes(GIEM2G_ANOMALY_TYPE a, GIEM2G_DATA_TYPE* data) {
+  const int rc = guard([&] {
+    G2G_CHECK(data, -2, "null data");
+    auto h = std::make_unique<Handle>();
+    const long long Nx = a.Nx, Ny = a.Ny, Nz = a.Nz;
+    data->tensor_size = (long long)sizeof(Handle);
+    data->ie_kernel_buffer_length = Nx * Ny * 2 * Nz * (2 * Nz + 1);
+    data->fft_buffers_length = 4 * Nx * Ny * 3 * Nz;
+    data->giem2g_tensor = h.release();
+  });
+  if (rc) log_msg(std::string("giem2g_calc_data_sizes: ") + g_last_error);
+}
+
+void giem2g_prepare_ie_kernel(GIEM2G_ANOMALY_TYPE a, GIEM2G_DATA_TYPE data) {
+  const int rc = guard([&] {
+    Handle* h = H(data.giem2g_tensor);
+    op_init(h->op, a.Nx, a.Ny, a.Nz, a.dx, a.dy, static_cast<const double*>(a.z));
+  });
+  if (rc) log_msg(std::string("giem2g_prepare_ie_kernel: ") + g_last_error);
+}
+
+void giem2g_calc_ie_kernel(void* p, GIEM2G_BACKGROUND_TYPE bkg, GIEM2G_ANOMALY_TYPE a, double omega) {
+  (void)a;
+  const int rc = guard([&] {
+    Operator& op = H(p)->op;
+    set_background(op, bkg.Nl, static_cast<const double*>(bkg.csigb),
+                   static_cast<const double*>(bkg.thickness), omega);
+    build_tensor(op, 0);
+  });
+  if (rc) log_msg(std::string("giem2g_calc_ie_kernel: ") + g_last_error);
+}
+
+void giem2g_calc_fft_of_ie_kernel(void* p) {
+  const int rc = guard([&] { build_tensor(H(p)->op, 1); });
+  if (rc) log_msg(std::string("giem2g_calc_fft_of_ie_kernel: ") + g_last_error);
+}
+
+void giem2g_set_anomaly_conductivity(void* p, void* csiga) {
+  const int rc = guard([&] { set_sigma_host(H(p)->op, static_cast<const double*>(csiga)); });
+  if (rc) log_msg(std::string("giem2g_set_anomaly_conductivity: ") + g_last_error);
+}
+
+void giem2g_apply_ie_operator(void* p, void* in, void* out) {
+  const int rc = guard([&] { apply_host(H(p)->op, static_cast<const double*>(in), static_cast<double*>(out)); });
+  if (rc) log_msg(std::string("giem2g_apply_ie_operator: ") + g_last_error);
+}
+
+void giem2g_set_logger(void (*logger)(const char*)) { g_logger = logger; }
+
+void giem2g_print_stats(void* p) {
+  const int rc = guard([&] {
+    Operator& op = H(p)->op;
+    Counters& c = op.cnt;
+    const double n = c.matvecs > 0 ? (double)c.matvecs : 1.0;
+    char buf[512];
+    std::snprintf(buf, sizeof buf,
+                  "Time for multiplications: %.3e s | average fwd FFT %.3e s, contraction %.3e s, "
+                  "bwd FFT %.3e s, mult %.3e s | matvecs %lld (+%lld complex64) | tensor %.3e s, "
+                  "tensor FFT %.3e s",
+                  c.t_apply, c.t_fwd / n, c.t_contr / n, c.t_bwd / n, c.t_apply / n,
+                  (long long)c.matvecs, (long long)c.matvecs32, c.t_tensor, c.t_tensor_fft);
+    log_msg(buf);
+    c = Counters();
+  });
+  if (rc) log_msg(std::string("giem2g_print_stats: ") + g_last_error);
+}
+
+void giem2g_delete_ie_operator(void* p) {
+  const int rc = giem2g_op_destroy(p);
+  if (rc) log_msg(std::string("giem2g_delete_ie_operator: ") + g_last_error);
+}
+
+}  // extern "C"

@@ -1,0 +1,560 @@
+// g2g_solver.hip -- device-resident nested FGMRES.
+//
+// Reference path replaced: SolveEquation -> GIEM2G_FGMRES -> SEQ_FGMRES_SOLVE
+// -> GMRES_SOLVE (ie_solver_module.F90:43-258; GFGMRES/gmres.F90:110-549;
+// GFGMRES/seq_fgmres.F90:109-226).  The algorithm is kept step for step --
+// depth-2 nesting (flexible outer FGMRES, one cycle of plain GMRES with
+// x0 = b as the right preconditioner), iterated classical Gram-Schmidt with
+// the 2|w| > |h| test and at most 3 re-orthogonalisations, Givens rotations
+// (reference-BLAS ZROTG/ZROT semantics), restart from the Arnoldi residual,
+// final true residual -- so iteration counts compare with the reference.
+// Krylov vectors live in HBM; the (m+1)x(m+1) Hessenberg matrix and the
+// rotations stay on the host.  The inner solver can run on complex64
+// vectors with the complex64 operator (mixed precision; the outer stays FP64
+// and acts as iterative refinement).
+#include <complex>
+#include <functional>
+#include <vector>
+
+#include "g2g_op.hpp"
+#include "g2g_solver.hpp"
+
+namespace g2g {
+
+using zc = std::complex<double>;
+constexpr int BT = 256;
+constexpr int NBLK = 1024;  // blocks for reductions
+constexpr int KC = 8;       // columns per multi-dot pass
+
+template <typename A> __device__ inline A wave_sum(A v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  return v;
+}
+
+// part[blockIdx.x][2*KC]: partial sums of conj(V[:,k]) . w for k < kc
+template <typename T2, typename A>
+__global__ __launch_bounds__(BT) void k_mdot(const T2* __restrict__ V, int64_t N, int kc,
+                                             const T2* __restrict__ w, A* __restrict__ part) {
+  A re[KC], im[KC];
+#pragma unroll
+  for (int k = 0; k < KC; k++) re[k] = im[k] = 0;
+  for (int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x; i < N; i += (int64_t)gridDim.x * BT) {
+    const T2 wv = w[i];
+#pragma unroll
+    for (int k = 0; k < KC; k++)
+      if (k < kc) {
+        const T2 v = V[(int64_t)k * N + i];
+        re[k] += (A)v.x * (A)wv.x + (A)v.y * (A)wv.y;
+        im[k] += (A)v.x * (A)wv.y - (A)v.y * (A)wv.x;
+      }
+  }
+  __shared__ A red[BT / 64][2 * KC];
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < KC; k++) {
+    A a = wave_sum(re[k]), b = wave_sum(im[k]);
+    if (lane == 0) {
+      red[wid][2 * k] = a;
+      red[wid][2 * k + 1] = b;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * KC) {
+    A s = 0;
+    for (int q = 0; q < BT / 64; q++) s += red[q][threadIdx.x];
+    part[blockIdx.x * 2 * KC + threadIdx.x] = s;
+  }
+}
+
+// out[j] = sum_b part[b][j]; one block, 2*KC outputs (written as double)
+template <typename A>
+__global__ __launch_bounds__(BT) void k_reduce(const A* __restrict__ part, int nb, double* out) {
+  __shared__ double red[BT / 64][2 * KC];
+  double acc[2 * KC];
+  for (int j = 0; j < 2 * KC; j++) acc[j] = 0;
+  for (int b = threadIdx.x; b < nb; b += BT)
+    for (int j = 0; j < 2 * KC; j++) acc[j] += (double)part[b * 2 * KC + j];
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int j = 0; j < 2 * KC; j++) {
+    double a = wave_sum(acc[j]);
+    if (lane == 0) red[wid][j] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * KC) {
+    double s = 0;
+    for (int q = 0; q < BT / 64; q++) s += red[q][threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+
+// w -= sum_k V[:,k] h[k]; part[b][0] = partial |w|^2
+template <typename T2, typename A>
+__global__ __launch_bounds__(BT) void k_update_norm(T2* __restrict__ w, const T2* __restrict__ V,
+                                                    int64_t N, int K, const zd* __restrict__ h,
+                                                    A* __restrict__ part) {
+  __shared__ T2 hs[64];
+  for (int k = threadIdx.x; k < K; k += BT) hs[k] = mk<T2>(h[k].x, h[k].y);
+  __syncthreads();
+  A nrm = 0;
+  for (int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x; i < N; i += (int64_t)gridDim.x * BT) {
+    T2 acc = w[i];
+    for (int k = 0; k < K; k++) {
+      const T2 v = V[(int64_t)k * N + i], hk = hs[k];
+      acc.x -= v.x * hk.x - v.y * hk.y;
+      acc.y -= v.x * hk.y + v.y * hk.x;
+    }
+    w[i] = acc;
+    nrm += (A)acc.x * (A)acc.x + (A)acc.y * (A)acc.y;
+  }
+  __shared__ A red[BT / 64];
+  A a = wave_sum(nrm);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    A s = 0;
+    for (int q = 0; q < BT / 64; q++) s += red[q];
+    part[blockIdx.x * 2 * KC] = s;
+    for (int j = 1; j < 2 * KC; j++) part[blockIdx.x * 2 * KC + j] = 0;
+  }
+}
+
+// x = sum_k B[:,k] y[k] (+ x if accumulate)
+template <typename T2>
+__global__ __launch_bounds__(BT) void k_lincomb(T2* __restrict__ x, const T2* __restrict__ B,
+                                                int64_t N, int K, const zd* __restrict__ y,
+                                                int accumulate) {
+  __shared__ T2 ys[64];
+  for (int k = threadIdx.x; k < K; k += BT) ys[k] = mk<T2>(y[k].x, y[k].y);
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x; i < N; i += (int64_t)gridDim.x * BT) {
+    T2 acc = accumulate ? x[i] : mk<T2>(0, 0);
+    for (int k = 0; k < K; k++) cmac(acc, B[(int64_t)k * N + i], ys[k]);
+    x[i] = acc;
+  }
+}
+
+template <typename T2, typename R>
+__global__ void k_scale(T2* __restrict__ dst, const T2* __restrict__ src, int64_t N, R s) {
+  for (int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x; i < N; i += (int64_t)gridDim.x * BT)
+    dst[i] = src[i] * s;
+}
+template <typename T2> __global__ void k_bminus(T2* __restrict__ r, const T2* __restrict__ b, int64_t N) {
+  for (int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x; i < N; i += (int64_t)gridDim.x * BT)
+    r[i] = b[i] - r[i];
+}
+template <typename T2> __global__ void k_add(T2* __restrict__ x, const T2* __restrict__ a, int64_t N) {
+  for (int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x; i < N; i += (int64_t)gridDim.x * BT)
+    x[i] = x[i] + a[i];
+}
+template <typename Ti, typename To>
+__global__ void k_convert(To* __restrict__ o, const Ti* __restrict__ a, int64_t N) {
+  for (int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x; i < N; i += (int64_t)gridDim.x * BT)
+    o[i] = mk<To>((decltype(To::x))a[i].x, (decltype(To::x))a[i].y);
+}
+// rhs / guess scaling of SolveEquation (ie_solver_module.F90:64-97)
+__global__ void k_rhs_scale(zd* __restrict__ out, const zd* __restrict__ in,
+                            const zd* __restrict__ asiga, const double* __restrict__ sq,
+                            int64_t ncxy, int Nz, int64_t N, int mode) {
+  // mode 0: out = sq*in; 1: out = in/asiga; 2: out = asiga*in
+  for (int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x; i < N; i += (int64_t)gridDim.x * BT) {
+    const int64_t ci = i % (ncxy * Nz);
+    const int z = (int)(ci / ncxy);
+    if (mode == 0) out[i] = in[i] * sq[z];
+    else if (mode == 1) out[i] = cdiv(in[i], asiga[ci]);
+    else out[i] = asiga[ci] * in[i];
+  }
+}
+
+static inline dim3 gridN(int64_t N) {
+  int64_t b = (N + BT - 1) / BT;
+  return dim3((unsigned)(b < 4096 ? b : 4096));
+}
+
+// ---------------------------------------------------------------- BLAS ---
+struct DevBlas {
+  hipStream_t s;
+  void* part = nullptr;  // NBLK * 2KC doubles
+  double* res = nullptr; // 2KC
+  zd* hbuf = nullptr; // 64
+  double hres[2 * KC];
+  int64_t* dotcount = nullptr;
+  void init(hipStream_t st) {
+    s = st;
+    G2G_HIP(hipMalloc(&part, sizeof(double) * NBLK * 2 * KC));
+    G2G_HIP(hipMalloc(&res, sizeof(double) * 2 * KC));
+    G2G_HIP(hipMalloc(&hbuf, sizeof(zd) * 64));
+  }
+  void fini() {
+    if (part) (void)hipFree(part);
+    if (res) (void)hipFree(res);
+    if (hbuf) (void)hipFree(hbuf);
+    part = nullptr;
+  }
+  // res[k] = V[:,k]^H w for k < K
+  template <typename T2, typename A>
+  void mdot(const T2* V, int64_t N, int K, const T2* w, zc* out) {
+    for (int k0 = 0; k0 < K; k0 += KC) {
+      const int kc = K - k0 < KC ? K - k0 : KC;
+      const int nb = (int)std::min<int64_t>(NBLK, (N + BT - 1) / BT);
+      hipLaunchKernelGGL((k_mdot<T2, A>), dim3(nb), dim3(BT), 0, s, V + (int64_t)k0 * N, N, kc, w,
+                         (A*)part);
+      hipLaunchKernelGGL((k_reduce<A>), dim3(1), dim3(BT), 0, s, (const A*)part, nb, res);
+      G2G_HIP(hipMemcpyAsync(hres, res, sizeof(double) * 2 * KC, hipMemcpyDeviceToHost, s));
+      G2G_HIP(hipStreamSynchronize(s));
+      for (int k = 0; k < kc; k++) out[k0 + k] = zc(hres[2 * k], hres[2 * k + 1]);
+    }
+    if (dotcount) *dotcount += K;
+  }
+  template <typename T2, typename A> double norm(const T2* v, int64_t N) {
+    zc t;
+    mdot<T2, A>(v, N, 1, v, &t);
+    return std::sqrt(t.real());
+  }
+  // w -= V h, returns |w|
+  template <typename T2, typename A>
+  double update_norm(T2* w, const T2* V, int64_t N, int K, const zc* h) {
+    G2G_HIP(hipMemcpyAsync(hbuf, h, sizeof(zd) * K, hipMemcpyHostToDevice, s));
+    const int nb = (int)std::min<int64_t>(NBLK, (N + BT - 1) / BT);
+    hipLaunchKernelGGL((k_update_norm<T2, A>), dim3(nb), dim3(BT), 0, s, w, V, N, K, hbuf, (A*)part);
+    hipLaunchKernelGGL((k_reduce<A>), dim3(1), dim3(BT), 0, s, (const A*)part, nb, res);
+    G2G_HIP(hipMemcpyAsync(hres, res, sizeof(double) * 2 * KC, hipMemcpyDeviceToHost, s));
+    G2G_HIP(hipStreamSynchronize(s));
+    if (dotcount) *dotcount += 1;
+    return std::sqrt(hres[0]);
+  }
+  template <typename T2>
+  void lincomb(T2* x, const T2* B, int64_t N, int K, const zc* y, bool acc) {
+    G2G_HIP(hipMemcpyAsync(hbuf, y, sizeof(zd) * K, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL((k_lincomb<T2>), gridN(N), dim3(BT), 0, s, x, B, N, K, hbuf, acc ? 1 : 0);
+  }
+  template <typename T2, typename R> void scale(T2* d, const T2* src, int64_t N, R f) {
+    hipLaunchKernelGGL((k_scale<T2, R>), gridN(N), dim3(BT), 0, s, d, src, N, f);
+  }
+  template <typename T2> void copy(T2* d, const T2* src, int64_t N) {
+    G2G_HIP(hipMemcpyAsync(d, src, sizeof(T2) * N, hipMemcpyDeviceToDevice, s));
+  }
+  template <typename T2> void bminus(T2* r, const T2* b, int64_t N) {
+    hipLaunchKernelGGL((k_bminus<T2>), gridN(N), dim3(BT), 0, s, r, b, N);
+  }
+  template <typename T2> void add(T2* x, const T2* a, int64_t N) {
+    hipLaunchKernelGGL((k_add<T2>), gridN(N), dim3(BT), 0, s, x, a, N);
+  }
+  template <typename Ti, typename To> void convert(To* o, const Ti* a, int64_t N) {
+    hipLaunchKernelGGL((k_convert<Ti, To>), gridN(N), dim3(BT), 0, s, o, a, N);
+  }
+};
+
+// reference BLAS ZROTG / ZROT
+static void zrotg(zc& ca, zc cb, double& c, zc& sn) {
+  if (std::abs(ca) == 0.0) {
+    c = 0.0;
+    sn = 1.0;
+    ca = cb;
+  } else {
+    const double scale = std::abs(ca) + std::abs(cb);
+    const double a1 = std::abs(ca / scale), b1 = std::abs(cb / scale);
+    const double nrm = scale * std::sqrt(a1 * a1 + b1 * b1);
+    const zc alpha = ca / std::abs(ca);
+    c = std::abs(ca) / nrm;
+    sn = alpha * std::conj(cb) / nrm;
+    ca = alpha * nrm;
+  }
+}
+static void zrot(zc& cx, zc& cy, double c, zc s) {
+  const zc t = c * cx + s * cy;
+  cy = c * cy - std::conj(s) * cx;
+  cx = t;
+}
+
+// ------------------------------------------------------------- GMRES ---
+template <typename R> struct Gmres {
+  using T2 = cx_t<R>;
+  using Apply = std::function<void(const T2*, T2*)>;
+  int64_t N = 0;
+  int M = 0;
+  bool flex = false;
+  int maxit = 1;
+  double tol = 1e-8;
+  T2 *x = nullptr, *r0 = nullptr, *w = nullptr, *V = nullptr, *Z = nullptr;
+  std::vector<zc> H, gs, y, dp;
+  std::vector<double> gc;
+  int iternum = 0;
+  Apply A;
+  Apply precond;  // right preconditioner (flex only); identity when empty
+  DevBlas* blas = nullptr;
+  int64_t* matvecs = nullptr;
+  std::vector<double>* hist = nullptr;
+  size_t bytes = 0;
+
+  zc& HH(int i, int j) { return H[(size_t)i + (size_t)(M + 1) * j]; }
+
+  void alloc(int64_t n, int m, bool f) {
+    N = n;
+    M = m;
+    flex = f;
+    auto Al = [&](T2** p, int64_t cnt) {
+      G2G_HIP(hipMalloc(p, sizeof(T2) * cnt));
+      bytes += sizeof(T2) * cnt;
+    };
+    Al(&x, N);
+    Al(&r0, N);
+    Al(&w, N);
+    Al(&V, N * (M + 1));
+    if (flex) Al(&Z, N * M);
+    H.assign((size_t)(M + 1) * (M + 1), 0);
+    gs.assign(M + 1, 0);
+    gc.assign(M + 1, 0);
+    y.assign(M + 1, 0);
+    dp.assign(M + 1, 0);
+  }
+  void fini() {
+    for (T2** p : {&x, &r0, &w, &V, &Z})
+      if (*p) {
+        (void)hipFree(*p);
+        *p = nullptr;
+      }
+  }
+  void apply(const T2* in, T2* out) {
+    A(in, out);
+    if (matvecs) (*matvecs)++;
+  }
+  double norm(const T2* v) { return blas->norm<T2, R>(v, N); }
+  T2* Vc(int k) { return V + (int64_t)k * N; }
+
+  // ApplyRightPreconditioner (gmres.F90:491-506)
+  void right_precond(const T2* vin, T2* vout, int K) {
+    if (precond) precond(vin, vout);
+    else blas->copy(vout, vin, N);
+    if (flex) blas->copy(Z + (int64_t)K * N, vout, N);
+  }
+  // ConstructCurrentSolution (:423-458)
+  void construct(int jH, T2* solution) {
+    const int L = jH - 1;
+    for (int i = 0; i < L; i++) y[i] = HH(i, M);
+    for (int i = L - 1; i >= 0; i--) {
+      zc t = y[i];
+      for (int j = i + 1; j < L; j++) t -= HH(i, j) * y[j];
+      y[i] = t / HH(i, i);
+    }
+    if (L > 0) {
+      blas->lincomb(x, flex ? Z : V, N, L, y.data(), false);
+      // non-flexible: right preconditioner is the identity here
+      blas->add(solution, x, N);
+    }
+    blas->copy(x, solution, N);
+  }
+  // FinalActions (:382-421)
+  double final_actions(int jH, double sPb, const T2* rhs, T2* solution, int* stat) {
+    construct(jH, solution);
+    blas->copy(w, solution, N);
+    apply(w, r0);
+    blas->bminus(r0, rhs, N);
+    const double tr = norm(r0);
+    const double res = tr / sPb;
+    *stat = res >= tol ? 2 : 1;
+    return res;
+  }
+  // KrylovSubspaceConstruction (:216-329)
+  double krylov(double bn, int* jHp) {
+    int jH = *jHp;
+    double bea = -1;
+    for (;;) {
+      const int j = jH - 1;
+      right_precond(Vc(j), w, j);
+      apply(w, r0);
+      std::swap(w, r0);
+      for (int i = 0; i < jH; i++) HH(i, j) = 0;
+      int northo = 0;
+      double dnormw = 0;
+      for (;;) {
+        northo++;
+        blas->mdot<T2, R>(V, N, jH, w, dp.data());
+        for (int i = 0; i < jH; i++) HH(i, j) += dp[i];
+        dnormw = blas->update_norm<T2, R>(w, V, N, jH, dp.data());
+        double dloo = 0;
+        for (int i = 0; i < jH; i++) dloo += std::norm(dp[i]);
+        dloo = std::sqrt(dloo);
+        if ((2.0 * dnormw > dloo) || (northo > 3)) break;
+      }
+      HH(jH, j) = dnormw;
+      blas->scale(Vc(jH), w, N, (R)(1.0 / dnormw));
+      for (int i = 0; i < j; i++) zrot(HH(i, j), HH(i + 1, j), gc[i], gs[i]);
+      zc h1 = HH(j, j), h2 = HH(j + 1, j);
+      zrotg(h1, h2, gc[j], gs[j]);
+      zrot(HH(j, M), HH(j + 1, M), gc[j], gs[j]);
+      zrot(HH(j, j), HH(j + 1, j), gc[j], gs[j]);
+      HH(j + 1, j) = 0;
+      bea = std::abs(HH(j + 1, M)) / bn;
+      iternum++;
+      if (hist) hist->push_back(bea);
+      jH++;
+      if (bea < tol || jH > M) break;
+    }
+    *jHp = jH;
+    return bea;
+  }
+  // PrepareRestart (:333-379), iterative residual
+  double restart(T2* solution, int jH) {
+    construct(jH, solution);
+    const double beta = std::abs(HH(M, M));
+    for (int J = M - 1; J >= 0; J--) {
+      HH(J, M) = 0;
+      zrot(HH(J, M), HH(J + 1, M), gc[J], -gs[J]);
+    }
+    std::vector<zc> col(M + 1);
+    for (int J = 0; J <= M; J++) col[J] = HH(J, M);
+    blas->lincomb(w, V, N, M + 1, col.data(), false);
+    return beta;
+  }
+  // GMRES_SOLVE (:110-213)
+  SolveResult solve(T2* xo, const T2* x0, const T2* b) {
+    SolveResult r;
+    std::fill(H.begin(), H.end(), zc(0));
+    iternum = 0;
+    const double bn = norm(b);
+    if (bn < 1e-14) {
+      G2G_HIP(hipMemsetAsync(xo, 0, sizeof(T2) * N, blas->s));
+      r.stat = 1;
+      return r;
+    }
+    const double sPb = bn;
+    blas->copy(xo, x0, N);
+    apply(x0, r0);
+    blas->bminus(r0, b, N);
+    blas->copy(w, r0, N);
+    double beta = norm(w);
+    if (beta / bn < tol) {
+      r.stat = 1;
+      r.be = r.bea = beta / bn;
+      return r;
+    }
+    int iter = 0, jH = 1, stat = -1;
+    double bea = -1;
+    for (;;) {
+      HH(0, M) = beta;
+      blas->scale(V, w, N, (R)(1.0 / beta));
+      for (int i = 0; i < M; i++) HH(i + 1, M) = 0;
+      jH = 1;
+      bea = krylov(bn, &jH);
+      if (bea < tol) {
+        r.be = final_actions(jH, sPb, b, xo, &stat);
+        r.stat = stat;
+        r.iterations = iternum;
+        r.bea = bea;
+        return r;
+      }
+      iter++;
+      if (iter >= maxit) break;
+      beta = restart(xo, jH);
+    }
+    r.be = final_actions(jH, sPb, b, xo, &stat);
+    r.bea = bea;
+    r.iterations = iternum;
+    r.stat = -1;
+    return r;
+  }
+};
+
+// ------------------------------------------------------- nested solve ---
+SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
+                         const SolveParams& p, std::vector<double>* hist,
+                         const PrecondHook* learned) {
+  const int64_t N = op.nvec();
+  DevBlas blas;
+  blas.init(op.stream);
+  blas.dotcount = &op.cnt.dotprods;
+  Gmres<double> outer;
+  outer.blas = &blas;
+  outer.maxit = p.max_restart;
+  outer.tol = p.tol;
+  outer.alloc(N, p.outer_m, true);
+  outer.matvecs = &op.cnt.matvecs;
+  outer.hist = hist;
+  outer.A = [&](const zd* in, zd* out) { op_apply<double>(op, op.d64, in, out, op.stream, false); };
+  Gmres<double> in64;
+  Gmres<float> in32;
+  zf *b32 = nullptr, *x32 = nullptr;
+  zd* tmp = nullptr;
+  SolveResult res;
+  try {
+    if (p.inner_precision == 0) {
+      in64.blas = &blas;
+      in64.maxit = 1;
+      in64.tol = p.tol;
+      in64.alloc(N, p.inner_m, false);
+      in64.matvecs = &op.cnt.matvecs;
+      in64.A = outer.A;
+    } else {
+      if (!op.have_g32) op_make_g32(op);
+      in32.blas = &blas;
+      in32.maxit = 1;
+      in32.tol = p.tol;
+      in32.alloc(N, p.inner_m, false);
+      in32.matvecs = &op.cnt.matvecs32;
+      in32.A = [&](const zf* in, zf* out) { op_apply<float>(op, op.d32, in, out, op.stream, false); };
+      G2G_HIP(hipMalloc(&b32, sizeof(zf) * N));
+      G2G_HIP(hipMalloc(&x32, sizeof(zf) * N));
+    }
+    G2G_HIP(hipMalloc(&tmp, sizeof(zd) * N));
+    // FGMRES_BASED_PRECONDITIONER (seq_fgmres.F90:216-226): inner GMRES with x0 = b = v
+    auto inner_gmres = [&](const zd* v, zd* z) {
+      if (p.inner_precision == 0) {
+        in64.solve(z, v, v);
+      } else {
+        blas.convert<zd, zf>(b32, v, N);
+        in32.solve(x32, b32, b32);
+        blas.convert<zf, zd>(z, x32, N);
+      }
+    };
+    if (learned && learned->apply && p.precond == 1) {
+      // learned correction c = M(v); accept z = c + inner GMRES on the
+      // remaining residual only when it cuts ||v - A c||, else plain inner.
+      outer.precond = [&](const zd* v, zd* z) {
+        res.precond_calls++;
+        learned->apply(learned->ctx, v, z);
+        op_apply<double>(op, op.d64, z, tmp, op.stream, false);
+        op.cnt.matvecs++;
+        blas.bminus(tmp, v, N);  // tmp = v - A c
+        const double rn = blas.norm<zd, double>(tmp, N), vn = blas.norm<zd, double>(v, N);
+        if (!(rn < learned->accept * vn)) {
+          res.precond_fallbacks++;
+          inner_gmres(v, z);
+          return;
+        }
+        zd* c = z;
+        zd* zz;
+        G2G_HIP(hipMalloc(&zz, sizeof(zd) * N));
+        inner_gmres(tmp, zz);
+        blas.add(c, zz, N);
+        (void)hipFree(zz);
+      };
+    } else {
+      outer.precond = inner_gmres;
+    }
+    SolveResult r = outer.solve(xout, x0, b);
+    r.precond_calls = res.precond_calls;
+    r.precond_fallbacks = res.precond_fallbacks;
+    res = r;
+  } catch (...) {
+    outer.fini(); in64.fini(); in32.fini(); blas.fini();
+    if (b32) (void)hipFree(b32);
+    if (x32) (void)hipFree(x32);
+    if (tmp) (void)hipFree(tmp);
+    throw;
+  }
+  G2G_HIP(hipStreamSynchronize(op.stream));
+  outer.fini(); in64.fini(); in32.fini(); blas.fini();
+  if (b32) (void)hipFree(b32);
+  if (x32) (void)hipFree(x32);
+  if (tmp) (void)hipFree(tmp);
+  return res;
+}
+
+void solve_scale(Operator& op, zd* out, const zd* in, int mode) {
+  const int64_t N = op.nvec();
+  hipLaunchKernelGGL(k_rhs_scale, gridN(N), dim3(BT), 0, op.stream, out, in, op.d64.asiga, op.d64.sq,
+                     (int64_t)op.Nx * op.Ny, op.Nz, N, mode);
+  G2G_HIP(hipGetLastError());
+}
+
+}  // namespace g2g

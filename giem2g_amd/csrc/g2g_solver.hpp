@@ -1,0 +1,35 @@
+// g2g_solver.hpp -- nested FGMRES interface (see g2g_solver.hip).
+#pragma once
+#include <vector>
+
+#include "g2g_op.hpp"
+
+namespace g2g {
+
+struct SolveParams {
+  int outer_m = 30, inner_m = 30, max_restart = 100;
+  double tol = 1e-8;
+  int inner_precision = 0;  // 0 FP64, 1 complex64
+  int precond = 0;          // 0 inner GMRES, 1 learned + inner GMRES fallback
+};
+
+struct SolveResult {
+  int iterations = 0;
+  int stat = -1;
+  double bea = -1, be = -1;
+  long long precond_calls = 0, precond_fallbacks = 0;
+};
+
+// A learned right preconditioner: z = M(v) on HBM vectors (complex128).
+struct PrecondHook {
+  void (*apply)(void* ctx, const zd* v, zd* z) = nullptr;
+  void* ctx = nullptr;
+  double accept = 1.0;  // accept the learned step when ||v - A M(v)|| < accept*||v||
+};
+
+SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
+                         const SolveParams& p, std::vector<double>* hist, const PrecondHook* learned);
+// mode 0: out = sqrt(Re sb)*in (rhs); 1: out = in/asiga (warm start); 2: out = asiga*in (field)
+void solve_scale(Operator& op, zd* out, const zd* in, int mode);
+
+}  // namespace g2g

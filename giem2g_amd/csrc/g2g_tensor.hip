@@ -1,0 +1,726 @@
+// g2g_tensor.hip -- FP64 Green-tensor build and tensor spectrum on the GPU.
+//
+// Reference path replaced: CalcIntegralGreenTensor ->
+// CalcIntegralGreenTensor3dElementGeneral -> {COMPUTE_WEIGHTS ->
+// VBTransformWeightsAllDoubleInt, Calc_Double_Integral_U}
+// (calc_ie_tensor_module.F90:54-352, VolumeBesselTransforms.F90:89-403,
+// ie_kernel_hankel_module.F90:44-197, apq_module.F90:31-312) and
+// CalcFFTofIETensorGeneral (Integral_Equation_Module.F90:319-471).
+//
+// The reference evaluates the layered vertical kernel U(lambda_K, W) once per
+// (offset, tap).  U is linear in the six filter weights with coefficients
+// that depend on lambda only, so here:
+//   1. k_weights   : one block per lateral offset computes the 501x6 digital
+//                    filter weights (1024-point FFT deconvolution in LDS),
+//                    the 1e-12 tap mask and the lambda scaling;  these are
+//                    frequency independent and cached;
+//   2. k_lambda    : one block per tap K evaluates the weight-free kernel
+//                    tables a,b,c (packed symmetric) and d (full) of
+//                    Calc_Double_Integral_U;
+//   3. k_wgemm     : G(offset, entry) = sum_K w(offset,K) * table(K, entry),
+//                    one lane per (offset, z, z') output -- an FP64
+//                    (NxNy x 501) x (501 x Nz^2) product per component;
+//   4. k_tfft_y/x  : symmetric extension with per-component signs, 2-D FFT,
+//                    Nyquist cancellation, in place.
+#include "g2g_fft.hpp"
+#include "g2g_op.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace g2g {
+
+constexpr int NTAPS = 501, NFIRST = 212, VBN = 1024;
+constexpr double VB_YSTEP = 1e-1 * 2e0;
+constexpr double VB_XI = -4.82205425726955430357775925664462079e-2 * 2e0;
+constexpr double VB_Q = VB_YSTEP / 2.0;
+constexpr double VB_P = VB_Q + VB_XI;
+constexpr double PI_D = 3.1415926535897932384626433832795028841971693993751;
+enum { IE_DXX = 0, IE_DXY = 1, IE_DYY = 2, IE_DX = 3, IE_DY = 4, IE_D0 = 5 };
+
+// ------------------------------------------------------------ weights ---
+__device__ inline double vb_inputfunction(double t) {
+  const double y = exp(-t), y2 = y * y, y4 = y2 * y2;
+  return 8.0 * exp(-y2) * (y4 - 4.0 * y2 + 2.0) * y;
+}
+
+// spectrum of the input function (VBTransformInit, VolumeBesselTransforms.F90:89-109)
+__global__ __launch_bounds__(256) void k_f1(zd* f1out, const zd* twg) {
+  __shared__ zd buf[VBN + 1];
+  __shared__ zd tw[VBN];
+  for (int i = threadIdx.x; i < VBN; i += 256) tw[i] = twg[i];
+  for (int i = threadIdx.x; i < VBN; i += 256) {
+    const int i0 = i & ~1;
+    double y1 = (double)(i0 - 512) * VB_YSTEP + VB_Q;
+    if (i & 1) y1 = y1 + VB_YSTEP;
+    const double v = vb_inputfunction(y1);
+    buf[i] = mkz((i & 1) ? -v : v, 0.0);
+  }
+  __syncthreads();
+  lds_fft<zd, VBN, 1, VBN + 1, 256, +1>(buf, tw);
+  for (int i = threadIdx.x; i < VBN; i += 256) f1out[i] = buf[i];
+}
+
+struct WParams {
+  int Nx, Ny;
+  double dx, dy, rmin, log_rmin;
+};
+
+// outfunc_int4_all for knot I (VolumeBesselTransforms.F90:304-550)
+__device__ inline void outfunc4(double explt, double cphi, double sphi, double alpha, double beta,
+                                double* R) {
+  double f[2][3][3];  // [x/y][D0,D1,D2][1..3]
+#pragma unroll
+  for (int J = 0; J < 2; J++) {
+    const double c = J == 0 ? cphi : sphi, h = J == 0 ? alpha : beta;
+    double x1 = explt * c;
+    const double dd = h * explt;
+    x1 = x1 - dd * 0.5;
+    double x[3] = {x1, x1 + dd, x1 - dd}, x2[3], ex[3], er[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      x[k] = x[k] * 0.5;
+      x2[k] = x[k] * x[k];
+      ex[k] = exp(-x2[k]);
+      er[k] = erf(x[k]);
+    }
+#define D2(a) ((a)[1] + (a)[2] - 2.0 * (a)[0])
+    double t[3], r0, r1, r2;
+    // DOUBLE_INT_D0 via INT_2d_edt
+    for (int k = 0; k < 3; k++) t[k] = x[k] * er[k];
+    r0 = D2(t) * sqrt(PI_D) * 0.25;
+    r1 = D2(ex);
+    for (int k = 0; k < 3; k++) t[k] = ex[k] * x2[k] / 4.0;
+    r2 = D2(t);
+    f[J][0][0] = 32.0 * (1.5 * r0 + r1 + r2);
+    f[J][0][1] = 8.0 * r0 + 4.0 * r1;
+    f[J][0][2] = 4.0 * r0 + r1;
+    // DOUBLE_INT_D1 via INT_1d_edt
+    r0 = D2(er) * sqrt(PI_D) * 0.5;
+    for (int k = 0; k < 3; k++) t[k] = ex[k] * x[k];
+    r1 = D2(t);
+    for (int k = 0; k < 3; k++) t[k] = t[k] * x2[k];
+    r2 = D2(t);
+    f[J][1][0] = -8.0 * r2 - 12.0 * r1 + 12.0 * r0;
+    f[J][1][1] = -2.0 * r1 + 2.0 * r0;
+    f[J][1][2] = r0;
+    // DOUBLE_INT_D2
+    double e2[3];
+    for (int k = 0; k < 3; k++) e2[k] = ex[k] / 2.0;
+    for (int k = 0; k < 3; k++) t[k] = x2[k] * e2[k] * 4.0;
+    f[J][2][1] = D2(t);
+    for (int k = 0; k < 3; k++) t[k] = x2[k] * t[k] * 4.0;
+    f[J][2][0] = D2(t);
+    f[J][2][2] = D2(e2);
+#undef D2
+  }
+#define CMB(a, b) (f[0][a][2] * f[1][b][0] + f[0][a][0] * f[1][b][2] + 2.0 * f[0][a][1] * f[1][b][1])
+  const double e2 = explt * explt, e3 = e2 * explt;
+  R[0] = CMB(0, 0) / e3;
+  R[1] = CMB(2, 0) / explt;
+  R[2] = CMB(1, 1) / explt;
+  R[3] = CMB(1, 0) / e2;
+  R[4] = CMB(0, 2) / explt;
+  R[5] = CMB(0, 1) / e2;
+#undef CMB
+}
+
+__device__ inline double block_max(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  double r = red[0];
+  for (int i = 1; i < (int)(blockDim.x >> 6); i++) r = fmax(r, red[i]);
+  return r;
+}
+
+// One block per offset b = lx + Nx*ly.  Output wts[c][K][b] (IE order).
+__global__ __launch_bounds__(256) void k_weights(double* __restrict__ wts,
+                                                 const zd* __restrict__ f1,
+                                                 const zd* __restrict__ twg,
+                                                 const double* __restrict__ lms, WParams P) {
+  constexpr int NRW = 3, LS = VBN + 1;
+  __shared__ zd buf[NRW * LS];
+  __shared__ zd tw[VBN];
+  __shared__ double wt[6][NTAPS];
+  __shared__ double red[8];
+  const int b = blockIdx.x, lx = b % P.Nx, ly = b / P.Nx;
+  const double x = lx * P.dx + P.dx / 2.0, y = ly * P.dy + P.dy / 2.0;
+  const double rho = sqrt(x * x + y * y);
+  const double rp[6] = {rho * rho * rho, rho, rho, rho * rho, rho, rho * rho};
+  const double alpha = P.dx / rho, beta = P.dy / rho, phi = atan2(y, x);
+  const double cphi = cos(phi), sphi = sin(phi);
+  const double s = log(rho) - P.log_rmin;
+  for (int i = threadIdx.x; i < VBN; i += 256) tw[i] = twg[i];
+  for (int half = 0; half < 2; half++) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < VBN; i += 256) {
+      const int i0 = i & ~1;
+      double y2 = (double)(i0 - 512) * VB_YSTEP + VB_P + s;
+      if (i & 1) y2 = y2 + VB_YSTEP;
+      const double explt = exp(y2);
+      double R[6];
+      outfunc4(explt, cphi, sphi, alpha, beta, R);
+      const double sg = (i & 1) ? -1.0 : 1.0;
+      for (int c = 0; c < NRW; c++) buf[c * LS + i] = mkz(sg * R[half * 3 + c], 0.0);
+    }
+    __syncthreads();
+    lds_fft<zd, VBN, NRW, LS, 256, +1>(buf, tw);
+    for (int i = threadIdx.x; i < NRW * VBN; i += 256) {
+      const int c = i / VBN, J = i - c * VBN;
+      const zd h = cdiv(buf[c * LS + J], f1[J]);
+      buf[c * LS + J] = (J & 1) ? -h : h;
+    }
+    __syncthreads();
+    lds_fft<zd, VBN, NRW, LS, 256, -1>(buf, tw);
+    for (int i = threadIdx.x; i < NRW * NTAPS; i += 256) {
+      const int c = i / NTAPS, K = i - c * NTAPS, J = K + NFIRST;
+      double h = buf[c * LS + J].x / (double)VBN;
+      if (J & 1) h = -h;
+      const int col = half * 3 + c;  // reference output order D0,DXX,DXY,DX,DYY,DY
+      h = h / P.dx / P.dy * rp[col];
+      const int ie = col == 0 ? IE_D0 : col == 1 ? IE_DXX : col == 2 ? IE_DXY
+                   : col == 3 ? IE_DX : col == 4 ? IE_DYY : IE_DY;
+      wt[ie][K] = h;
+    }
+  }
+  __syncthreads();
+  // COMPUTE_WEIGHTS W0 logic (calc_ie_tensor_module.F90:327-351)
+  double W0[6];
+  auto colmax = [&](int c) {
+    double m = 0;
+    for (int K = threadIdx.x; K < NTAPS; K += 256) m = fmax(m, fabs(wt[c][K]));
+    return block_max(m, red);
+  };
+  W0[IE_DXX] = colmax(IE_DXX);
+  if (lx != 0 && ly != 0) {
+    double m = fmax(fmax(colmax(IE_DXY), colmax(IE_DYY)), fmax(colmax(IE_DX), colmax(IE_DY)));
+    W0[IE_DXY] = W0[IE_DYY] = W0[IE_DX] = W0[IE_DY] = m;
+  } else if (lx == 0) {
+    W0[IE_DXY] = -W0[IE_DXX];
+    W0[IE_DYY] = colmax(IE_DYY);
+    W0[IE_DX] = -W0[IE_DXX];
+    W0[IE_DY] = (ly != 0) ? colmax(IE_DY) : -W0[IE_DXX];
+  } else {
+    W0[IE_DXY] = -W0[IE_DXX];
+    W0[IE_DYY] = colmax(IE_DYY);
+    W0[IE_DX] = colmax(IE_DX);
+    W0[IE_DY] = -W0[IE_DXX];
+  }
+  W0[IE_D0] = colmax(IE_D0);
+  const int64_t M = (int64_t)P.Nx * P.Ny;
+  for (int K = threadIdx.x; K < NTAPS; K += 256) {
+    double w[6];
+    for (int c = 0; c < 6; c++) w[c] = wt[c][K];
+    if (lx == 0) {  // :330-342
+      w[IE_DXY] = 0;
+      w[IE_DX] = 0;
+      if (ly == 0) w[IE_DY] = 0;
+    } else if (ly == 0) {  // :343-349
+      w[IE_DXY] = 0;
+      w[IE_DY] = 0;
+    }
+    double Wm = 0;
+    for (int c = 0; c < 6; c++) {
+      const double t = fabs(w[c] / W0[c]);
+      if (c == 0 || t > Wm) Wm = t;
+    }
+    const bool on = Wm > 1e-12;
+    const double lm = lms[K];
+    double o[6];
+    o[IE_DXX] = w[IE_DXX] / lm;
+    o[IE_DYY] = w[IE_DYY] / lm;
+    o[IE_DXY] = w[IE_DXY] / lm;
+    o[IE_DX] = w[IE_DX] * lm;
+    o[IE_DY] = w[IE_DY] * lm;
+    o[IE_D0] = w[IE_D0] * lm;
+    for (int c = 0; c < 6; c++) wts[((int64_t)c * NTAPS + K) * M + b] = on ? o[c] : 0.0;
+  }
+}
+
+// ------------------------------------------------------ lambda tables ---
+struct LParams {
+  int Nl, Nz;
+  zd iwm;
+  const zd* bk;  // [csigma 0..Nl | k2 0..Nl | (thick, depth) 0..Nl as zd]
+};
+
+
+__device__ inline zd cz2(double a, double b) { return mkz(a, b); }
+__device__ inline zd cadd(zd a, double b) { return mkz(a.x + b, a.y); }
+__device__ inline zd csqrt_d(zd z) {
+  // principal square root
+  const double r = hypot(z.x, z.y);
+  if (r == 0.0) return mkz(0.0, 0.0);
+  double t = sqrt(0.5 * (r + fabs(z.x)));
+  if (z.x >= 0.0) return mkz(t, z.y / (2.0 * t));
+  return mkz(fabs(z.y) / (2.0 * t), z.y >= 0 ? t : -t);
+}
+__device__ inline zd cexp_d(zd z) {
+  const double e = exp(z.x);
+  double s, c;
+  sincos(z.y, &s, &c);
+  return mkz(e * c, e * s);
+}
+__device__ inline zd operator/(zd a, zd b) { return cdiv(a, b); }
+__device__ inline zd rdiv(double a, zd b) { return cdiv(mkz(a, 0.0), b); }
+
+// tables per tap K: a[P], b[P], c[P] packed upper (i<=j at i + j(j+1)/2), d[Nz*Nz] column-major
+__global__ __launch_bounds__(64) void k_lambda(zd* __restrict__ tabs,
+                                               const double* __restrict__ lms,
+                                               const double* __restrict__ dzg,
+                                               const double* __restrict__ zg,
+                                               const int* __restrict__ lnumg, LParams L) {
+  extern __shared__ unsigned char sm_raw[];
+  const int Nz = L.Nz, Nl = L.Nl;
+  zd* expz = reinterpret_cast<zd*>(sm_raw);  // 1..Nz
+  zd* pez = expz + (Nz + 1);                       // [I][2]
+  zd* qez = pez + 2 * (Nz + 1);
+  zd* fct = qez + 2 * (Nz + 1);  // fcont_t [I][2]
+  zd* tfr = fct + 2 * (Nz + 1);  // t_frame [I][3]
+  zd* fcb = tfr + 3 * (Nz + 1);
+  zd* bfr = fcb + (Nz + 1);
+  zd* Ft = bfr + (Nz + 1);  // [I][3]
+  zd* Fb = Ft + 3 * (Nz + 1);
+  zd* GII = Fb + (Nz + 1);  // [I][4]
+  __shared__ zd eta[65], p[65][2], q[65][2], Arr[65][2];
+  __shared__ zd sh_cs[65], sh_k2[65], sh_td[65];
+  for (int i = threadIdx.x; i <= Nl; i += blockDim.x) {
+    sh_cs[i] = L.bk[i];
+    sh_k2[i] = L.bk[Nl + 1 + i];
+    sh_td[i] = L.bk[2 * (Nl + 1) + i];
+  }
+  __syncthreads();
+  const int K = blockIdx.x;
+  const double lm = lms[K];
+  const int P = Nz * (Nz + 1) / 2;
+  zd* ta = tabs + (int64_t)K * (3 * P + Nz * Nz);
+  zd* tb = ta + P;
+  zd* tc = tb + P;
+  zd* td = tc + P;
+  if (threadIdx.x == 0) {
+    // Calc_Apq (apq_module.F90:230-249)
+    zd el[65];
+    for (int i = 1; i <= Nl; i++) eta[i] = csqrt_d(cz2(lm * lm - sh_k2[i].x, -sh_k2[i].y));
+    for (int i = 1; i <= Nl - 1; i++) el[i] = cexp_d(eta[i] * (-2.0 * sh_td[i].x));
+    el[Nl] = cz2(0, 0);
+    // Calc_p
+    {
+      zd eta0 = csqrt_d(cz2(lm * lm - sh_k2[0].x, -sh_k2[0].y));
+      zd ceta = eta0 / eta[1];
+      p[1][0] = cz2(1.0 - ceta.x, -ceta.y) / cadd(ceta, 1.0);
+      zd cg = sh_cs[0] / sh_cs[1];
+      p[1][1] = (cg - ceta) / (cg + ceta);
+      for (int i = 2; i <= Nl; i++) {
+        zd a0 = eta[i - 1] / eta[i];
+        zd a1 = a0 * sh_cs[i] / sh_cs[i - 1];
+        for (int k = 0; k < 2; k++) {
+          zd w = el[i - 1] * p[i - 1][k];
+          w = cadd(w, -1.0) / cadd(w, 1.0);
+          w = w * (k == 0 ? a0 : a1);
+          p[i][k] = cadd(w, 1.0) / cz2(1.0 - w.x, -w.y);
+        }
+      }
+    }
+    // Calc_q
+    {
+      q[Nl][0] = q[Nl][1] = cz2(0, 0);
+      if (Nl > 1) {
+        zd a0 = eta[Nl] / eta[Nl - 1];
+        zd a1 = a0 * sh_cs[Nl - 1] / sh_cs[Nl];
+        q[Nl - 1][0] = cz2(1.0 - a0.x, -a0.y) / cadd(a0, 1.0);
+        q[Nl - 1][1] = cz2(1.0 - a1.x, -a1.y) / cadd(a1, 1.0);
+        for (int i = Nl - 2; i >= 1; i--) {
+          zd b0 = eta[i + 1] / eta[i];
+          zd b1 = b0 * sh_cs[i] / sh_cs[i + 1];
+          for (int k = 0; k < 2; k++) {
+            zd w = el[i + 1] * q[i + 1][k];
+            w = cadd(w, -1.0) / cadd(w, 1.0);
+            w = w * (k == 0 ? b0 : b1);
+            q[i][k] = cadd(w, 1.0) / cz2(1.0 - w.x, -w.y);
+          }
+        }
+      }
+    }
+    for (int i = 1; i <= Nl; i++)
+      for (int k = 0; k < 2; k++)
+        Arr[i][k] = rdiv(1.0, eta[i]) / cz2(1.0 - (p[i][k] * q[i][k] * el[i]).x,
+                                             -(p[i][k] * q[i][k] * el[i]).y);
+    for (int I = 1; I <= Nz; I++) expz[I] = cexp_d(eta[lnumg[I]] * (-dzg[I]));
+    // Calc_pexpz_All
+    int l = lnumg[0];
+    zd e;
+    if (l > 1) e = cexp_d(eta[l] * (-2.0 * (zg[0] - sh_td[l - 1].y)));
+    else if (l == 1) e = cexp_d(eta[1] * (-2.0 * zg[0]));
+    else e = cz2(1, 0);
+    for (int I = 1; I <= Nz; I++) {
+      if (l != lnumg[I]) e = cz2(1, 0);
+      l = lnumg[I];
+      pez[2 * I] = e * p[l][0];
+      pez[2 * I + 1] = e * p[l][1];
+      e = e * expz[I] * expz[I];
+    }
+    // Calc_qexpz_All
+    l = lnumg[Nz];
+    if (l != Nl) e = cexp_d(eta[l] * (-2.0 * (sh_td[l].y - zg[Nz])));
+    else e = cz2(0, 0);
+    for (int I = Nz; I >= 1; I--) {
+      if (l != lnumg[I]) e = cz2(1, 0);
+      l = lnumg[I];
+      qez[2 * I] = e * q[l][0];
+      qez[2 * I + 1] = e * q[l][1];
+      e = e * expz[I] * expz[I];
+    }
+  }
+  __syncthreads();
+  for (int I = 1 + threadIdx.x; I <= Nz; I += blockDim.x) {
+    const int l = lnumg[I];
+    const zd ez = expz[I], et = eta[l], sg = sh_cs[l];
+    if (I <= Nz - 1) {  // Calc_tframe_p (apq_module.F90:83-110)
+      zd a[2], bb[2], c[2];
+      for (int k = 0; k < 2; k++) {
+        a[k] = pez[2 * I + k];
+        bb[k] = a[k] * ez;
+        c[k] = bb[k] * ez;
+        a[k] = cadd(a[k], 1.0);
+        bb[k] = cadd(bb[k], 1.0);
+        c[k] = cadd(c[k], 1.0);
+      }
+      const zd w = cz2(1.0 - ez.x, -ez.y) / et;
+      for (int k = 0; k < 2; k++) fct[2 * I + k] = ez * a[k] / c[k];
+      tfr[3 * I + 0] = w * bb[0] / c[0];
+      tfr[3 * I + 1] = cz2(1.0 - fct[2 * I + 1].x, -fct[2 * I + 1].y) / sg;
+      tfr[3 * I + 2] = (w * bb[1] / c[1]) / sg;
+    }
+    if (I >= 2) {  // Calc_bframe_q (138-153)
+      zd a = qez[2 * I + 1];
+      const zd d = cadd(a * ez * ez, 1.0);
+      a = cadd(a, 1.0);
+      fcb[I] = ez * a / d;
+      bfr[I] = cadd(fcb[I], -1.0) / sg;
+    }
+    // Calc_FtFb (31-79)
+    zd a[2], a1[2], a2[2], p1[2], p2[2], pe1[2], pe2[2], q1[2], q2[2], qe1[2], qe2[2], Fb3[3],
+        w[2];
+    for (int k = 0; k < 2; k++) {
+      a[k] = cz2(1.0 - ez.x, -ez.y) * Arr[l][k];
+      a1[k] = a[k] / et;
+      a2[k] = a1[k] / et;
+      const zd pz = pez[2 * I + k], qz = qez[2 * I + k];
+      p1[k] = cadd(pz, 1.0);
+      p2[k] = cz2(1.0 - pz.x, -pz.y);
+      pe1[k] = cadd(pz * ez, 1.0);
+      pe2[k] = cadd(pz * ez, -1.0);
+      q1[k] = cadd(qz, 1.0);
+      q2[k] = cz2(1.0 - qz.x, -qz.y);
+      qe1[k] = cadd(qz * ez, 1.0);
+      qe2[k] = cz2(1.0 - (qz * ez).x, -(qz * ez).y);
+    }
+    const zd eta2 = et * et;
+    for (int k = 0; k < 2; k++) {
+      Ft[3 * I + k] = a1[k] * p1[k] * qe1[k];
+      Fb3[k] = a1[k] * pe1[k] * q1[k];
+      w[k] = -(a2[k] * (p2[k] * qe1[k] + pe1[k] * q2[k]));
+    }
+    Ft[3 * I + 2] = a[1] * p1[1] * qe2[1];
+    Fb3[2] = a[1] * pe2[1] * q1[1];
+    const zd dzt = rdiv(2.0, eta2) * dzg[I];
+    GII[4 * I + 0] = w[0] + dzt;
+    GII[4 * I + 1] = Fb3[1] - Ft[3 * I + 1];
+    GII[4 * I + 2] = (w[1] + dzt) * L.iwm + eta2 / sg * w[1];
+    GII[4 * I + 3] = Fb3[2] - Ft[3 * I + 2] - eta2 * w[0];
+    Fb[I] = Fb3[1];
+  }
+  __syncthreads();
+  // tables: one thread per source layer Izs (column)
+  const double lm2 = lm * lm;
+  for (int Izs = 1 + threadIdx.x; Izs <= Nz; Izs += blockDim.x) {
+    const int js = Izs - 1;
+    const zd sg = sh_cs[lnumg[Izs]];
+    const zd* g4 = GII + 4 * Izs;
+    const int dia = js + js * (js + 1) / 2;
+    ta[dia] = (g4[0] * lm2 + g4[3]) / sg;  // coefficient of W_DXX (diag, :89-103)
+    tb[dia] = g4[0] * L.iwm;               // coefficient of W_D0
+    tc[dia] = g4[2];                       // EZZ
+    td[js + Nz * js] = g4[1] / sg;         // EXZ/EYZ
+    // GoByShellsU (ie_kernel_hankel_module.F90:125-173)
+    zd u1 = Ft[3 * Izs + 0] * L.iwm, u2 = Ft[3 * Izs + 1], u3 = Ft[3 * Izs + 2];
+    for (int Izr = Izs - 1; Izr >= 1; Izr--) {
+      const int ir = Izr - 1, pi = ir + js * (js + 1) / 2;
+      const zd* fr = tfr + 3 * Izr;
+      const zd t1 = u1 * fr[0];
+      ta[pi] = t1 + u3 * fr[1];
+      tb[pi] = t1;
+      tc[pi] = (u2 * lm2) * fr[2];
+      td[ir + Nz * js] = u2 * fr[1];
+      u1 = u1 * fct[2 * Izr];
+      u2 = u2 * fct[2 * Izr + 1];
+      u3 = u3 * fct[2 * Izr + 1];
+    }
+    // GoByShellsL (175-197)
+    zd v = Fb[Izs];
+    for (int Izr = Izs + 1; Izr <= Nz; Izr++) {
+      td[(Izr - 1) + Nz * js] = v * bfr[Izr];
+      v = v * fcb[Izr];
+    }
+  }
+}
+
+// --------------------------------------------------------- weighted sum ---
+// C[b][off + e] = scale * sum_seg sum_K A_seg[K][b] * B_seg[K][e]
+// (A real, B complex).  Tile 64 offsets x 32 entries, K-chunks of 16.
+struct GemmSeg {
+  const double* A;   // [K][M]
+  const zd* B;  // [K][ldb], entry e at B[K*ldb + e]
+};
+__global__ __launch_bounds__(256) void k_wgemm(zd* __restrict__ C, int64_t M, int64_t BS,
+                                               int coff, int Ne, int64_t ldb, GemmSeg s0,
+                                               GemmSeg s1, int nseg, double scale) {
+  constexpr int TM = 64, TN = 32, TK = 16;
+  __shared__ double As[TK][TM];
+  __shared__ zd Bs[TK][TN + 1];
+  const int m0 = blockIdx.x * TM, n0 = blockIdx.y * TN;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;  // 16 x 16
+  zd acc[4][2];
+  for (int a = 0; a < 4; a++)
+    for (int c = 0; c < 2; c++) acc[a][c] = mkz(0, 0);
+  for (int sgi = 0; sgi < nseg; sgi++) {
+    const GemmSeg sg = sgi == 0 ? s0 : s1;
+    for (int k0 = 0; k0 < NTAPS; k0 += TK) {
+      __syncthreads();
+      for (int i = threadIdx.x; i < TK * TM; i += 256) {
+        const int kk = i / TM, mm = i - kk * TM;
+        const int K = k0 + kk;
+        As[kk][mm] = (K < NTAPS && m0 + mm < M) ? sg.A[(int64_t)K * M + m0 + mm] : 0.0;
+      }
+      for (int i = threadIdx.x; i < TK * TN; i += 256) {
+        const int kk = i / TN, nn = i - kk * TN;
+        const int K = k0 + kk;
+        Bs[kk][nn] = (K < NTAPS && n0 + nn < Ne) ? sg.B[(int64_t)K * ldb + n0 + nn] : mkz(0, 0);
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int kk = 0; kk < TK; kk++) {
+        double a[4];
+        for (int r = 0; r < 4; r++) a[r] = As[kk][ty + 16 * r];
+        zd bv[2] = {Bs[kk][tx], Bs[kk][tx + 16]};
+        for (int r = 0; r < 4; r++)
+          for (int c = 0; c < 2; c++) {
+            acc[r][c].x = fma(a[r], bv[c].x, acc[r][c].x);
+            acc[r][c].y = fma(a[r], bv[c].y, acc[r][c].y);
+          }
+      }
+    }
+  }
+  for (int r = 0; r < 4; r++)
+    for (int c = 0; c < 2; c++) {
+      const int64_t m = m0 + ty + 16 * r;
+      const int n = n0 + tx + 16 * c;
+      if (m < M && n < Ne) C[m * BS + coff + n] = acc[r][c] * scale;
+    }
+}
+
+// ------------------------------------------------------------ tensor FFT ---
+__device__ inline void entry_signs(int Nz, int64_t e, double* sx, double* sy) {
+  const int64_t P = (int64_t)Nz * (Nz + 1) / 2;
+  if (e < P) { *sx = 1; *sy = 1; }
+  else if (e < 2 * P) { *sx = -1; *sy = -1; }
+  else if (e < 4 * P) { *sx = 1; *sy = 1; }
+  else if (e < 4 * P + (int64_t)Nz * Nz) { *sx = -1; *sy = 1; }
+  else { *sx = 1; *sy = -1; }
+}
+
+template <int L> constexpr int ne_for() { return L <= 64 ? 32 : (L <= 128 ? 16 : (L <= 512 ? 8 : 4)); }
+
+// AXIS 0: transform along y for a fixed lx (blockIdx.x); AXIS 1: along x for fixed ky.
+template <int L, int AXIS>
+__global__ __launch_bounds__(256) void k_tfft(zd* __restrict__ G, const zd* __restrict__ twg,
+                                              int Nx, int Ny, int Nz, int64_t BS) {
+  constexpr int NE = ne_for<L>(), LS = L + 1;
+  __shared__ zd buf[NE * LS];
+  __shared__ zd tw[L];
+  const int N = L / 2;  // Ny (AXIS 0) or Nx (AXIS 1)
+  const int fixed = blockIdx.x;
+  const int64_t e0 = (int64_t)blockIdx.y * NE;
+  for (int i = threadIdx.x; i < L; i += 256) tw[i] = twg[i];
+  auto gidx = [&](int l) -> int64_t {
+    return AXIS == 0 ? ((int64_t)fixed + (int64_t)Nx * l) * BS : ((int64_t)l + (int64_t)Nx * fixed) * BS;
+  };
+  for (int i = threadIdx.x; i < NE * N; i += 256) {
+    const int l = i / NE, s = i - l * NE;
+    const int64_t e = e0 + s;
+    zd g = mkz(0, 0);
+    double sx = 1, sy = 1;
+    if (e < BS) {
+      g = G[gidx(l) + e];
+      entry_signs(Nz, e, &sx, &sy);
+    }
+    const double sgn = AXIS == 0 ? sy : sx;
+    buf[s * LS + l] = g;
+    if (l > 0) buf[s * LS + L - l] = g * sgn;
+    else buf[s * LS + N] = mkz(0, 0);
+  }
+  __syncthreads();
+  lds_fft<zd, L, NE, LS, 256, -1>(buf, tw);
+  for (int i = threadIdx.x; i < NE * N; i += 256) {
+    const int k = i / NE, s = i - k * NE;
+    const int64_t e = e0 + s;
+    if (e < BS) {
+      const zd t = buf[s * LS + N];
+      const double sg = (k & 1) ? -1.0 : 1.0;
+      G[gidx(k) + e] = buf[s * LS + k] - t * sg;
+    }
+  }
+}
+
+static void make_tw64(int L, int dir, zd** d) {
+  std::vector<zd> h(L);
+  for (int t = 0; t < L; t++) {
+    long double a = -2.0L * 3.14159265358979323846264338327950288L * t / L;
+    h[t] = mkz((double)cosl(a), (double)sinl(a));
+  }
+  (void)dir;
+  G2G_HIP(hipMalloc(d, sizeof(zd) * L));
+  G2G_HIP(hipMemcpy(*d, h.data(), sizeof(zd) * L, hipMemcpyHostToDevice));
+}
+
+void op_tensor_fft(Operator& op, zd* G) {
+  const int Nx2 = 2 * op.Nx, Ny2 = 2 * op.Ny;
+  zd *twx, *twy;
+  make_tw64(Nx2, -1, &twx);
+  make_tw64(Ny2, -1, &twy);
+  pow2_dispatch(Ny2, [&](auto LC) {
+    constexpr int LL = decltype(LC)::value;
+    constexpr int NE = ne_for<LL>();
+    hipLaunchKernelGGL((k_tfft<LL, 0>), dim3(op.Nx, (unsigned)((op.BS + NE - 1) / NE)), dim3(256), 0,
+                       op.stream, G, twy, op.Nx, op.Ny, op.Nz, op.BS);
+  });
+  pow2_dispatch(Nx2, [&](auto LC) {
+    constexpr int LL = decltype(LC)::value;
+    constexpr int NE = ne_for<LL>();
+    hipLaunchKernelGGL((k_tfft<LL, 1>), dim3(op.Ny, (unsigned)((op.BS + NE - 1) / NE)), dim3(256), 0,
+                       op.stream, G, twx, op.Nx, op.Ny, op.Nz, op.BS);
+  });
+  G2G_HIP(hipGetLastError());
+  G2G_HIP(hipStreamSynchronize(op.stream));
+  (void)hipFree(twx);
+  (void)hipFree(twy);
+}
+
+void op_free_tensor_scratch(Operator& op) {
+  if (op.wts) (void)hipFree(op.wts);
+  if (op.tabs) (void)hipFree(op.tabs);
+  op.wts = nullptr;
+  op.tabs = nullptr;
+  op.have_wts = false;
+}
+
+void op_build_tensor(Operator& op) {
+  G2G_CHECK(op.have_bkg, -4, "background not set");
+  G2G_CHECK(op.Nl <= 64 && op.Nz <= 64, -3, "tensor build supports Nl <= 64 and Nz <= 64");
+  const int64_t M = (int64_t)op.Nx * op.Ny;
+  const int Nz = op.Nz, P = Nz * (Nz + 1) / 2;
+  const double rmin = std::sqrt(op.dx * op.dx + op.dy * op.dy) / 2.0;  // :102
+  std::vector<double> lms(NTAPS);
+  for (int j = 0; j < NTAPS; j++) lms[j] = std::exp(VB_XI + (double)(j + NFIRST - 512) * VB_YSTEP) / rmin;
+  double* d_lms;
+  G2G_HIP(hipMalloc(&d_lms, sizeof(double) * NTAPS));
+  G2G_HIP(hipMemcpy(d_lms, lms.data(), sizeof(double) * NTAPS, hipMemcpyHostToDevice));
+  hipEvent_t e0, e1;
+  G2G_HIP(hipEventCreate(&e0));
+  G2G_HIP(hipEventCreate(&e1));
+  G2G_HIP(hipEventRecord(e0, op.stream));
+  if (!op.have_wts) {
+    zd *tw, *f1;
+    make_tw64(VBN, -1, &tw);
+    G2G_HIP(hipMalloc(&f1, sizeof(zd) * VBN));
+    if (!op.wts) G2G_HIP(hipMalloc(&op.wts, sizeof(double) * 6 * NTAPS * M));
+    hipLaunchKernelGGL(k_f1, dim3(1), dim3(256), 0, op.stream, f1, tw);
+    WParams wp{op.Nx, op.Ny, op.dx, op.dy, rmin, std::log(rmin)};
+    hipLaunchKernelGGL(k_weights, dim3((unsigned)M), dim3(256), 0, op.stream, op.wts, f1, tw, d_lms, wp);
+    G2G_HIP(hipGetLastError());
+    G2G_HIP(hipStreamSynchronize(op.stream));
+    (void)hipFree(tw);
+    (void)hipFree(f1);
+    op.have_wts = true;
+  }
+  // lambda tables
+  LParams lp;
+  lp.Nl = op.Nl;
+  lp.Nz = Nz;
+  const double mw = 4.0 * PI_D * 1e-7 * op.omega;
+  lp.iwm = mkz(0.0, mw);
+  std::vector<zd> bk(3 * (op.Nl + 1), mkz(0, 0));
+  for (int i = 0; i <= op.Nl; i++) {
+    bk[i] = op.csigma[i];
+    bk[op.Nl + 1 + i] = mkz(-op.csigma[i].y * mw, op.csigma[i].x * mw);  // i*sigma*mu0*w
+  }
+  {
+    double depth = 0;
+    for (int i = 1; i <= op.Nl - 1; i++) {
+      depth += op.thick[i];
+      bk[2 * (op.Nl + 1) + i] = mkz(op.thick[i], depth);
+    }
+  }
+  zd* d_bk;
+  G2G_HIP(hipMalloc(&d_bk, sizeof(zd) * bk.size()));
+  G2G_HIP(hipMemcpy(d_bk, bk.data(), sizeof(zd) * bk.size(), hipMemcpyHostToDevice));
+  lp.bk = d_bk;
+  const size_t tab_n = (size_t)NTAPS * (3 * P + Nz * Nz);
+  if (!op.tabs || op.tabs_bytes < tab_n * sizeof(zd)) {
+    if (op.tabs) (void)hipFree(op.tabs);
+    G2G_HIP(hipMalloc(&op.tabs, tab_n * sizeof(zd)));
+    op.tabs_bytes = tab_n * sizeof(zd);
+  }
+  double *d_dz, *d_z;
+  int* d_ln;
+  std::vector<double> dz1(Nz + 1, 0.0);
+  for (int i = 1; i <= Nz; i++) dz1[i] = op.dz[i - 1];
+  G2G_HIP(hipMalloc(&d_dz, sizeof(double) * (Nz + 1)));
+  G2G_HIP(hipMalloc(&d_z, sizeof(double) * (Nz + 1)));
+  G2G_HIP(hipMalloc(&d_ln, sizeof(int) * (Nz + 1)));
+  G2G_HIP(hipMemcpy(d_dz, dz1.data(), sizeof(double) * (Nz + 1), hipMemcpyHostToDevice));
+  G2G_HIP(hipMemcpy(d_z, op.z.data(), sizeof(double) * (Nz + 1), hipMemcpyHostToDevice));
+  G2G_HIP(hipMemcpy(d_ln, op.lnum.data(), sizeof(int) * (Nz + 1), hipMemcpyHostToDevice));
+  const size_t sm = sizeof(zd) * (size_t)(Nz + 1) * 20;
+  hipLaunchKernelGGL(k_lambda, dim3(NTAPS), dim3(64), sm, op.stream, (zd*)op.tabs, d_lms, d_dz,
+                     d_z, d_ln, lp);
+  G2G_HIP(hipGetLastError());
+  // weighted sums into the real-space tensor (stored in d64.G), x 1/(4 pi)
+  const zd* T = (const zd*)op.tabs;
+  const int64_t ldb = 3 * P + (int64_t)Nz * Nz;
+  const double* W = op.wts;
+  auto Wc = [&](int c) { return W + (int64_t)c * NTAPS * M; };
+  const double scale = 1.0 / PI_D / 4.0;
+  struct Job { int coff, Ne, nseg; GemmSeg a, b; };
+  const Job jobs[6] = {
+      {0, P, 2, {Wc(IE_DXX), T}, {Wc(IE_D0), T + P}},
+      {P, P, 1, {Wc(IE_DXY), T}, {nullptr, nullptr}},
+      {2 * P, P, 2, {Wc(IE_DYY), T}, {Wc(IE_D0), T + P}},
+      {3 * P, P, 1, {Wc(IE_D0), T + 2 * P}, {nullptr, nullptr}},
+      {4 * P, Nz * Nz, 1, {Wc(IE_DX), T + 3 * P}, {nullptr, nullptr}},
+      {4 * P + Nz * Nz, Nz * Nz, 1, {Wc(IE_DY), T + 3 * P}, {nullptr, nullptr}},
+  };
+  for (const Job& j : jobs) {
+    dim3 grid((unsigned)((M + 63) / 64), (unsigned)((j.Ne + 31) / 32));
+    hipLaunchKernelGGL(k_wgemm, grid, dim3(256), 0, op.stream, op.d64.G, M, op.BS, j.coff, j.Ne, ldb,
+                       j.a, j.b, j.nseg, scale);
+  }
+  G2G_HIP(hipGetLastError());
+  G2G_HIP(hipEventRecord(e1, op.stream));
+  G2G_HIP(hipStreamSynchronize(op.stream));
+  float ms = 0;
+  G2G_HIP(hipEventElapsedTime(&ms, e0, e1));
+  op.cnt.t_tensor = ms * 1e-3;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(d_lms);
+  (void)hipFree(d_bk);
+  (void)hipFree(d_dz);
+  (void)hipFree(d_z);
+  (void)hipFree(d_ln);
+}
+
+}  // namespace g2g

@@ -1,0 +1,167 @@
+"""Python mirror of the reference operator interface on the HIP library.
+
+IEOperator follows the life cycle of giem2g_interfaces.F90:76-254
+(calc_data_sizes -> prepare_ie_kernel -> calc_ie_kernel -> calc_fft_of_ie_kernel
+-> set_anomaly_conductivity -> apply -> print_stats -> delete) and adds the
+device-pointer and solver entry points.  Arrays: numpy (host) or torch
+tensors on the GPU (device).  Everything goes through the C ABI.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import lib as L
+
+ORDER_XYZ = 0
+ORDER_YXZ = 1
+
+
+def _ptr(x):
+    """Raw address of a numpy array or torch tensor (contiguous)."""
+    if isinstance(x, np.ndarray):
+        assert x.flags["C_CONTIGUOUS"]
+        return x.ctypes.data
+    if hasattr(x, "data_ptr"):
+        assert x.is_contiguous()
+        return x.data_ptr()
+    if x is None:
+        return None
+    return int(x)
+
+
+class IEOperator:
+    def __init__(self, Nx, Ny, Nz, dx, dy, z, order=ORDER_XYZ):
+        self.lib = L.load()
+        self.Nx, self.Ny, self.Nz = int(Nx), int(Ny), int(Nz)
+        self.dx, self.dy = float(dx), float(dy)
+        self.z = np.ascontiguousarray(z, np.float64)
+        h = C.c_void_p()
+        L.check(self.lib.giem2g_op_create(self.Nx, self.Ny, self.Nz, self.dx, self.dy,
+                                          self.z.ctypes.data, C.byref(h)))
+        self.h = h
+        if order != ORDER_XYZ:
+            L.check(self.lib.giem2g_set_component_order(self.h, order))
+
+    @classmethod
+    def from_model(cls, m, order=ORDER_XYZ):
+        return cls(m.Nx, m.Ny, m.Nz, m.dx, m.dy, m.z, order)
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.check(self.lib.giem2g_op_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def block_size(self):
+        return 4 * self.Nz * (self.Nz + 1) // 2 + 2 * self.Nz * self.Nz
+
+    @property
+    def nvec(self):
+        return 3 * self.Nx * self.Ny * self.Nz
+
+    # --- frequency / tensor -------------------------------------------------
+    def set_background(self, csigb, thick, omega):
+        cs = np.ascontiguousarray(np.asarray(csigb, np.complex128))
+        th = np.ascontiguousarray(np.r_[np.asarray(thick, np.float64), 0.0])
+        L.check(self.lib.giem2g_op_set_background(self.h, len(cs) - 1, cs.ctypes.data,
+                                                  th.ctypes.data, float(omega)))
+
+    def build_tensor(self):
+        L.check(self.lib.giem2g_op_build_tensor(self.h))
+
+    def tensor_stage(self, kind):
+        L.check(self.lib.giem2g_op_tensor_stage(self.h, kind))
+
+    def get_tensor(self):
+        G = np.empty(self.Nx * self.Ny * self.block_size, np.complex128)
+        L.check(self.lib.giem2g_op_get_tensor(self.h, G.ctypes.data))
+        return G
+
+    def put_tensor(self, G, spectral=True):
+        G = np.ascontiguousarray(G, np.complex128)
+        assert G.size == self.Nx * self.Ny * self.block_size
+        L.check(self.lib.giem2g_op_put_tensor(self.h, G.ctypes.data, 1 if spectral else 0))
+
+    # --- conductivity --------------------------------------------------------
+    def set_sigma(self, csiga):
+        if isinstance(csiga, np.ndarray):
+            c = np.ascontiguousarray(csiga, np.complex128)
+            L.check(self.lib.giem2g_op_set_sigma(self.h, c.ctypes.data))
+        else:
+            L.check(self.lib.giem2g_op_set_sigma_dev(self.h, _ptr(csiga)))
+
+    # --- apply ---------------------------------------------------------------
+    def apply_host(self, v):
+        v = np.ascontiguousarray(v, np.complex128)
+        out = np.empty_like(v)
+        L.check(self.lib.giem2g_op_apply_host(self.h, v.ctypes.data, out.ctypes.data))
+        return out
+
+    def apply_dev(self, vin, vout):
+        """FP64 matvec on HBM (asynchronous on the operator stream)."""
+        L.check(self.lib.giem2g_op_apply_dev(self.h, _ptr(vin), _ptr(vout)))
+
+    def apply_dev_f32(self, vin, vout):
+        L.check(self.lib.giem2g_op_apply_dev_f32(self.h, _ptr(vin), _ptr(vout)))
+
+    def sync(self):
+        L.check(self.lib.giem2g_op_sync(self.h))
+
+    def time_apply(self, vin, vout, reps=10, precision=0):
+        ms = np.zeros(3)
+        L.check(self.lib.giem2g_op_time_apply(self.h, precision, _ptr(vin), _ptr(vout), reps,
+                                              ms.ctypes.data))
+        return ms
+
+    def stats(self):
+        o = np.zeros(11)
+        L.check(self.lib.giem2g_op_stats(self.h, o.ctypes.data))
+        keys = ["matvecs", "matvecs32", "dotprods", "t_apply", "t_fwd", "t_contr", "t_bwd", "t_dot",
+                "t_tensor", "t_tensor_fft", "device_bytes"]
+        return dict(zip(keys, o.tolist()))
+
+    # --- sources / solve -----------------------------------------------------
+    def plane_wave(self, pol):
+        E = np.empty(self.nvec, np.complex128)
+        L.check(self.lib.giem2g_plane_wave(self.h, pol, E.ctypes.data))
+        return E
+
+    def solve(self, Ebkg, E0=None, outer_m=30, inner_m=30, max_restart=100, tol=1e-8,
+              inner_precision=0, precond=0, verbose=0):
+        p = L.SolverParams(outer_m, inner_m, max_restart, tol, inner_precision, precond, verbose, 0)
+        info = L.SolveInfo()
+        if isinstance(Ebkg, np.ndarray):
+            Ebkg = np.ascontiguousarray(Ebkg, np.complex128)
+            out = np.empty_like(Ebkg)
+            e0 = None if E0 is None else np.ascontiguousarray(E0, np.complex128)
+            L.check(self.lib.giem2g_solve(self.h, Ebkg.ctypes.data,
+                                          None if e0 is None else e0.ctypes.data, C.byref(p),
+                                          out.ctypes.data, C.byref(info)))
+        else:
+            import torch
+            out = torch.empty_like(Ebkg)
+            L.check(self.lib.giem2g_solve_dev(self.h, _ptr(Ebkg), _ptr(E0), C.byref(p), _ptr(out),
+                                              C.byref(info)))
+        return out, info.as_dict()
+
+    def history(self, maxlen=100000):
+        buf = np.zeros(maxlen)
+        n = C.c_int()
+        L.check(self.lib.giem2g_last_history(self.h, buf.ctypes.data, maxlen, C.byref(n)))
+        return buf[: min(n.value, maxlen)]
+
+
+def operator_for(model, order=ORDER_XYZ, build=True):
+    """IEOperator with background, tensor and conductivity of a models.Model."""
+    op = IEOperator.from_model(model, order)
+    op.set_background(model.csigb(), model.thick, model.omega)
+    if build:
+        op.build_tensor()
+    op.set_sigma(model.csiga())
+    return op

@@ -1,0 +1,206 @@
+/*
+ * giem2g_amd.h -- C ABI of the MI355X-native GIEM2G integral-equation solve.
+ *
+ * Library: giem2g_amd/_lib/libgiem2g_amd.so (hipcc --offload-arch=gfx950).
+ * One process, one GPU, no MPI split.  Complex numbers are interleaved
+ * (re, im) doubles ("double _Complex" / Fortran COMPLEX(C_DOUBLE_COMPLEX)).
+ *
+ * Field vectors are Fortran-ordered (Nx, Ny, Nz, 3): element
+ * v[x + Nx*(y + Ny*(z + Nz*c))].  The component slot order defaults to
+ * (x, y, z) as in the reference CLI build; giem2g_set_component_order()
+ * selects the (y, x, z) order of the reference shared library, which is
+ * built with -D ExtrEMe (const_module.F90:72-86, Makefile:87).
+ *
+ * Section 1 mirrors the reference C ABI exactly (giem2g_interfaces.F90:
+ * 33-58 for the structs, 76-254 for the functions).  Section 2 adds
+ * status-returning, device-pointer and solver entry points that have no
+ * reference counterpart; each says so.
+ */
+#ifndef GIEM2G_AMD_H
+#define GIEM2G_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ===================================================================== */
+/* 1. Reference-compatible ABI                                           */
+/* ===================================================================== */
+
+/* GIEM2G_BACKGROUND_TYPE (giem2g_interfaces.F90:33-37): csigb points to
+ * Nl+1 complex conductivities, [0] = air, displacement currents already
+ * included (sigma - i*omega*eps0); thickness points to Nl-1 doubles. */
+typedef struct {
+  int Nl;
+  void *csigb;
+  void *thickness;
+} GIEM2G_BACKGROUND_TYPE;
+
+/* GIEM2G_ANOMALY_TYPE (giem2g_interfaces.F90:39-44): z -> Nz+1 layer
+ * borders (m, positive down), dz -> Nz thicknesses. */
+typedef struct {
+  int Nx, Ny, Nz;
+  double dx, dy;
+  void *z;
+  void *dz;
+} GIEM2G_ANOMALY_TYPE;
+
+/* GIEM2G_DATA_TYPE (giem2g_interfaces.F90:46-58). */
+typedef struct {
+  long long tensor_size;
+  long long ie_kernel_buffer_length;
+  long long fft_buffers_length;
+  long long comm64;
+  void *giem2g_tensor;
+  void *dz;
+  void *sqsigb;
+  void *csigb;
+  void *kernel_buffer;
+  void *fft_buffer_in;
+  void *fft_buffer_out;
+} GIEM2G_DATA_TYPE;
+
+/* Replaces GIEM2G_CALC_DATA_SIZES (giem2g_interfaces.F90:76-100).  Creates
+ * the operator object and returns it in data->giem2g_tensor; reports
+ * ie_kernel_buffer_length = Nx*Ny*2*Nz*(2*Nz+1) and fft_buffers_length =
+ * 4*Nx*Ny*3*Nz complex elements (the reference's sizes for one process).
+ * comm64 is ignored (one process). */
+void giem2g_calc_data_sizes(GIEM2G_ANOMALY_TYPE anomaly, GIEM2G_DATA_TYPE *data);
+
+/* Replaces GIEM2G_PREPARE_IE_KERNEL (giem2g_interfaces.F90:102-141).
+ * Allocates the device tensor and FFT buffers.  The caller's
+ * kernel_buffer / fft_buffer_in / fft_buffer_out are accepted and not used:
+ * the tensor and spectra live in HBM (they may be NULL). */
+void giem2g_prepare_ie_kernel(GIEM2G_ANOMALY_TYPE anomaly, GIEM2G_DATA_TYPE data);
+
+/* Replaces GIEM2G_CALC_INTEGRAL_GREEN_TENSOR (giem2g_interfaces.F90:144-174):
+ * FP64 Hankel-filter tensor build on the GPU for angular frequency omega. */
+void giem2g_calc_ie_kernel(void *ie_op, GIEM2G_BACKGROUND_TYPE bkg,
+                           GIEM2G_ANOMALY_TYPE anomaly, double omega);
+
+/* Replaces GIEM2G_CALC_FFT_OF_GREEN_TENSOR (giem2g_interfaces.F90:176-188). */
+void giem2g_calc_fft_of_ie_kernel(void *ie_op);
+
+/* Replaces GIEM2G_SET_ANOMALY_CONDUCTIVITY (giem2g_interfaces.F90:190-201).
+ * csiga: complex (Nx,Ny,Nz) host array.  The values are copied to the GPU
+ * at this call (the reference aliases the array; call again after changing
+ * it). */
+void giem2g_set_anomaly_conductivity(void *ie_op, void *csiga);
+
+/* Replaces GIEM2G_APPLY_INTEGRAL_OPERATOR (giem2g_interfaces.F90:203-232):
+ * out = A*in, both complex (Nx,Ny,Nz,3) HOST arrays (copied over PCIe). */
+void giem2g_apply_ie_operator(void *ie_op, void *in, void *out);
+
+/* Replaces GIEM2G_SET_LOGGER (giem2g_interfaces.F90:234-238). */
+void giem2g_set_logger(void (*logger)(const char *));
+
+/* Replaces GIEM2G_PRINT_STATS (giem2g_interfaces.F90:240-247): logs the
+ * operator counters (fwd FFT / contraction / bwd FFT / apply averages) and
+ * resets them. */
+void giem2g_print_stats(void *ie_op);
+
+/* Replaces GIEM2G_DELETE_OPERATOR (giem2g_interfaces.F90:248-254): frees
+ * the device buffers and the object. */
+void giem2g_delete_ie_operator(void *ie_op);
+
+/* ===================================================================== */
+/* 2. Extensions (no reference counterpart).  Return 0 on success, a     */
+/*    negative code on error; giem2g_last_error() gives the message.     */
+/* ===================================================================== */
+#define GIEM2G_ORDER_XYZ 0
+#define GIEM2G_ORDER_YXZ 1
+
+const char *giem2g_last_error(void);
+
+/* Component slot order of in/out vectors (default GIEM2G_ORDER_XYZ). */
+int giem2g_set_component_order(void *ie_op, int order);
+
+/* Status-returning creation without the reference structs. */
+int giem2g_op_create(int Nx, int Ny, int Nz, double dx, double dy,
+                     const double *z /* Nz+1 */, void **ie_op);
+int giem2g_op_destroy(void *ie_op);
+/* csigb: 2*(Nl+1) doubles (complex, [0] = air); thick: Nl-1 doubles. */
+int giem2g_op_set_background(void *ie_op, int Nl, const double *csigb,
+                             const double *thick, double omega);
+/* Tensor build + tensor spectrum (giem2g_calc_ie_kernel followed by
+ * giem2g_calc_fft_of_ie_kernel). */
+int giem2g_op_build_tensor(void *ie_op);
+/* Only the real-space tensor build (kind 0) or only its spectrum (kind 1). */
+int giem2g_op_tensor_stage(void *ie_op, int kind);
+/* Copy the current tensor (real-space or spectral, Nx*Ny blocks of
+ * 4*Nz*(Nz+1)/2 + 2*Nz*Nz complex, block b = kx + Nx*ky) to/from the host. */
+int giem2g_op_get_tensor(void *ie_op, double *G_host);
+int giem2g_op_put_tensor(void *ie_op, const double *G_host, int spectral);
+/* Copy the complex (Nx,Ny,Nz) conductivity from host memory / HBM. */
+int giem2g_op_set_sigma(void *ie_op, const double *csiga_host);
+int giem2g_op_set_sigma_dev(void *ie_op, const void *csiga_dev);
+/* out = A*in on HBM pointers, complex128 (the FP64 drop-in) or complex64. */
+int giem2g_op_apply_dev(void *ie_op, const void *in_dev, void *out_dev);
+int giem2g_op_apply_dev_f32(void *ie_op, const void *in_dev, void *out_dev);
+/* Same on host pointers (PCIe copies included). */
+int giem2g_op_apply_host(void *ie_op, const double *in, double *out);
+/* Stream used by the operator (hipStream_t) and device synchronisation. */
+int giem2g_op_stream(void *ie_op, void **stream);
+int giem2g_op_sync(void *ie_op);
+/* Counters: [matvecs, matvecs32, dotprods, t_apply, t_fwd, t_contr, t_bwd,
+ * t_dot, t_tensor, t_tensor_fft, device_bytes] */
+int giem2g_op_stats(void *ie_op, double *out11);
+/* Per-kernel timing of one apply: [fwd FFT, contraction, bwd FFT] in ms,
+ * measured with HIP events on the operator stream (precision 0 FP64, 1 FP32). */
+int giem2g_op_time_apply(void *ie_op, int precision, const void *in_dev,
+                         void *out_dev, int reps, double *ms3);
+
+/* Plane-wave background field (PlaneWaveIntegral, sources_module.F90:67-101)
+ * for polarization pol (0 = EX, 1 = EY) into a complex (Nx,Ny,Nz,3) host
+ * array, using the operator's current background and geometry. */
+int giem2g_plane_wave(void *ie_op, int pol, double *E_host);
+
+/* Nested FGMRES solve (SolveEquation, ie_solver_module.F90:43-114 with
+ * GFGMRES/gmres.F90): outer flexible FGMRES(outer_m) in FP64, inner
+ * GMRES(inner_m) as the right preconditioner, restarts up to max_restart,
+ * tolerance on the relative residual of the scaled system.
+ *   inner_precision: 0 = FP64 inner (reference semantics), 1 = complex64
+ *                    inner operator and Krylov basis (FP32 accumulation)
+ *   precond:         0 = inner GMRES (reference), 1 = learned
+ *                    preconditioner then inner GMRES on its residual,
+ *                    falling back to plain inner GMRES when it does not cut
+ *                    the residual (requires giem2g_set_learned_precond). */
+typedef struct {
+  int outer_m;
+  int inner_m;
+  int max_restart;
+  double tol;
+  int inner_precision;
+  int precond;
+  int verbose;
+  int reserved;
+} giem2g_solver_params;
+
+typedef struct {
+  int outer_iterations; /* Arnoldi steps of the outer solver */
+  int status;           /* 1 converged, 2 Arnoldi-converged only, -1 not */
+  double arnoldi_residual;
+  double true_residual; /* FP64 ||b - A x|| / ||b|| */
+  long long matvecs_fp64;
+  long long matvecs_fp32;
+  long long precond_calls;
+  long long precond_fallbacks;
+  double seconds;
+} giem2g_solve_info;
+
+/* E_bkg, E0 (nullable warm start), E_sol: complex (Nx,Ny,Nz,3).  _dev takes
+ * HBM pointers, the plain form host pointers. */
+int giem2g_solve(void *ie_op, const double *E_bkg, const double *E0,
+                 const giem2g_solver_params *p, double *E_sol,
+                 giem2g_solve_info *info);
+int giem2g_solve_dev(void *ie_op, const void *E_bkg_dev, const void *E0_dev,
+                     const giem2g_solver_params *p, void *E_sol_dev,
+                     giem2g_solve_info *info);
+/* Residual history of the last solve (outer Arnoldi residuals). */
+int giem2g_last_history(void *ie_op, double *out, int maxlen, int *len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

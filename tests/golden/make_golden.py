@@ -1,0 +1,100 @@
+"""Write the golden fixtures of tests/golden/ from the reference's own code.
+
+Run in the build container (needs oracle/_ref/liboracle_ref.so, which
+`make -C oracle ref` compiles from the reference's Fortran sources):
+
+    python tests/golden/make_golden.py
+
+Every array is produced by the reference's routines (VolumeBesselTransforms,
+Calc_Double_Integral_U, Calc_APQ, GFGMRES) through oracle/ref_harness.F90;
+the inputs are small synthetic cases.  Output: tests/golden/reference_vectors.npz.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+from oracle import pyoracle as po  # noqa: E402
+
+MU0 = 4e-7 * np.pi
+EPS0 = 1.0 / (MU0 * 299792458.0 ** 2)
+
+# (x, y, hx, hy) offsets for the Hankel weights: diagonal cell, on-axis, far
+WEIGHT_CASES = np.array([[50.0, 50.0, 100.0, 100.0], [150.0, 250.0, 100.0, 100.0],
+                         [3150.0, 50.0, 100.0, 50.0], [50.0, 1250.0, 100.0, 100.0],
+                         [12550.0, 9050.0, 100.0, 100.0]])
+# backgrounds: (sigma list, thick list, f)
+BKGS = [([0.01, 0.1, 1.0], [1000.0, 9000.0], 1.0), ([0.1, 0.01, 10.0], [10000.0, 20000.0], 0.1),
+        ([0.05], [], 10.0)]
+ZS = [np.linspace(0, 400, 9), np.linspace(900, 1100, 9), np.linspace(0, 1280, 17)]
+LAMBDAS = [1e-5, 3e-4, 1e-2, 0.3]
+W6 = np.array([0.3, -0.2, 0.5, 1.1, -0.7, 0.9])
+
+
+def csig(sig, f):
+    w = 2 * np.pi * f
+    return np.array([1e-12 - 1j * w * EPS0] + [s - 1j * w * EPS0 for s in sig])
+
+
+def main():
+    R = po.ref()
+    out = {}
+    rmin = np.sqrt(100.0 ** 2 + 100.0 ** 2) / 2
+    wts = []
+    for x, y, hx, hy in WEIGHT_CASES:
+        WT = np.zeros(501 * 6)
+        lms = np.zeros(501)
+        R.ref_vbt_weights(rmin, x, y, hx, hy, WT, lms)
+        wts.append(WT.reshape(6, 501).T)
+        out["lms"] = lms
+    out["weight_cases"] = WEIGHT_CASES
+    out["weight_rmin"] = np.array(rmin)
+    out["weights"] = np.array(wts)
+    k = 0
+    for bi, (sig, th, f) in enumerate(BKGS):
+        cs = csig(sig, f)
+        csd = np.ascontiguousarray(cs.view(np.float64))
+        thd = np.array(th + [0.0])
+        Nl = len(sig)
+        for zi, z in enumerate(ZS):
+            Nz = len(z) - 1
+            for lm in LAMBDAS:
+                G = np.zeros(6 * Nz * Nz, np.complex128)
+                R.ref_double_integral_u(Nl, csd, thd, 2 * np.pi * f, Nz, np.ascontiguousarray(z), lm,
+                                        W6, G)
+                out["U_%d_%d_%g" % (bi, zi, lm)] = G
+                k += 1
+        Arr = np.zeros(2 * Nl, np.complex128)
+        p = np.zeros_like(Arr)
+        q = np.zeros_like(Arr)
+        eta = np.zeros(Nl, np.complex128)
+        A = np.zeros(2 * Nl * Nl, np.complex128)
+        R.ref_calc_apq(Nl, csd, thd, 2 * np.pi * f, 0.0, Arr, p, q, eta, A)
+        out["apq_%d" % bi] = np.concatenate([Arr, p, q, eta, A])
+    # GFGMRES on dense complex systems
+    rng = np.random.Generator(np.random.PCG64(7))
+    cases = [(60, 10, 5, 50, 1e-10), (120, 8, 4, 100, 1e-9), (50, 30, 30, 100, 1e-8),
+             (80, 5, 3, 3, 1e-12)]
+    for ci, (N, mo, mi, maxit, tol) in enumerate(cases):
+        Am = np.eye(N) * 3 + (rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))) / np.sqrt(N)
+        b = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+        x0 = b.copy()
+        xr = np.zeros(N, np.complex128)
+        it, st, bea, be, nmv = C.c_int(), C.c_int(), C.c_double(), C.c_double(), C.c_longlong()
+        R.ref_fgmres_dense(N, np.ascontiguousarray(Am.T.copy()), b, x0, mo, mi, maxit, tol, xr,
+                           C.byref(it), C.byref(st), C.byref(bea), C.byref(be), C.byref(nmv))
+        out["gm%d_A" % ci] = Am
+        out["gm%d_b" % ci] = b
+        out["gm%d_params" % ci] = np.array([N, mo, mi, maxit, tol])
+        out["gm%d_x" % ci] = xr
+        out["gm%d_info" % ci] = np.array([it.value, st.value, bea.value, be.value, nmv.value])
+    path = os.path.join(HERE, "reference_vectors.npz")
+    np.savez_compressed(path, **out)
+    print("wrote", path, len(out), "arrays")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,211 @@
+"""GPU parity of the IE operator and the tensor build against the oracle.
+
+Bar: FP64 operator apply within 1e-12 relative (2-norm) of the oracle on
+identical inputs; FP64 tensor within 1e-11; complex64 operator within 2e-6.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from tests.conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+MU0 = 4e-7 * np.pi
+EPS0 = 1.0 / (MU0 * 299792458.0 ** 2)
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available(), "GPU test without a GPU"
+    return torch
+
+
+def oracle_model(po, m):
+    return po.Model(m.csigb(), m.thick, m.omega, m.Nx, m.Ny, m.Nz, m.dx, m.dy, m.z)
+
+
+def rand_vec(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(n) + 1j * rng.standard_normal(n)
+
+
+@pytest.mark.parametrize("Nx,Ny,Nz", [(8, 8, 4), (16, 8, 3), (8, 16, 5), (32, 32, 8)])
+def test_apply_matches_oracle_on_oracle_tensor(oracle, Nx, Ny, Nz):
+    from giem2g_amd import models, operator as gop
+    print("case", Nx, Ny, Nz, flush=True)
+    m = models.small(Nx, Ny, Nz, seed=Nx + Ny + Nz, freq=2.0)
+    om = oracle_model(oracle, m)
+    Gs = om.tensor_fft(om.tensor_realspace())
+    v = rand_vec(3 * m.ncell, 1)
+    ref = om.apply(Gs, m.csiga(), v)
+    op = gop.IEOperator.from_model(m)
+    op.set_background(m.csigb(), m.thick, m.omega)
+    op.put_tensor(Gs, spectral=True)
+    op.set_sigma(m.csiga())
+    assert rel(op.apply_host(v), ref) < 1e-12
+    op.close()
+
+
+@pytest.mark.parametrize("case", ["c0", "interface", "halfspace"])
+def test_tensor_build_matches_oracle(oracle, case):
+    from giem2g_amd import models, operator as gop
+    print("case", case, flush=True)
+    if case == "c0":
+        m = models.c0()
+    elif case == "interface":  # anomaly crossing the 1 km interface
+        m = models.small(16, 16, 8, seed=4, freq=0.3, dz=25.0)
+        m.z = 900.0 + np.arange(9) * 25.0
+    else:
+        m = models.small(16, 8, 4, seed=5, freq=30.0)
+        m.sigma_bkg, m.thick = [0.05], []
+    om = oracle_model(oracle, m)
+    Gr = om.tensor_realspace()
+    Gs = om.tensor_fft(Gr)
+    op = gop.IEOperator.from_model(m)
+    op.set_background(m.csigb(), m.thick, m.omega)
+    op.tensor_stage(0)
+    assert rel(op.get_tensor(), Gr) < 1e-11
+    op.tensor_stage(1)
+    assert rel(op.get_tensor(), Gs) < 1e-11
+    op.set_sigma(m.csiga())
+    v = rand_vec(3 * m.ncell, 2)
+    assert rel(op.apply_host(v), om.apply(Gs, m.csiga(), v)) < 1e-12
+    op.close()
+
+
+def test_c1_full_size_apply(oracle):
+    """C1 (128x128x32, 10 Hz): sampled tensor blocks vs the oracle, 100
+    repeated matvecs identical, one compared with the oracle operator at
+    1e-12, linearity and the sigma_a = sigma_b identity."""
+    torch = _torch()
+    from giem2g_amd import models, operator as gop
+    m = models.c1()
+    om = oracle_model(oracle, m)
+    op = gop.IEOperator.from_model(m)
+    op.set_background(m.csigb(), m.thick, m.omega)
+    op.tensor_stage(0)
+    Gr = op.get_tensor()
+    BS = op.block_size
+    for lx, ly in [(0, 0), (1, 0), (0, 5), (17, 33), (127, 127), (64, 3)]:
+        b = lx + m.Nx * ly
+        blk = om.tensor_element(lx, ly)
+        assert rel(Gr[b * BS:(b + 1) * BS], blk) < 1e-11, (lx, ly)
+    print("tensor blocks ok", flush=True)
+    op.tensor_stage(1)
+    Gs = op.get_tensor()
+    op.set_sigma(m.csiga())
+    v = models.c1_currents(m)
+    vt = torch.from_numpy(v).cuda()
+    out = torch.empty_like(vt)
+    first = None
+    for k in range(100):
+        op.apply_dev(vt, out)
+        if k == 0:
+            torch.cuda.synchronize()
+            first = out.clone()
+    torch.cuda.synchronize()
+    assert torch.equal(out, first)
+    res = first.cpu().numpy()
+    ref = om.apply(Gs, m.csiga(), v)
+    assert rel(res, ref) < 1e-12
+    print("apply vs oracle ok", flush=True)
+    u = torch.from_numpy(rand_vec(v.size, 9)).cuda()
+    au, av, auv = torch.empty_like(u), torch.empty_like(u), torch.empty_like(u)
+    a, bb = 0.3 - 1.1j, -2.0 + 0.5j
+    op.apply_dev(u, au)
+    op.apply_dev(vt, av)
+    op.apply_dev(a * u + bb * vt, auv)
+    torch.cuda.synchronize()
+    assert rel(auv, a * au + bb * av) < 1e-13
+    # sigma_a = sigma_b  =>  A = I
+    ln = om.lnum()
+    sb = np.array([m.csigb()[ln[i + 1]] for i in range(m.Nz)])
+    op.set_sigma(np.repeat(sb, m.Nx * m.Ny))
+    op.apply_dev(vt, out)
+    torch.cuda.synchronize()
+    assert rel(out, vt) < 1e-14
+    op.close()
+
+
+def test_complex64_operator_close_to_fp64(oracle):
+    torch = _torch()
+    from giem2g_amd import models, operator as gop
+    m = models.c0()
+    op = gop.operator_for(m)
+    v = torch.from_numpy(rand_vec(3 * m.ncell, 3)).cuda()
+    o64 = torch.empty_like(v)
+    o32 = torch.empty(v.numel(), dtype=torch.complex64, device="cuda")
+    op.apply_dev(v, o64)
+    op.apply_dev_f32(v.to(torch.complex64), o32)
+    torch.cuda.synchronize()
+    e = rel(o32.to(torch.complex128), o64)
+    assert e < 2e-6, e
+    op.close()
+
+
+def test_component_order_yxz(oracle):
+    """(y,x,z) slot order of the reference shared library (-D ExtrEMe)."""
+    from giem2g_amd import models, operator as gop
+    m = models.small(8, 8, 3, seed=7)
+    a = gop.operator_for(m)
+    b = gop.operator_for(m, order=gop.ORDER_YXZ)
+    v = rand_vec(3 * m.ncell, 4).reshape(3, -1)
+    va = a.apply_host(v.reshape(-1)).reshape(3, -1)
+    vb = b.apply_host(v[[1, 0, 2]].reshape(-1)).reshape(3, -1)
+    assert rel(vb[[1, 0, 2]], va) < 1e-15
+    ex = a.plane_wave(0).reshape(3, -1)
+    ey = b.plane_wave(0).reshape(3, -1)
+    assert np.array_equal(ex[0], ey[1])
+    a.close()
+    b.close()
+
+
+def test_reference_abi_lifecycle(oracle):
+    """giem2g_calc_data_sizes -> prepare -> calc_ie_kernel -> calc_fft ->
+    set_anomaly_conductivity -> apply -> print_stats -> delete
+    (giem2g_interfaces.F90:76-254) gives the oracle's operator."""
+    from giem2g_amd import lib as L, models
+    lib = L.load()
+    m = models.small(16, 16, 4, seed=8, freq=5.0)
+    z = np.ascontiguousarray(m.z)
+    dz = np.ascontiguousarray(np.diff(m.z))
+    an = L.Anomaly(m.Nx, m.Ny, m.Nz, m.dx, m.dy, z.ctypes.data, dz.ctypes.data)
+    data = L.Data()
+    lib.giem2g_calc_data_sizes(an, C.byref(data))
+    assert data.ie_kernel_buffer_length == m.Nx * m.Ny * 2 * m.Nz * (2 * m.Nz + 1)
+    assert data.fft_buffers_length == 4 * m.Nx * m.Ny * 3 * m.Nz
+    lib.giem2g_prepare_ie_kernel(an, data)
+    h = data.giem2g_tensor
+    cs = np.ascontiguousarray(m.csigb())
+    th = np.ascontiguousarray(np.array(m.thick, np.float64))
+    bkg = L.Background(m.Nl, cs.ctypes.data, th.ctypes.data)
+    lib.giem2g_calc_ie_kernel(h, bkg, an, m.omega)
+    lib.giem2g_calc_fft_of_ie_kernel(h)
+    sa = m.csiga()
+    lib.giem2g_set_anomaly_conductivity(h, sa.ctypes.data)
+    v = rand_vec(3 * m.ncell, 5)
+    out = np.empty_like(v)
+    lib.giem2g_apply_ie_operator(h, v.ctypes.data, out.ctypes.data)
+    msgs = []
+    cb = L.LOGGER(lambda s: msgs.append(s.decode()))
+    lib.giem2g_set_logger(cb)
+    lib.giem2g_print_stats(h)
+    lib.giem2g_delete_ie_operator(h)
+    assert msgs and "matvecs 1" in msgs[0]
+    om = oracle_model(oracle, m)
+    Gs = om.tensor_fft(om.tensor_realspace())
+    assert rel(out, om.apply(Gs, sa, v)) < 1e-12
+
+
+def test_error_paths():
+    from giem2g_amd import lib as L, operator as gop
+    with pytest.raises(L.G2GError, match="powers of two"):
+        gop.IEOperator(6, 8, 2, 100.0, 100.0, [0, 1, 2])
+    with pytest.raises(L.G2GError, match="Nz"):
+        gop.IEOperator(8, 8, 70, 100.0, 100.0, np.arange(71.0))
+    op = gop.IEOperator(8, 8, 2, 100.0, 100.0, [0, 10, 20])
+    with pytest.raises(L.G2GError, match="not ready"):
+        op.apply_host(np.zeros(3 * 128, np.complex128))
+    op.close()

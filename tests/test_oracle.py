@@ -1,0 +1,218 @@
+"""CPU checks of the oracle (oracle/giem2g_oracle.c).
+
+The oracle is pinned against
+  * the reference's own Fortran routines: Hankel filter weights
+    (VolumeBesselTransforms.F90), the layered kernel (Calc_Double_Integral_U,
+    ie_kernel_hankel_module.F90), Calc_APQ (apq_module.F90) and GFGMRES, via
+    the fixtures in tests/golden/reference_vectors.npz (written by
+    tests/golden/make_golden.py from oracle/_ref) and, when oracle/_ref is
+    built in this container, directly;
+  * mathematics: the FFT operator equals the direct block-Toeplitz sum, the
+    convolution matrix is complex-symmetric (reciprocity), sigma_a = sigma_b
+    gives the identity, the plane wave on a half-space has a closed form,
+    FGMRES reproduces numpy.linalg.solve.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from tests.conftest import rel
+
+MU0 = 4e-7 * np.pi
+EPS0 = 1.0 / (MU0 * 299792458.0 ** 2)
+
+
+def csig(sig, f):
+    w = 2 * np.pi * f
+    return np.array([1e-12 - 1j * w * EPS0] + [s - 1j * w * EPS0 for s in sig])
+
+
+def make_model(po, Nx=4, Ny=4, Nz=2, f=1.0, sig=(0.01, 0.1, 1.0), th=(1000.0, 9000.0), dz=50.0,
+               z0=0.0, dxy=100.0):
+    z = z0 + np.arange(Nz + 1) * dz
+    return po.Model(csig(sig, f), list(th), 2 * np.pi * f, Nx, Ny, Nz, dxy, dxy, z)
+
+
+# ----------------------------------------------------------- golden pins --
+def test_hankel_weights_match_reference_fixtures(oracle, golden):
+    rmin = float(golden["weight_rmin"])
+    for case, WTr in zip(golden["weight_cases"], golden["weights"]):
+        WT = oracle.vbt_weights(*case, rmin)
+        assert np.abs(WT - WTr).max() <= 1e-12 * np.abs(WTr).max()
+    assert np.allclose(oracle.vbt_lms(rmin), golden["lms"], rtol=1e-15, atol=0)
+
+
+def test_layered_kernel_matches_reference_fixtures(oracle, golden):
+    from tests.golden.make_golden import BKGS, LAMBDAS, W6, ZS
+    for bi, (sig, th, f) in enumerate(BKGS):
+        for zi, z in enumerate(ZS):
+            m = oracle.Model(csig(sig, f), th, 2 * np.pi * f, 4, 4, len(z) - 1, 100.0, 100.0, z)
+            for lm in LAMBDAS:
+                G = m.double_integral_U(lm, W6)
+                Gr = golden["U_%d_%d_%g" % (bi, zi, lm)]
+                assert np.abs(G - Gr).max() <= 1e-12 * np.abs(Gr).max(), (bi, zi, lm)
+
+
+def test_apq_matches_reference_fixtures(oracle, golden):
+    from tests.golden.make_golden import BKGS
+    for bi, (sig, th, f) in enumerate(BKGS):
+        m = oracle.Model(csig(sig, f), th, 2 * np.pi * f, 4, 4, 2, 100.0, 100.0, [0, 10, 20])
+        Arr, p, q, eta, A = m.calc_apq(0.0)
+        got = np.concatenate([Arr, p, q, eta, A])
+        ref = golden["apq_%d" % bi]
+        assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_fgmres_matches_reference_gfgmres(oracle, golden):
+    for ci in range(4):
+        A = golden["gm%d_A" % ci]
+        b = golden["gm%d_b" % ci]
+        N, mo, mi, maxit, tol = golden["gm%d_params" % ci]
+        it, st, bea, be, nmv = golden["gm%d_info" % ci]
+        x, info, _ = oracle.fgmres(lambda v: A @ v, b, b.copy(), int(mo), int(mi), int(maxit), tol)
+        assert info.iterations == int(it)
+        assert info.stat == int(st)
+        assert info.matvecs == int(nmv)
+        assert abs(info.bea - bea) <= 1e-6 * bea
+        assert rel(x, golden["gm%d_x" % ci]) < 1e-12
+
+
+@pytest.mark.skipif("not __import__('oracle.pyoracle', fromlist=['x']).ref_available()")
+def test_oracle_vs_reference_build_direct(oracle):
+    """Fresh random cases against oracle/_ref (only where it is built)."""
+    R = oracle.ref()
+    rng = np.random.default_rng(5)
+    for _ in range(5):
+        dx, dy = rng.uniform(20, 500, 2)
+        rmin = np.hypot(dx, dy) / 2
+        lx, ly = rng.integers(0, 200, 2)
+        x, y = lx * dx + dx / 2, ly * dy + dy / 2
+        WT = oracle.vbt_weights(x, y, dx, dy, rmin)
+        WTr = np.zeros(501 * 6)
+        lms = np.zeros(501)
+        R.ref_vbt_weights(rmin, x, y, dx, dy, WTr, lms)
+        WTr = WTr.reshape(6, 501).T
+        assert np.abs(WT - WTr).max() <= 1e-12 * np.abs(WTr).max()
+
+
+# ------------------------------------------------------ operator pins ----
+@pytest.mark.parametrize("Nx,Ny,Nz", [(4, 4, 2), (6, 4, 3), (8, 6, 4), (4, 8, 1)])
+def test_fft_operator_equals_direct_toeplitz_sum(oracle, Nx, Ny, Nz):
+    m = make_model(oracle, Nx, Ny, Nz, f=3.0)
+    rng = np.random.default_rng(Nx * 100 + Ny * 10 + Nz)
+    Gr = m.tensor_realspace(2)
+    Gs = m.tensor_fft(Gr, 2)
+    sa = 10 ** rng.uniform(-3, 0, Nx * Ny * Nz) - 1j * 6 * np.pi * EPS0
+    v = rng.standard_normal(3 * Nx * Ny * Nz) + 1j * rng.standard_normal(3 * Nx * Ny * Nz)
+    assert rel(m.apply(Gs, sa, v, 2), m.apply_bruteforce(Gr, sa, v, 2)) < 1e-13
+
+
+def test_operator_identity_when_anomaly_equals_background(oracle):
+    m = make_model(oracle, 8, 8, 4)
+    Gs = m.tensor_fft(m.tensor_realspace(2), 2)
+    ln = m.lnum()
+    sb = np.array([m.csigma[ln[i + 1]] for i in range(4)])
+    sa = np.repeat(sb, 64)
+    v = np.random.default_rng(0).standard_normal(3 * 256) + 0j
+    assert rel(m.apply(Gs, sa, v, 2), v) < 1e-15
+
+
+def test_convolution_matrix_is_complex_symmetric(oracle):
+    """Reciprocity of the Galerkin matrix (giem2g.pdf Eq. 20): recover the
+    convolution matrix from the operator by probing unit vectors."""
+    Nx, Ny, Nz = 4, 4, 3
+    m = make_model(oracle, Nx, Ny, Nz, f=2.0, dz=40.0)
+    Gr = m.tensor_realspace(2)
+    Gs = m.tensor_fft(Gr, 2)
+    rng = np.random.default_rng(3)
+    n = Nx * Ny * Nz
+    sa = 10 ** rng.uniform(-3, 0, n) - 1j * 4 * np.pi * EPS0
+    ln = m.lnum()
+    sb = np.array([m.csigma[ln[iz + 1]] for iz in range(Nz)])
+    sbc = np.repeat(sb, Nx * Ny)
+    sq = np.sqrt(sbc.real)
+    a = 2 * sq / (sa + np.conj(sbc))
+    g = (sa - sbc) * a
+    dz = np.repeat(np.diff(m.z), Nx * Ny)
+    Cm = np.zeros((3 * n, 3 * n), np.complex128)
+    for s in range(3 * n):
+        e = np.zeros(3 * n, np.complex128)
+        e[s] = 1.0
+        out = m.apply(Gs, sa, e, 2)
+        col = -(out / np.tile(sq, 3) - np.tile(a, 3) * e) * np.tile(dz, 3) / np.tile(g, 3)[s]
+        Cm[:, s] = col
+    assert np.linalg.norm(Cm - Cm.T) / np.linalg.norm(Cm) < 1e-12
+    assert np.linalg.norm(Cm - Cm.conj().T) / np.linalg.norm(Cm) > 1e-6  # symmetric, not Hermitian
+
+
+def test_tensor_odd_components_vanish_on_axes(oracle):
+    """xy is odd in x and y, xz odd in x, yz odd in y: their real-space
+    values vanish on the lx = 0 / ly = 0 axes (COMPUTE_WEIGHTS zeroing)."""
+    m = make_model(oracle, 4, 4, 3)
+    Nz, P = 3, 6
+    for lx, ly in [(0, 2), (3, 0), (0, 0)]:
+        blk = m.tensor_element(lx, ly)
+        xy = blk[P:2 * P]
+        xz = blk[4 * P:4 * P + Nz * Nz]
+        yz = blk[4 * P + Nz * Nz:]
+        assert np.all(xy == 0)
+        if lx == 0:
+            assert np.all(xz == 0)
+        if ly == 0:
+            assert np.all(yz == 0)
+        assert np.abs(blk[:P]).max() > 0
+
+
+def test_plane_wave_halfspace_closed_form(oracle):
+    f, s = 3.0, 0.2
+    w = 2 * np.pi * f
+    z = np.array([0.0, 30.0, 75.0, 200.0])
+    m = oracle.Model(csig([s], f), [], w, 4, 4, 3, 100.0, 100.0, z)
+    E = m.plane_wave(0).reshape(3, 3, 16)
+    k2 = 1j * w * MU0 * (s - 1j * w * EPS0)
+    eta = np.sqrt(-k2)
+    for iz in range(3):
+        avg = (np.exp(-eta * z[iz]) - np.exp(-eta * z[iz + 1])) / (eta * (z[iz + 1] - z[iz]))
+        assert np.allclose(E[0, iz], avg, rtol=1e-13, atol=0)
+    assert np.all(E[1] == 0) and np.all(E[2] == 0)
+
+
+# --------------------------------------------------------- solver pins ----
+def test_fgmres_solves_dense_systems(oracle):
+    rng = np.random.default_rng(11)
+    for N in (30, 90):
+        A = np.eye(N) * 2 + (rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))) / (2 * np.sqrt(N))
+        b = rng.standard_normal(N) + 1j * rng.standard_normal(N)
+        x, info, hist = oracle.fgmres(lambda v: A @ v, b, b.copy(), 10, 5, 50, 1e-12, hist_len=100)
+        assert info.stat == 1
+        assert rel(x, np.linalg.solve(A, b)) < 1e-10
+        assert np.all(np.diff(hist) <= 1e-14)  # Arnoldi residual is non-increasing
+
+
+def test_fgmres_zero_rhs_and_nonconvergence(oracle):
+    A = np.eye(10) + 0j
+    x, info, _ = oracle.fgmres(lambda v: A @ v, np.zeros(10, complex), np.ones(10, complex), 5, 5, 5, 1e-8)
+    assert info.stat == 1 and np.all(x == 0)
+    rng = np.random.default_rng(2)
+    A = rng.standard_normal((40, 40)) + 1j * rng.standard_normal((40, 40))
+    b = rng.standard_normal(40) + 0j
+    x, info, _ = oracle.fgmres(lambda v: A @ v, b, b.copy(), 2, 1, 2, 1e-14)
+    assert info.stat == -1 and info.iterations == 4
+
+
+def test_solve_converges_and_satisfies_equation(oracle):
+    m = make_model(oracle, 8, 8, 4, f=1.0)
+    Gs = m.tensor_fft(m.tensor_realspace(2), 2)
+    rng = np.random.default_rng(4)
+    sa = 10 ** rng.uniform(-3, 0, 256) - 1j * 2 * np.pi * EPS0
+    E = m.plane_wave(1)
+    Es, info = m.solve(Gs, sa, E, m_out=10, m_in=10, maxit=20, tol=1e-10, nthreads=2)
+    assert info.stat == 1 and info.be < 1e-10
+    # E = a x with A x = sqrt(Re sb) E_bkg
+    ln = m.lnum()
+    sb = np.repeat(np.array([m.csigma[ln[i + 1]] for i in range(4)]), 64)
+    a = np.tile(2 * np.sqrt(sb.real) / (sa + np.conj(sb)), 3)
+    x = Es / a
+    r = m.apply(Gs, sa, x, 2) - np.tile(np.sqrt(sb.real), 3) * E
+    assert np.linalg.norm(r) / np.linalg.norm(np.tile(np.sqrt(sb.real), 3) * E) < 1e-10
