@@ -1,0 +1,193 @@
+"""Benchmark of the MI355X IE solve (one GPU).
+
+Headline workload (C1 of the benchmark set): the FP64 integral-equation
+operator apply A*v on a 128x128x32 grid (100 m cells, 25 m layers, layered
+background 0.01/0.1/1 S/m, f = 10 Hz), complex128 currents re/im ~ N(0,1).
+`value` is matvecs per second over K timed applications with inputs
+resident in HBM.  The JSON line also carries the roofline of the dominant
+kernel (the tensor contraction), the oracle's CPU throughput on the same
+workload, and (unless --no-solve) the time to a 1e-8 FP64 true residual of
+the 256x256x32 random-box model for the plain FP64 solve and the mixed
+complex64-inner / FP64-outer solve.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(Nx, Ny, Nz, s=16):
+    """Bytes per matvec by kernel: n_c cells, n_v = 3 n_c, n_s = 12 Nx Ny Nz
+    spectrum, n_G = Nx Ny 2Nz(2Nz+1) tensor entries; s = element size."""
+    nc = Nx * Ny * Nz
+    nv = 3 * nc
+    ns = 4 * Nx * Ny * 3 * Nz
+    nG = Nx * Ny * 2 * Nz * (2 * Nz + 1)
+    return {"fwd": s * (nv + nc + ns), "contraction": s * (nG + 2 * ns),
+            "bwd": s * (ns + 2 * nv + nc), "total": s * (nG + 4 * ns + 3 * nv + 2 * nc)}
+
+
+def read_traffic(path):
+    """HBM bytes per contraction launch measured with rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy-kernel calibration)."""
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch"):
+                return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(m, Gspec, v, budget_s=12.0):
+    """The oracle's FP64 operator (C restatement of the reference algorithm)
+    on the host cores, same inputs, bounded sample of whole matvecs."""
+    from oracle import pyoracle as po
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    om = po.Model(m.csigb(), m.thick, m.omega, m.Nx, m.Ny, m.Nz, m.dx, m.dy, m.z)
+    t0 = time.perf_counter()
+    om.apply(Gspec, m.csiga(), v, nthreads)
+    t1 = time.perf_counter() - t0
+    reps = max(1, min(50, int(budget_s / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = om.apply(Gspec, m.csiga(), v, nthreads)
+    dt = (time.perf_counter() - t0) / reps
+    split = om.apply_timings()
+    return {"value": 1.0 / dt, "unit": "matvec/s", "cores": nthreads, "kind": "port",
+            "sample": "%d FP64 matvecs of the 128x128x32 operator (oracle/giem2g_oracle.c, "
+                      "OpenMP, radix-2 FFT), split fwd/contraction/bwd = %.3f/%.3f/%.3f s"
+                      % (reps, split[0], split[1], split[2])}, out
+
+
+def solve_bench(torch, gop, models):
+    """Time to a 1e-8 FP64 true residual on the 256x256x32 random-box model
+    (0.1 Hz, EY polarization): plain FP64 nested FGMRES vs complex64 inner."""
+    m = models.c2()
+    t0 = time.perf_counter()
+    op = gop.IEOperator.from_model(m)
+    op.set_background(m.csigb(), m.thick, m.omega)
+    op.build_tensor()
+    op.set_sigma(m.csiga())
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    st = op.stats()
+    E = torch.from_numpy(op.plane_wave(1)).cuda()
+    res = {"model": "256x256x32 x 25 m, 20 random boxes 1e-3..10 S/m in 0.01 S/m host, f=0.1 Hz, EY",
+           "tensor_build_s": st["t_tensor"], "tensor_fft_s": st["t_tensor_fft"], "setup_s": t_setup}
+    sols = {}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        Es, info = op.solve(E, outer_m=30, inner_m=30, max_restart=100, tol=1e-8,
+                            inner_precision=prec)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        sols[name] = Es
+        res[name] = {"time_to_1e-8_s": dt, "outer_iterations": info["outer_iterations"],
+                     "matvecs_fp64": info["matvecs_fp64"], "matvecs_fp32": info["matvecs_fp32"],
+                     "true_residual": info["true_residual"], "status": info["status"]}
+        log("solve", name, res[name])
+    d = torch.linalg.vector_norm(sols["mixed"] - sols["fp64"]) / torch.linalg.vector_norm(sols["fp64"])
+    res["mixed_vs_fp64_rel"] = float(d)
+    res["speedup_mixed"] = res["fp64"]["time_to_1e-8_s"] / res["mixed"]["time_to_1e-8_s"]
+    op.close()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the operator would shard)", file=sys.stderr)
+        return 2
+    import torch
+    from giem2g_amd import models, operator as gop
+    assert torch.cuda.is_available(), "no GPU"
+    m = models.c1()
+    t0 = time.perf_counter()
+    op = gop.operator_for(m)
+    torch.cuda.synchronize()
+    log("setup %.3f s" % (time.perf_counter() - t0), op.stats())
+    v = torch.from_numpy(models.c1_currents(m)).cuda()
+    out = torch.empty_like(v)
+    for _ in range(a.warmup):
+        op.apply_dev(v, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        op.apply_dev(v, out)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per = 1e3 * dt / a.steps
+    value = a.steps / dt
+    # per-kernel times with HIP events on the operator's stream
+    stage_ms = op.time_apply(v, out, reps=max(10, min(100, a.steps)))
+    v32 = v.to(torch.complex64)
+    o32 = torch.empty_like(v32)
+    stage32 = op.time_apply(v32, o32, reps=max(10, min(100, a.steps)), precision=1)
+    bytes64 = algorithmic_bytes(m.Nx, m.Ny, m.Nz, 16)
+    bytes32 = algorithmic_bytes(m.Nx, m.Ny, m.Nz, 8)
+    ach = bytes64["contraction"] / (stage_ms[1] * 1e-3) / 1e9
+    traffic = read_traffic(os.path.join(ROOT, "profiles", "contraction_c1_fp64_traffic.txt"))
+    line = {
+        "metric": "time-to-1e-8 residual + IE matvecs/s: mixed+learned vs plain FP64, 1 GPU",
+        "value": value, "unit": "matvec/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms_per, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded PCG64 model and currents)",
+        "config": {"workload": "C1 IE operator apply, FP64: 128x128x32 cells of 100x100x25 m, "
+                               "background 0.01/0.1/1 S/m, f=10 Hz, complex128 currents N(0,1)",
+                   "grid": [m.Nx, m.Ny, m.Nz], "precision": "complex128"},
+        "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_contract<double> (tensor contraction)",
+                     "algorithmic_bytes_per_launch": bytes64["contraction"],
+                     "avg_launch_ms": float(stage_ms[1])},
+        "kernels_ms": {"fwd_fft": float(stage_ms[0]), "contraction": float(stage_ms[1]),
+                       "bwd_fft": float(stage_ms[2])},
+        "kernels_gbs": {k: bytes64[k2] / (t * 1e-3) / 1e9 for k, k2, t in
+                        (("fwd_fft", "fwd", stage_ms[0]), ("contraction", "contraction", stage_ms[1]),
+                         ("bwd_fft", "bwd", stage_ms[2]))},
+        "matvec_total_gbs": bytes64["total"] / (ms_per * 1e-3) / 1e9,
+        "roofline_bound_matvec_s": HBM_PEAK_GBS * 1e9 / bytes64["total"],
+        "complex64": {"kernels_ms": [float(x) for x in stage32],
+                      "matvec_s": 1e3 / float(np.sum(stage32)),
+                      "contraction_gbs": bytes32["contraction"] / (stage32[1] * 1e-3) / 1e9},
+    }
+    if not a.no_cpu:
+        Gs = op.get_tensor()
+        cb, ref = cpu_baseline(m, Gs, models.c1_currents(m))
+        res = out.cpu().numpy()
+        cb["gpu_vs_cpu_rel"] = float(np.linalg.norm(res - ref) / np.linalg.norm(ref))
+        line["cpu_baseline"] = cb
+    else:
+        line["cpu_baseline"] = None
+    op.close()
+    if not a.no_solve:
+        line["solve"] = solve_bench(torch, gop, models)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -34,6 +34,67 @@ LAMBDAS = [1e-5, 3e-4, 1e-2, 0.3]
 W6 = np.array([0.3, -0.2, 0.5, 1.1, -0.7, 0.9])
 
 
+TENSOR_OFFSETS = np.array([[0, 0], [1, 0], [0, 3], [5, 7], [31, 31], [12, 2]])
+TENSOR_MODELS = [([0.01, 0.1, 1.0], [1000.0, 9000.0], 1.0, np.arange(9) * 50.0, "c0"),
+                 ([0.01, 0.1, 1.0], [1000.0, 9000.0], 0.3, 900.0 + np.arange(9) * 25.0, "iface")]
+
+
+def ref_tensor_element(R, sig, th, f, z, lx, ly, dx=100.0, dy=100.0):
+    """Block of the real-space tensor from the reference's Fortran routines."""
+    Nz = len(z) - 1
+    P = Nz * (Nz + 1) // 2
+    cs = np.ascontiguousarray(csig(sig, f).view(np.float64))
+    thd = np.array(list(th) + [0.0])
+    rmin = np.hypot(dx, dy) / 2
+    WT0 = np.zeros(501 * 6)
+    lms = np.zeros(501)
+    R.ref_vbt_weights(rmin, lx * dx + dx / 2, ly * dy + dy / 2, dx, dy, WT0, lms)
+    WT0 = WT0.reshape(6, 501).T  # D0, DXX, DXY, DX, DYY, DY
+    WT = np.stack([WT0[:, 1], WT0[:, 2], WT0[:, 4], WT0[:, 3], WT0[:, 5], WT0[:, 0]], 1)
+    W0 = np.zeros(6)
+    W0[0] = np.abs(WT[:, 0]).max()
+    if lx != 0 and ly != 0:
+        W0[1:5] = np.abs(WT[:, 1:5]).max()
+    elif lx == 0:
+        WT[:, 1] = 0
+        W0[1] = -W0[0]
+        W0[2] = np.abs(WT[:, 2]).max()
+        WT[:, 3] = 0
+        W0[3] = -W0[0]
+        if ly != 0:
+            W0[4] = np.abs(WT[:, 4]).max()
+        else:
+            WT[:, 4] = 0
+            W0[4] = -W0[0]
+    else:
+        WT[:, 1] = 0
+        W0[1] = -W0[0]
+        W0[2] = np.abs(WT[:, 2]).max()
+        W0[3] = np.abs(WT[:, 3]).max()
+        WT[:, 4] = 0
+        W0[4] = -W0[0]
+    W0[5] = np.abs(WT[:, 5]).max()
+    G = np.zeros(6 * Nz * Nz, np.complex128)
+    for K in range(501):
+        if np.max(np.abs(WT[K] / W0)) <= 1e-12:
+            continue
+        lm = lms[K]
+        W = WT[K].copy()
+        W[[0, 1, 2]] /= lm
+        W[[3, 4, 5]] *= lm
+        R.ref_double_integral_u(len(sig), cs, thd, 2 * np.pi * f, Nz, np.ascontiguousarray(z), lm,
+                                W, G)
+    G = G.reshape(Nz, Nz, 6) / np.pi / 4  # [j][i][comp]
+    blk = np.zeros(4 * P + 2 * Nz * Nz, np.complex128)
+    for j in range(Nz):
+        for i in range(j + 1):
+            pi = i + j * (j + 1) // 2
+            blk[pi], blk[P + pi], blk[2 * P + pi], blk[3 * P + pi] = G[j, i, 0], G[j, i, 1], G[j, i, 3], G[j, i, 5]
+    blk[4 * P:4 * P + Nz * Nz] = G[:, :, 2].reshape(-1)
+    blk[4 * P + Nz * Nz:] = G[:, :, 4].reshape(-1)
+    return blk
+
+
 def csig(sig, f):
     w = 2 * np.pi * f
     return np.array([1e-12 - 1j * w * EPS0] + [s - 1j * w * EPS0 for s in sig])
@@ -91,6 +152,13 @@ def main():
         out["gm%d_params" % ci] = np.array([N, mo, mi, maxit, tol])
         out["gm%d_x" % ci] = xr
         out["gm%d_info" % ci] = np.array([it.value, st.value, bea.value, be.value, nmv.value])
+    # tensor elements composed from the reference's own weights and kernel
+    # (COMPUTE_WEIGHTS masking and the tap sum restated here, 20 lines of
+    # calc_ie_tensor_module.F90:257-288)
+    out["tensor_offsets"] = TENSOR_OFFSETS
+    for sig, th, f, z, tag in TENSOR_MODELS:
+        for lx, ly in TENSOR_OFFSETS:
+            out["tensor_%s_%d_%d" % (tag, lx, ly)] = ref_tensor_element(R, sig, th, f, z, lx, ly)
     path = os.path.join(HERE, "reference_vectors.npz")
     np.savez_compressed(path, **out)
     print("wrote", path, len(out), "arrays")

@@ -78,6 +78,20 @@ def test_fgmres_matches_reference_gfgmres(oracle, golden):
         assert rel(x, golden["gm%d_x" % ci]) < 1e-12
 
 
+def test_tensor_element_matches_reference_composition(oracle, golden):
+    """Real-space tensor blocks against blocks composed from the reference's
+    own weights and layered kernel.  Far offsets sum taps with cancellation
+    up to 1e7, so the bound is 1e-7 per block and 1e-10 of the self block."""
+    from tests.golden.make_golden import TENSOR_MODELS
+    for sig, th, f, z, tag in TENSOR_MODELS:
+        m = oracle.Model(csig(sig, f), th, 2 * np.pi * f, 32, 32, len(z) - 1, 100.0, 100.0, z)
+        b00 = golden["tensor_%s_0_0" % tag]
+        for lx, ly in golden["tensor_offsets"]:
+            ref = golden["tensor_%s_%d_%d" % (tag, lx, ly)]
+            d = np.linalg.norm(m.tensor_element(int(lx), int(ly)) - ref)
+            assert d < 1e-7 * np.linalg.norm(ref) and d < 1e-10 * np.linalg.norm(b00), (tag, lx, ly)
+
+
 @pytest.mark.skipif("not __import__('oracle.pyoracle', fromlist=['x']).ref_available()")
 def test_oracle_vs_reference_build_direct(oracle):
     """Fresh random cases against oracle/_ref (only where it is built)."""
