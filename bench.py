@@ -116,6 +116,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 1 GiB device copy (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -124,6 +126,13 @@ def main():
     import torch
     from giem2g_amd import models, operator as gop
     assert torch.cuda.is_available(), "no GPU"
+    if a.copy_calibration:
+        src = torch.empty(1 << 27, dtype=torch.complex64, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)  # reads and writes exactly 1 GiB
+        torch.cuda.synchronize()
+        del src, dst
     m = models.c1()
     t0 = time.perf_counter()
     op = gop.operator_for(m)
