@@ -8,6 +8,7 @@
 #include "../../include/giem2g_amd.h"
 #include "g2g_op.hpp"
 #include "g2g_solver.hpp"
+#include "g2g_unet.hpp"
 
 using namespace g2g;
 using zc = std::complex<double>;
@@ -42,7 +43,24 @@ struct Handle {
   Operator op;
   std::vector<double> hist;
   PrecondHook learned;
+  UNet* net = nullptr;  // attached learned preconditioner (not owned)
+  LearnedCtx lctx;
 };
+
+void learned_hook(void* ctx, const zd* v, zd* z) {
+  LearnedCtx* c = static_cast<LearnedCtx*>(ctx);
+  learned_apply(*c, v, z, c->op->stream);
+}
+
+UNet* NET(void* p) {
+  G2G_CHECK(p != nullptr, -2, "null network handle");
+  return static_cast<UNet*>(p);
+}
+
+void prepare_learned(Handle& h) {
+  G2G_CHECK(h.net != nullptr, -4, "no learned preconditioner attached (giem2g_op_set_learned)");
+  learned_prepare(h.lctx, h.op, *h.net);
+}
 
 Handle* H(void* p) {
   G2G_CHECK(p != nullptr, -2, "null operator handle");
@@ -299,6 +317,7 @@ void solve_dev(Handle& h, const zd* Ebkg, const zd* E0, const giem2g_solver_para
     solve_scale(op, rhs, Ebkg, 0);  // rhs = sqrt(Re sb) E_bkg (ie_solver_module.F90:64-66)
     if (E0) solve_scale(op, guess, E0, 1);
     else G2G_HIP(hipMemcpyAsync(guess, rhs, sizeof(zd) * N, hipMemcpyDeviceToDevice, op.stream));
+    if (p.precond != 0) prepare_learned(h);
     r = nested_solve(op, rhs, guess, Esol, p, &h.hist, h.learned.apply ? &h.learned : nullptr);
     solve_scale(op, Esol, Esol, 2);  // E = asiga x (:85-97)
     G2G_HIP(hipEventRecord(b, op.stream));
@@ -352,6 +371,7 @@ int giem2g_op_create(int Nx, int Ny, int Nz, double dx, double dy, const double*
 int giem2g_op_destroy(void* p) {
   return guard([&] {
     Handle* h = H(p);
+    learned_free(h->lctx);
     op_fini(h->op);
     delete h;
   });
@@ -516,6 +536,146 @@ int giem2g_last_history(void* p, double* out, int maxlen, int* len) {
     const int n = (int)h->hist.size();
     for (int i = 0; i < n && i < maxlen; i++) out[i] = h->hist[i];
     *len = n;
+  });
+}
+
+int giem2g_op_scale(void* p, int mode, const void* in_dev, void* out_dev) {
+  return guard([&] {
+    Handle* h = H(p);
+    G2G_CHECK(h->op.have_sigma, -4, "conductivity not set");
+    G2G_CHECK(mode >= 0 && mode <= 2, -3, "mode must be 0, 1 or 2");
+    solve_scale(h->op, (zd*)out_dev, (const zd*)in_dev, mode);
+    G2G_HIP(hipStreamSynchronize(h->op.stream));
+  });
+}
+
+// ------------------------------------------------ learned preconditioner ---
+int giem2g_unet_create(void** net) {
+  return guard([&] {
+    G2G_CHECK(net, -2, "null output pointer");
+    std::unique_ptr<UNet> n(new UNet());
+    unet_init(*n, 0);
+    *net = n.release();
+  });
+}
+
+int giem2g_unet_destroy(void* net) {
+  return guard([&] {
+    UNet* n = NET(net);
+    unet_free(*n);
+    delete n;
+  });
+}
+
+int giem2g_unet_nparams(long long* n) {
+  return guard([&] {
+    G2G_CHECK(n, -2, "null output pointer");
+    *n = unet_param_count();
+  });
+}
+
+int giem2g_unet_set_params(void* net, const float* host) {
+  return guard([&] {
+    UNet* n = NET(net);
+    G2G_HIP(hipMemcpy(n->params, host, sizeof(float) * n->nparams, hipMemcpyHostToDevice));
+    G2G_HIP(hipMemset(n->m1, 0, sizeof(float) * n->nparams));
+    G2G_HIP(hipMemset(n->m2, 0, sizeof(float) * n->nparams));
+    n->adam_t = 0;
+    n->wdirty = true;
+  });
+}
+
+int giem2g_unet_get_params(void* net, float* host) {
+  return guard([&] {
+    UNet* n = NET(net);
+    G2G_HIP(hipStreamSynchronize(n->stream));
+    G2G_HIP(hipMemcpy(host, n->params, sizeof(float) * n->nparams, hipMemcpyDeviceToHost));
+  });
+}
+
+int giem2g_unet_get_grads(void* net, float* host) {
+  return guard([&] {
+    UNet* n = NET(net);
+    G2G_HIP(hipStreamSynchronize(n->stream));
+    G2G_HIP(hipMemcpy(host, n->grads, sizeof(float) * n->nparams, hipMemcpyDeviceToHost));
+  });
+}
+
+int giem2g_unet_forward_dev(void* net, const float* x_dev, float* y_dev, int B, int nx, int ny, int nz,
+                            int keep_for_backward) {
+  return guard([&] {
+    UNet* n = NET(net);
+    unet_reserve(*n, B, nx, ny, nz, keep_for_backward != 0);
+    unet_load_input(*n, x_dev, n->stream);
+    unet_forward(*n, n->stream);
+    if (y_dev)
+      G2G_HIP(hipMemcpyAsync(y_dev, n->out, sizeof(float) * 6 * (size_t)B * nx * ny * nz,
+                             hipMemcpyDeviceToDevice, n->stream));
+    G2G_HIP(hipStreamSynchronize(n->stream));
+  });
+}
+
+int giem2g_unet_backward_dev(void* net, const float* dy_dev) {
+  return guard([&] {
+    UNet* n = NET(net);
+    G2G_CHECK(n->training, -4, "no forward pass kept for backward");
+    const size_t nv = (size_t)n->B * n->nx * n->ny * n->nz;
+    // dout rows are 8 wide (channels 6, 7 zero)
+    G2G_HIP(hipMemsetAsync(n->dout, 0, sizeof(float) * 8 * nv, n->stream));
+    G2G_HIP(hipMemcpy2DAsync(n->dout, 8 * sizeof(float), dy_dev, 6 * sizeof(float), 6 * sizeof(float), nv,
+                             hipMemcpyDeviceToDevice, n->stream));
+    unet_backward(*n, n->stream);
+    G2G_HIP(hipStreamSynchronize(n->stream));
+  });
+}
+
+int giem2g_unet_train_step(void* net, const float* x_dev, const float* t_dev, int B, int nx, int ny, int nz,
+                           float lr, double* loss) {
+  return guard([&] {
+    UNet* n = NET(net);
+    unet_reserve(*n, B, nx, ny, nz, true);
+    unet_load_input(*n, x_dev, n->stream);
+    unet_forward(*n, n->stream);
+    const double l = unet_mse(*n, t_dev, n->stream);
+    unet_backward(*n, n->stream);
+    unet_adam(*n, lr, n->stream);
+    G2G_HIP(hipStreamSynchronize(n->stream));
+    if (loss) *loss = l;
+  });
+}
+
+int giem2g_op_set_learned(void* p, void* net, double accept) {
+  return guard([&] {
+    Handle* h = H(p);
+    if (!net) {
+      h->net = nullptr;
+      h->learned = PrecondHook();
+      learned_free(h->lctx);
+      return;
+    }
+    G2G_CHECK(accept > 0, -3, "accept threshold must be positive");
+    h->net = NET(net);
+    h->learned.apply = learned_hook;
+    h->learned.ctx = &h->lctx;
+    h->learned.accept = accept;
+  });
+}
+
+int giem2g_op_learned_features(void* p, const void* r_dev, const void* e_dev, float* x_dev, float* t_dev) {
+  return guard([&] {
+    Handle* h = H(p);
+    prepare_learned(*h);
+    learned_features(h->lctx, (const zd*)r_dev, (const zd*)e_dev, x_dev, t_dev, h->op.stream);
+    G2G_HIP(hipStreamSynchronize(h->op.stream));
+  });
+}
+
+int giem2g_op_learned_apply(void* p, const void* v_dev, void* z_dev) {
+  return guard([&] {
+    Handle* h = H(p);
+    prepare_learned(*h);
+    learned_apply(h->lctx, (const zd*)v_dev, (zd*)z_dev, h->op.stream);
+    G2G_HIP(hipStreamSynchronize(h->op.stream));
   });
 }
 

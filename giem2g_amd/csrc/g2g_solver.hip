@@ -281,6 +281,10 @@ template <typename R> struct Gmres {
   int iternum = 0;
   Apply A;
   Apply precond;  // right preconditioner (flex only); identity when empty
+  // optional check of a variable preconditioner: accept_step(v, A z) false
+  // replaces z by fallback(v) before the Gram-Schmidt step
+  std::function<bool(const T2*, const T2*)> accept_step;
+  Apply fallback;
   DevBlas* blas = nullptr;
   int64_t* matvecs = nullptr;
   std::vector<double>* hist = nullptr;
@@ -362,6 +366,12 @@ template <typename R> struct Gmres {
       const int j = jH - 1;
       right_precond(Vc(j), w, j);
       apply(w, r0);
+      if (accept_step && !accept_step(Vc(j), r0)) {
+        // the variable preconditioner's step is rejected: redo it with the fallback
+        fallback(Vc(j), w);
+        if (flex) blas->copy(Z + (int64_t)j * N, w, N);
+        apply(w, r0);
+      }
       std::swap(w, r0);
       for (int i = 0; i < jH; i++) HH(i, j) = 0;
       int northo = 0;
@@ -473,8 +483,8 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
   outer.A = [&](const zd* in, zd* out) { op_apply<double>(op, op.d64, in, out, op.stream, false); };
   Gmres<double> in64;
   Gmres<float> in32;
-  zf *b32 = nullptr, *x32 = nullptr;
-  zd* tmp = nullptr;
+  zf *b32 = nullptr, *x32 = nullptr, *r32 = nullptr;
+  zd *tmp = nullptr, *tmp2 = nullptr;
   SolveResult res;
   try {
     if (p.inner_precision == 0) {
@@ -494,8 +504,10 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
       in32.A = [&](const zf* in, zf* out) { op_apply<float>(op, op.d32, in, out, op.stream, false); };
       G2G_HIP(hipMalloc(&b32, sizeof(zf) * N));
       G2G_HIP(hipMalloc(&x32, sizeof(zf) * N));
+      if (p.precond == 1) G2G_HIP(hipMalloc(&r32, sizeof(zf) * N));
     }
     G2G_HIP(hipMalloc(&tmp, sizeof(zd) * N));
+    if (p.precond == 1 && p.inner_precision == 0) G2G_HIP(hipMalloc(&tmp2, sizeof(zd) * N));
     // FGMRES_BASED_PRECONDITIONER (seq_fgmres.F90:216-226): inner GMRES with x0 = b = v
     auto inner_gmres = [&](const zd* v, zd* z) {
       if (p.inner_precision == 0) {
@@ -506,28 +518,61 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
         blas.convert<zf, zd>(z, x32, N);
       }
     };
-    if (learned && learned->apply && p.precond == 1) {
-      // learned correction c = M(v); accept z = c + inner GMRES on the
-      // remaining residual only when it cuts ||v - A c||, else plain inner.
+    G2G_CHECK(p.precond == 0 || (learned && learned->apply), -4,
+              "learned preconditioner requested but none is attached");
+    if (p.precond == 1) {
+      // learned start + inner GMRES: c = M(v); accept it when it cuts the
+      // residual, ||v - A c|| < accept ||v|| (in the inner precision), then
+      // z = c + inner GMRES on v - A c; otherwise z = plain inner GMRES(v).
       outer.precond = [&](const zd* v, zd* z) {
         res.precond_calls++;
         learned->apply(learned->ctx, v, z);
-        op_apply<double>(op, op.d64, z, tmp, op.stream, false);
-        op.cnt.matvecs++;
-        blas.bminus(tmp, v, N);  // tmp = v - A c
-        const double rn = blas.norm<zd, double>(tmp, N), vn = blas.norm<zd, double>(v, N);
-        if (!(rn < learned->accept * vn)) {
-          res.precond_fallbacks++;
-          inner_gmres(v, z);
-          return;
+        if (p.inner_precision == 0) {
+          op_apply<double>(op, op.d64, z, tmp, op.stream, false);
+          op.cnt.matvecs++;
+          blas.bminus(tmp, v, N);  // tmp = v - A c
+          const double rn = blas.norm<zd, double>(tmp, N), vn = blas.norm<zd, double>(v, N);
+          if (!(rn < learned->accept * vn)) {
+            res.precond_fallbacks++;
+            in64.solve(z, v, v);
+            return;
+          }
+          in64.solve(tmp2, tmp, tmp);
+          blas.add(z, tmp2, N);
+        } else {
+          blas.convert<zd, zf>(b32, v, N);
+          blas.convert<zd, zf>(x32, z, N);
+          op_apply<float>(op, op.d32, x32, r32, op.stream, false);
+          op.cnt.matvecs32++;
+          blas.bminus(r32, b32, N);  // r32 = v - A c
+          const double rn = blas.norm<zf, float>(r32, N), vn = blas.norm<zf, float>(b32, N);
+          if (!(rn < learned->accept * vn)) {
+            res.precond_fallbacks++;
+            in32.solve(x32, b32, b32);
+            blas.convert<zf, zd>(z, x32, N);
+            return;
+          }
+          in32.solve(x32, r32, r32);
+          blas.convert<zf, zd>(tmp, x32, N);
+          blas.add(z, tmp, N);
         }
-        zd* c = z;
-        zd* zz;
-        G2G_HIP(hipMalloc(&zz, sizeof(zd) * N));
-        inner_gmres(tmp, zz);
-        blas.add(c, zz, N);
-        (void)hipFree(zz);
       };
+    } else if (p.precond == 2) {
+      // learned step only, z = M(v); the outer checks ||v - A z|| < accept ||v||
+      // with the A z it computes anyway and falls back to inner GMRES(v).
+      outer.precond = [&](const zd* v, zd* z) {
+        res.precond_calls++;
+        learned->apply(learned->ctx, v, z);
+      };
+      outer.accept_step = [&](const zd* v, const zd* Az) {
+        blas.copy(tmp, Az, N);
+        blas.bminus(tmp, v, N);
+        const double rn = blas.norm<zd, double>(tmp, N), vn = blas.norm<zd, double>(v, N);
+        if (rn < learned->accept * vn) return true;
+        res.precond_fallbacks++;
+        return false;
+      };
+      outer.fallback = inner_gmres;
     } else {
       outer.precond = inner_gmres;
     }
@@ -537,16 +582,14 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
     res = r;
   } catch (...) {
     outer.fini(); in64.fini(); in32.fini(); blas.fini();
-    if (b32) (void)hipFree(b32);
-    if (x32) (void)hipFree(x32);
-    if (tmp) (void)hipFree(tmp);
+    for (void* q : {(void*)b32, (void*)x32, (void*)r32, (void*)tmp, (void*)tmp2})
+      if (q) (void)hipFree(q);
     throw;
   }
   G2G_HIP(hipStreamSynchronize(op.stream));
   outer.fini(); in64.fini(); in32.fini(); blas.fini();
-  if (b32) (void)hipFree(b32);
-  if (x32) (void)hipFree(x32);
-  if (tmp) (void)hipFree(tmp);
+  for (void* q : {(void*)b32, (void*)x32, (void*)r32, (void*)tmp, (void*)tmp2})
+    if (q) (void)hipFree(q);
   return res;
 }
 

@@ -23,7 +23,11 @@ EXTENDED_SYMBOLS = [
     "giem2g_op_set_sigma_dev", "giem2g_op_apply_dev", "giem2g_op_apply_dev_f32",
     "giem2g_op_apply_host", "giem2g_op_stream", "giem2g_op_sync", "giem2g_op_stats",
     "giem2g_op_time_apply", "giem2g_plane_wave", "giem2g_solve", "giem2g_solve_dev",
-    "giem2g_last_history",
+    "giem2g_last_history", "giem2g_unet_create", "giem2g_unet_destroy", "giem2g_unet_nparams",
+    "giem2g_unet_set_params", "giem2g_unet_get_params", "giem2g_unet_get_grads",
+    "giem2g_unet_forward_dev", "giem2g_unet_backward_dev", "giem2g_unet_train_step",
+    "giem2g_op_set_learned", "giem2g_op_learned_features", "giem2g_op_learned_apply",
+    "giem2g_op_scale",
 ]
 
 
@@ -110,6 +114,19 @@ def load():
         "giem2g_solve": (i, [vp, vp, vp, C.POINTER(SolverParams), vp, C.POINTER(SolveInfo)]),
         "giem2g_solve_dev": (i, [vp, vp, vp, C.POINTER(SolverParams), vp, C.POINTER(SolveInfo)]),
         "giem2g_last_history": (i, [vp, vp, i, C.POINTER(i)]),
+        "giem2g_unet_create": (i, [C.POINTER(vp)]),
+        "giem2g_unet_destroy": (i, [vp]),
+        "giem2g_unet_nparams": (i, [C.POINTER(C.c_longlong)]),
+        "giem2g_unet_set_params": (i, [vp, vp]),
+        "giem2g_unet_get_params": (i, [vp, vp]),
+        "giem2g_unet_get_grads": (i, [vp, vp]),
+        "giem2g_unet_forward_dev": (i, [vp, vp, vp, i, i, i, i, i]),
+        "giem2g_unet_backward_dev": (i, [vp, vp]),
+        "giem2g_unet_train_step": (i, [vp, vp, vp, i, i, i, i, C.c_float, C.POINTER(d)]),
+        "giem2g_op_set_learned": (i, [vp, vp, d]),
+        "giem2g_op_learned_features": (i, [vp, vp, vp, vp, vp]),
+        "giem2g_op_learned_apply": (i, [vp, vp, vp]),
+        "giem2g_op_scale": (i, [vp, i, vp, vp]),
         "giem2g_calc_data_sizes": (None, [Anomaly, C.POINTER(Data)]),
         "giem2g_prepare_ie_kernel": (None, [Anomaly, Data]),
         "giem2g_calc_ie_kernel": (None, [vp, Background, Anomaly, d]),

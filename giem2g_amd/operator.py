@@ -150,6 +150,25 @@ class IEOperator:
                                               C.byref(info)))
         return out, info.as_dict()
 
+    def scale(self, mode, vin, vout):
+        """mode 0: vout = sqrt(Re sb) vin (rhs), 1: vin / a (field -> x),
+        2: a vin (x -> field); torch complex128 tensors on the GPU."""
+        L.check(self.lib.giem2g_op_scale(self.h, mode, _ptr(vin), _ptr(vout)))
+
+    def set_learned(self, net, accept=1.0):
+        """Attach a learned.UNet as the right preconditioner of precond modes
+        1 and 2 (None detaches)."""
+        self._net = net  # keep the network alive while attached
+        L.check(self.lib.giem2g_op_set_learned(self.h, None if net is None else net.h, float(accept)))
+
+    def learned_features(self, r, e, x, t=None):
+        """Network input x [vox][8] of residual r and target t [vox][6] of
+        correction e (torch tensors on the GPU)."""
+        L.check(self.lib.giem2g_op_learned_features(self.h, _ptr(r), _ptr(e), _ptr(x), _ptr(t)))
+
+    def learned_apply(self, v, z):
+        L.check(self.lib.giem2g_op_learned_apply(self.h, _ptr(v), _ptr(z)))
+
     def history(self, maxlen=100000):
         buf = np.zeros(maxlen)
         n = C.c_int()

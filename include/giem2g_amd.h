@@ -162,10 +162,14 @@ int giem2g_plane_wave(void *ie_op, int pol, double *E_host);
  * tolerance on the relative residual of the scaled system.
  *   inner_precision: 0 = FP64 inner (reference semantics), 1 = complex64
  *                    inner operator and Krylov basis (FP32 accumulation)
- *   precond:         0 = inner GMRES (reference), 1 = learned
- *                    preconditioner then inner GMRES on its residual,
- *                    falling back to plain inner GMRES when it does not cut
- *                    the residual (requires giem2g_set_learned_precond). */
+ *   precond:         0 = inner GMRES (reference semantics);
+ *                    1 = learned start: c = M(v), accepted when
+ *                        ||v - A c|| < accept*||v|| (inner precision), then
+ *                        z = c + inner GMRES on v - A c; else plain inner GMRES;
+ *                    2 = learned step only: z = M(v), accepted when the
+ *                        outer's own A z gives ||v - A z|| < accept*||v||,
+ *                        else z = inner GMRES(v).
+ *                    Modes 1 and 2 need giem2g_op_set_learned. */
 typedef struct {
   int outer_m;
   int inner_m;
@@ -197,8 +201,49 @@ int giem2g_solve(void *ie_op, const double *E_bkg, const double *E0,
 int giem2g_solve_dev(void *ie_op, const void *E_bkg_dev, const void *E0_dev,
                      const giem2g_solver_params *p, void *E_sol_dev,
                      giem2g_solve_info *info);
+/* Scalings between the field and the scaled system (ie_solver_module.F90:
+ * 64-66, 85-97) on complex128 HBM vectors: mode 0 out = sqrt(Re sb)*in (the
+ * right-hand side from E_bkg), 1 out = in/a (a field to the unknown x),
+ * 2 out = a*in (x to the field), a = 2 sqrt(Re sb)/(sa + conj sb). */
+int giem2g_op_scale(void *ie_op, int mode, const void *in_dev, void *out_dev);
+
 /* Residual history of the last solve (outer Arnoldi residuals). */
 int giem2g_last_history(void *ie_op, double *out, int maxlen, int *len);
+
+/* ---- learned right preconditioner (no reference counterpart: the
+ * reference preconditions only with the inner GMRES of
+ * GFGMRES/seq_fgmres.F90:142-152) ----------------------------------------
+ * A 3-level 3D U-Net (widths 16/32/64, 3x3x3 convolutions, BF16 weights and
+ * activations on MFMA, FP32 accumulation).  Volumes are channels-last FP32
+ * arrays [B][Nz][Ny][Nx][C] in HBM: input C = 8 (residual re/im of x, y, z
+ * divided by its RMS, log10(Re sa / Re sb), 0), output C = 6 (a*e/RMS,
+ * re/im of x, y, z).  Nx, Ny, Nz must be multiples of 4. */
+int giem2g_unet_create(void **net);
+int giem2g_unet_destroy(void *net);
+int giem2g_unet_nparams(long long *n);
+/* FP32 parameters, layer by layer W[cout][27 or 1][cin] then b[cout];
+ * setting them resets the Adam state. */
+int giem2g_unet_set_params(void *net, const float *host);
+int giem2g_unet_get_params(void *net, float *host);
+int giem2g_unet_get_grads(void *net, float *host);
+/* y = net(x); keep_for_backward keeps the activations for one backward. */
+int giem2g_unet_forward_dev(void *net, const float *x_dev, float *y_dev, int B, int nx,
+                            int ny, int nz, int keep_for_backward);
+/* Parameter gradient of sum(dy * y) for the last kept forward. */
+int giem2g_unet_backward_dev(void *net, const float *dy_dev);
+/* Forward, MSE loss against t [B][vox][6], backward and one Adam step. */
+int giem2g_unet_train_step(void *net, const float *x_dev, const float *t_dev, int B, int nx,
+                           int ny, int nz, float lr, double *loss);
+/* Attach a network to an operator (net NULL detaches); accept is the
+ * residual-reduction threshold of precond modes 1 and 2. */
+int giem2g_op_set_learned(void *ie_op, void *net, double accept);
+/* Network input x [vox][8] of a residual r and, when e is not NULL, the
+ * training target t [vox][6] of the correction e (complex128 HBM vectors of
+ * the operator's layout). */
+int giem2g_op_learned_features(void *ie_op, const void *r_dev, const void *e_dev, float *x_dev,
+                               float *t_dev);
+/* z = M(v) on complex128 HBM vectors. */
+int giem2g_op_learned_apply(void *ie_op, const void *v_dev, void *z_dev);
 
 #ifdef __cplusplus
 }
