@@ -61,12 +61,49 @@ def build(jobs=4, verbose=False, extra=()):
     return LIB
 
 
+FSRC = os.path.join(HERE, "fortran")
+HOST = os.path.join(LIBDIR, "giem2g_amd_host")
+
+
+def fortran_compiler():
+    for c in (shutil.which("amdflang"), "/opt/rocm/bin/amdflang", shutil.which("flang")):
+        if c and os.path.exists(c):
+            return c
+    return None
+
+
+def build_fortran(verbose=False):
+    """Fortran driver giem2g_amd/_lib/giem2g_amd_host over the C ABI (ISO_C_BINDING);
+    returns its path, or None when no Fortran compiler is installed."""
+    fc = fortran_compiler()
+    if fc is None:
+        return None
+    objdir = os.path.join(LIBDIR, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for f in ("g2g_json.f90", "giem2g_amd_api.f90", "giem2g_amd_host.f90"):
+        obj = os.path.join(objdir, f[:-4] + ".o")
+        cmd = [fc, "-O2", "-module-dir", objdir, "-I", objdir, "-c", os.path.join(FSRC, f), "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("Fortran compile failed for %s:\n%s" % (f, r.stderr[-4000:]))
+        objs.append(obj)
+    cmd = [fc, "-o", HOST, *objs, "-L" + LIBDIR, "-lgiem2g_amd", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("Fortran link failed:\n" + r.stderr[-4000:])
+    return HOST
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", type=int, default=4)
     ap.add_argument("--verbose", action="store_true")
     a = ap.parse_args(argv)
     print(build(a.j, a.verbose))
+    print(build_fortran(a.verbose))
 
 
 if __name__ == "__main__":

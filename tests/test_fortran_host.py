@@ -1,0 +1,125 @@
+"""The Fortran driver giem2g_amd/_lib/giem2g_amd_host (ISO_C_BINDING over
+the C ABI): deck parsing on the CPU, and on the GPU the multi-frequency,
+two-polarization solve with warm starts and SOL_P{X,Y}_F*.bin checkpoints
+(giem2g.F90:323-507, mpi_saveload_module.F90:568-675) against the Python
+API and the oracle.
+
+The deck fixture tests/golden/reference_deck_giem2g_input.json is the
+reference's own test deck (test/giem2g_input.json), kept as data; its
+conductivity file is not part of the reference, so the GPU test writes its
+own deck and conductivity file.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests.conftest import rel
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+HOST = os.path.join(ROOT, "giem2g_amd", "_lib", "giem2g_amd_host")
+
+
+def _host():
+    if not os.path.exists(HOST):
+        pytest.skip("Fortran driver not built (no Fortran compiler)")
+    return HOST
+
+
+def test_deck_parse_reference_deck():
+    out = subprocess.run([_host(), "--check", os.path.join(HERE, "golden", "reference_deck_giem2g_input.json")],
+                         capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stdout + out.stderr
+    lines = {l.split()[0]: l.split()[1:] for l in out.stdout.splitlines() if l.strip()}
+    assert lines["Nl"] == ["3"]
+    assert np.allclose([float(v) for v in lines["Conductivity"]], [0.1, 0.01, 10.0])
+    assert np.allclose([float(v) for v in lines["Thickness"]], [10000, 20000])
+    assert lines["Grid"][:3] == ["128", "128", "50"]
+    assert np.isclose(float(lines["Grid"][3]), 312.5)
+    assert np.allclose([float(v) for v in lines["zborders"]], [0.0, 10000.0])
+    assert np.allclose([float(v) for v in lines["Frequencies"]], [1e-3, 1e-2, 1e-1, 1.0])
+    assert lines["FGMRES"][:3] == ["15", "15", "300"] and float(lines["FGMRES"][3]) == 1e-8
+    assert lines["SigmaFile"] == ["128_128_501.dat"]
+
+
+def test_deck_parse_errors(tmp_path):
+    p = tmp_path / "bad.json"
+    p.write_text('{"Background": {"Conductivity": [0.01, 0.1], "Thickness": []}}')
+    out = subprocess.run([_host(), "--check", str(p)], capture_output=True, text=True, timeout=60)
+    assert out.returncode != 0 and "Thickness" in out.stdout
+
+
+def write_case(d, m, freqs, outer=10, inner=10):
+    # list-directed text in Fortran (Nz, Nx, Ny) order: z fastest, then x, then y
+    sig = np.transpose(m.siga, (1, 2, 0)).reshape(-1)  # siga is (Nz, Ny, Nx), C order
+    np.savetxt(os.path.join(d, "sigma.dat"), sig, fmt="%.17e")
+    deck = {"Save Solution": True, "Solve Equation": True, "Recalculate Fields": False,
+            "Background": {"Conductivity": list(m.sigma_bkg), "Permittivity": [],
+                           "Thickness": list(m.thick)},
+            "Anomaly": {"Nx": m.Nx, "Ny": m.Ny, "Nz": m.Nz, "dx": m.dx, "dy": m.dy,
+                        "zborders": [float(v) for v in m.z], "Conductivity": "sigma.dat"},
+            "Frequencies": list(freqs),
+            "FGMRES": {"OuterBufferSize": outer, "InnerBufferSize": inner, "MaxRestart": 50,
+                       "Tolerance": 1e-8}}
+    with open(os.path.join(d, "deck.json"), "w") as f:
+        json.dump(deck, f, indent=1)
+
+
+def read_sol(path, m):
+    a = np.fromfile(path, dtype=np.complex128)
+    f4 = a.reshape(m.Ny, m.Nx, 3, m.Nz)  # Fortran (z,c,x,y) as C (y,x,c,z)
+    return np.transpose(f4, (2, 3, 0, 1)).reshape(-1)  # -> (c,z,y,x) = Fortran (x,y,z,c)
+
+
+@pytest.mark.gpu
+def test_fortran_host_multifrequency(tmp_path, oracle):
+    from giem2g_amd import models, operator as gop
+    host = _host()
+    m = models.small(16, 16, 4, seed=21, freq=1.0)
+    freqs = [0.5, 2.0]
+    write_case(str(tmp_path), m, freqs)
+    print("host run 1", flush=True)
+    r = subprocess.run([host, "deck.json"], cwd=str(tmp_path), capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    runs = [l for l in r.stdout.splitlines() if l.startswith("freq")]
+    assert len(runs) == 4 and all(" status   1" in l for l in runs), runs
+    sols = {}
+    for f in freqs:
+        # WRITE (fnum, '(F22.10)') with blanks of the first 11 columns -> '0'
+        fnum = "".join("0" if (i < 11 and c == " ") else c for i, c in enumerate("%22.10f" % f)).strip()
+        for p in ("PY", "PX"):
+            path = tmp_path / ("SOL_%s_F%s.bin" % (p, fnum))
+            assert path.exists(), path
+            sols[(f, p)] = read_sol(str(path), m)
+    # first frequency: cold start, same as the Python API and the oracle
+    m.freq = freqs[0]
+    op = gop.operator_for(m)
+    for p, pol in (("PY", 1), ("PX", 0)):
+        E = op.plane_wave(pol)
+        Es, info = op.solve(E, outer_m=10, inner_m=10, max_restart=50, tol=1e-8)
+        assert rel(sols[(freqs[0], p)], Es) < 1e-12
+        om = oracle.Model(m.csigb(), m.thick, m.omega, m.Nx, m.Ny, m.Nz, m.dx, m.dy, m.z)
+        Gs = om.tensor_fft(om.tensor_realspace())
+        Eo, _ = om.solve(Gs, m.csiga(), E, m_out=10, m_in=10, maxit=50, tol=1e-8)
+        assert rel(sols[(freqs[0], p)], Eo) < 1e-6
+    op.close()
+    # second frequency: warm start from the first frequency's solution
+    m.freq = freqs[1]
+    op = gop.operator_for(m)
+    for p, pol in (("PY", 1), ("PX", 0)):
+        E = op.plane_wave(pol)
+        Es, info = op.solve(E, E0=sols[(freqs[0], p)], outer_m=10, inner_m=10, max_restart=50, tol=1e-8)
+        assert rel(sols[(freqs[1], p)], Es) < 1e-12
+    op.close()
+    # rerun: every solve starts from its saved checkpoint and converges at once
+    print("host run 2", flush=True)
+    r2 = subprocess.run([host, "deck.json"], cwd=str(tmp_path), capture_output=True, text=True, timeout=300)
+    assert r2.returncode == 0, r2.stdout + r2.stderr
+    assert r2.stdout.count("Load IE Solution from") == 4
+    its = [int(l.split("outer")[1].split()[0]) for l in r2.stdout.splitlines() if l.startswith("freq")]
+    its1 = [int(l.split("outer")[1].split()[0]) for l in runs]
+    assert all(a <= 1 for a in its) and sum(its) < sum(its1), (its, its1)
