@@ -353,6 +353,218 @@ __global__ __launch_bounds__(NT) void k_contract(cx_t<R>* __restrict__ S,
 #undef G2G_STASH
 }
 
+// ------------------------------------------------------ K3 on MFMA ----
+// Same product as k_contract, as matrix-core tiles.  With the point signs
+// s (for -kx) and m (for -ky), every point uses the same 3Nz x 3Nz block
+// G0 = [[Sxx, Sxy, Axz], [Sxy, Syy, Ayz], [-Axz^T, -Ayz^T, Szz]]:
+//   out_p = D_p G0 (D_p v_p),  D_p = diag(s, m, 1) per component
+// (the sign pattern of VERTICAL_MULT_GENERAL, apply_ie_operator_module.F90:
+// 276-292).  The four points form the columns of a real 16-column operand
+// [Re v | Im v] (Gr pass) and [-Im v | Re v] (Gi pass) so that columns 0-3
+// accumulate Re(G0 v_p) and columns 4-7 Im(G0 v_p) in one
+// v_mfma_{f64,f32}_16x16x4 tile; columns 8-15 carry a second right-hand
+// side (NRHS = 2: both polarizations read the tensor once) or repeat the
+// first.  Lane l supplies
+// A[row l%16][k l/16] and B[k l/16][col l%16] and receives rows
+// 4(l/16)..4(l/16)+3 of column l%16.  Wave w owns TPW consecutive 16-row
+// tiles of the 3Nz output rows.  The six tensor components stream through
+// two LDS buffers (symmetric ones unpacked to full Nz x Nz), with two
+// components in flight in registers while the matrix cores work on a third.
+template <typename R> struct v4t;
+template <> struct v4t<double> { typedef double t __attribute__((ext_vector_type(4))); };
+template <> struct v4t<float> { typedef float t __attribute__((ext_vector_type(4))); };
+__device__ inline v4t<double>::t mma4(double a, double b, v4t<double>::t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ inline v4t<float>::t mma4(float a, float b, v4t<float>::t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <typename R, int NW, int PF, int NRHS>
+__global__ __launch_bounds__(64 * NW) void k_contract_mma(cx_t<R>* __restrict__ S0, cx_t<R>* __restrict__ S1,
+                                                         const cx_t<R>* __restrict__ G, Geo g,
+                                                         int64_t BS) {
+  using T2 = cx_t<R>;
+  using V4 = typename v4t<R>::t;
+  constexpr int NTH = 64 * NW;
+  static_assert(PF <= 16, "PF must be <= 16");
+  static_assert(NRHS == 1 || NRHS == 2, "one or two right-hand sides");
+  extern __shared__ unsigned char smem_raw[];
+  const int Nz = g.Nz, Nz3 = g.Nz3, NZP = Nz + 1;
+  const int P = Nz * (Nz + 1) / 2;
+  const int GB = Nz * NZP;
+  T2* vin = reinterpret_cast<T2*>(smem_raw);  // [NRHS][4][3Nz]
+  T2* gbuf = vin + NRHS * 4 * Nz3;            // [2][Nz x NZP], element (i,j) at j*NZP + i
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int b = blockIdx.x, kx = b % g.Nx, ky = b / g.Nx;
+  const int Nx2 = 2 * g.Nx, Ny2 = 2 * g.Ny;
+  const int gNx = g.Nx, gNy = g.Ny;
+#define PX(p) (((p) & 1) ? (kx == 0 ? gNx : Nx2 - kx) : kx)
+#define PY(p) (((p) & 2) ? (ky == 0 ? gNy : Ny2 - ky) : ky)
+#define VAL(p) ((!((p) & 1) || kx > 0) && (!((p) & 2) || ky > 0))
+  const T2* blk = G + (int64_t)b * BS;
+  const T2 zero = mk<T2>(0, 0);
+#define G2G_RDECL(q) T2 ra##q = zero, rb##q = zero;
+  G2G_FOR16(G2G_RDECL)
+#undef G2G_RDECL
+#define G2G_CLEN(c) ((c) < 4 ? P : Nz * Nz)
+#define G2G_CSRC(c) ((c) < 4 ? blk + (c) * P : blk + 4 * P + ((c) - 4) * Nz * Nz)
+#define G2G_LDA(q) if ((q) < PF && tid + (q) * NTH < n_) ra##q = src_[tid + (q) * NTH];
+#define G2G_LDB(q) if ((q) < PF && tid + (q) * NTH < n_) rb##q = src_[tid + (q) * NTH];
+#define G2G_PREF(c, SET)            \
+  {                                 \
+    const T2* src_ = G2G_CSRC(c);   \
+    const int n_ = G2G_CLEN(c);     \
+    G2G_FOR16(G2G_LD##SET)          \
+  }
+  // stash with the packed-symmetric components unpacked to full matrices
+#define G2G_STX(q, r)                                                   \
+  if ((q) < PF && tid + (q) * NTH < n_) {                               \
+    const int e = tid + (q) * NTH;                                      \
+    if (c_ < 4) {                                                       \
+      int j = (int)((sqrtf(8.0f * e + 1.0f) - 1.0f) * 0.5f);            \
+      if ((j + 1) * (j + 2) / 2 <= e) j++;                              \
+      if (j * (j + 1) / 2 > e) j--;                                     \
+      const int i = e - j * (j + 1) / 2;                                \
+      dst_[j * NZP + i] = r##q;                                         \
+      dst_[i * NZP + j] = r##q;                                         \
+    } else {                                                            \
+      dst_[(e / Nz) * NZP + e % Nz] = r##q;                             \
+    }                                                                   \
+  }
+#define G2G_STA(q) G2G_STX(q, ra)
+#define G2G_STB(q) G2G_STX(q, rb)
+#define G2G_STASH(c, SET, dst) \
+  {                            \
+    const int c_ = (c);        \
+    T2* dst_ = (dst);          \
+    const int n_ = G2G_CLEN(c_); \
+    G2G_FOR16(G2G_ST##SET)     \
+  }
+  G2G_PREF(0, A)
+  G2G_PREF(1, B)
+  for (int i = tid; i < NRHS * 4 * Nz3; i += NTH) {
+    const int rp = i / Nz3, e = i - rp * Nz3, p = rp & 3;
+    const T2* Sr = rp < 4 ? S0 : S1;
+    T2 vv = zero;
+    if (VAL(p)) vv = Sr[((int64_t)PX(p) * Ny2 + PY(p)) * Nz3 + e];
+    vin[i] = vv;
+  }
+  G2G_STASH(0, A, gbuf)
+  __syncthreads();
+  // per-lane operand roles
+  const int col = lane & 15, kq = lane >> 4, rr = lane & 15;
+  // column = (rhs, re/im, point): cols 4*part + pnt, part = 2*rhs + (0 re | 1 im);
+  // with one right-hand side columns 8-15 repeat columns 0-7 and are not stored
+  const int pnt = col & 3, part = col >> 2, rhs = (part >> 1) & (NRHS - 1);
+  const R sgx = (pnt & 1) ? R(-1) : R(1), sgy = (pnt & 2) ? R(-1) : R(1);
+  const int ntile = Nz3 / 16;
+  const int tpw = (ntile + NW - 1) / NW;
+  V4 acc0 = {0, 0, 0, 0}, acc1 = acc0, acc2 = acc0;
+  // one component's MFMA work for the wave's tiles
+#define G2G_TILE(k)                                                                    \
+  if ((k) < tpw) {                                                                     \
+    const int t = wave * tpw + (k);                                                    \
+    if (t < ntile) {                                                                   \
+      const int co = (16 * t) / Nz, i0 = (16 * t) % Nz;                                \
+      int ci = -1;                                                                     \
+      bool tr = false;                                                                 \
+      if (comp == 0 && co == 0) ci = 0;                                                \
+      if (comp == 1 && co == 0) ci = 1;                                                \
+      if (comp == 1 && co == 1) ci = 0;                                                \
+      if (comp == 2 && co == 1) ci = 1;                                                \
+      if (comp == 3 && co == 2) ci = 2;                                                \
+      if (comp == 4 && co == 0) ci = 2;                                                \
+      if (comp == 4 && co == 2) { ci = 0; tr = true; }                                 \
+      if (comp == 5 && co == 1) ci = 2;                                                \
+      if (comp == 5 && co == 2) { ci = 1; tr = true; }                                 \
+      if (ci >= 0) {                                                                   \
+        const R sv = ci == 0 ? sgx : (ci == 1 ? sgy : R(1));                           \
+        const R sa = tr ? R(-1) : R(1);                                                \
+        const T2* vp = vin + (rhs * 4 + pnt) * Nz3 + ci * Nz;                         \
+        const int i = i0 + rr;                                                         \
+        for (int j0 = 0; j0 < Nz; j0 += 4) {                                           \
+          const int j = j0 + kq;                                                       \
+          const T2 a = tr ? cur[i * NZP + j] : cur[j * NZP + i];                       \
+          const T2 v = vp[j];                                                          \
+          const R vr = v.x * sv, vi = v.y * sv;                                        \
+          R br = vr, bi = -vi;                                                         \
+          if (part & 1) { br = vi; bi = vr; }                                          \
+          acc##k = mma4(a.x * sa, br, acc##k);                                         \
+          acc##k = mma4(a.y * sa, bi, acc##k);                                         \
+        }                                                                              \
+      }                                                                                \
+    }                                                                                  \
+  }
+#define G2G_COMPUTE(c, buf)   \
+  {                           \
+    const int comp = (c);     \
+    const T2* cur = (buf);    \
+    G2G_TILE(0) G2G_TILE(1) G2G_TILE(2) \
+  }
+  T2* buf0 = gbuf;
+  T2* buf1 = gbuf + GB;
+  G2G_PREF(2, A)
+  G2G_COMPUTE(0, buf0)
+  G2G_STASH(1, B, buf1)
+  __syncthreads();
+  G2G_PREF(3, B)
+  G2G_COMPUTE(1, buf1)
+  G2G_STASH(2, A, buf0)
+  __syncthreads();
+  G2G_PREF(4, A)
+  G2G_COMPUTE(2, buf0)
+  G2G_STASH(3, B, buf1)
+  __syncthreads();
+  G2G_PREF(5, B)
+  G2G_COMPUTE(3, buf1)
+  G2G_STASH(4, A, buf0)
+  __syncthreads();
+  G2G_COMPUTE(4, buf0)
+  G2G_STASH(5, B, buf1)
+  __syncthreads();
+  G2G_COMPUTE(5, buf1)
+  __syncthreads();
+  // results -> LDS (reusing vin) with the output signs, then coalesced stores
+  R* vo = reinterpret_cast<R*>(vin);
+#define G2G_OUTT(k)                                                         \
+  if ((k) < tpw && (part >> 1) < NRHS) {                                    \
+    const int t = wave * tpw + (k);                                         \
+    if (t < ntile) {                                                        \
+      const int co = (16 * t) / Nz, i0 = (16 * t) % Nz;                     \
+      const R so = co == 0 ? sgx : (co == 1 ? sgy : R(1));                  \
+      for (int q = 0; q < 4; q++) {                                         \
+        const int row = co * Nz + i0 + 4 * kq + q;                          \
+        vo[2 * (((part >> 1) * 4 + pnt) * Nz3 + row) + (part & 1)] = acc##k[q] * so; \
+      }                                                                     \
+    }                                                                       \
+  }
+  G2G_OUTT(0) G2G_OUTT(1) G2G_OUTT(2)
+  __syncthreads();
+  for (int i = tid; i < NRHS * 4 * Nz3; i += NTH) {
+    const int rp = i / Nz3, e = i - rp * Nz3, p = rp & 3;
+    T2* Sr = rp < 4 ? S0 : S1;
+    T2 o = vin[i];
+    if (!VAL(p)) o = zero;
+    Sr[((int64_t)PX(p) * Ny2 + PY(p)) * Nz3 + e] = o;
+  }
+#undef G2G_OUTT
+#undef G2G_TILE
+#undef G2G_COMPUTE
+#undef G2G_LDA
+#undef G2G_LDB
+#undef G2G_PREF
+#undef G2G_STX
+#undef G2G_STA
+#undef G2G_STB
+#undef G2G_STASH
+#undef G2G_CLEN
+#undef G2G_CSRC
+#undef PX
+#undef PY
+#undef VAL
+}
+
 // ------------------------------------------------------------ dispatch ---
 template <typename R, template <typename, int> class KF> struct Dummy {};
 
@@ -410,7 +622,7 @@ template <typename R> void op_free_buffers(OpBuffers<R>& b) {
     if (p) (void)hipFree((void*)p);
     p = nullptr;
   };
-  F(b.G); F(b.X); F(b.S); F(b.gsig); F(b.asiga); F(b.sq); F(b.dzn); F(b.twx); F(b.twy);
+  F(b.G); F(b.X); F(b.S); F(b.S2); F(b.gsig); F(b.asiga); F(b.sq); F(b.dzn); F(b.twx); F(b.twy);
 }
 
 template <typename R>
@@ -445,8 +657,23 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
       }
       hipLaunchKernelGGL(kern, dim3(op.Nx * op.Ny), dim3(NT), sm, s, b.S, b.G, g, op.BS, split_log);
     };
-    if (op.Nz * op.Nz <= 4 * NT) launch(k_contract<R, 4>);
-    else launch(k_contract<R, 16>);
+    const bool mma = op.contraction != 1 && op.Nz % 16 == 0;
+    if (mma) {
+      auto launch_mma = [&](auto kern, int nth) {
+        if (sm > 65536) {
+          G2G_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      160 * 1024));
+        }
+        hipLaunchKernelGGL(kern, dim3(op.Nx * op.Ny), dim3(nth), sm, s, b.S, b.S, b.G, g, op.BS);
+      };
+      if (op.Nz <= 32) launch_mma(k_contract_mma<R, 3, 6, 1>, 192);
+      else if (op.Nz == 48) launch_mma(k_contract_mma<R, 3, 16, 1>, 192);
+      else launch_mma(k_contract_mma<R, 4, 16, 1>, 256);
+    } else if (op.Nz * op.Nz <= 4 * NT) {
+      launch(k_contract<R, 4>);
+    } else {
+      launch(k_contract<R, 16>);
+    }
   }
   if (timed) G2G_HIP(hipEventRecord(op.ev[2], s));
   pow2_dispatch(Ny2, [&](auto LC) {
@@ -462,6 +689,66 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
   if (timed) G2G_HIP(hipEventRecord(op.ev[3], s));
   G2G_HIP(hipGetLastError());
 }
+
+// Two right-hand sides through one tensor read: both vectors are
+// transformed into their own spectrum buffers, one two-column-set MFMA
+// contraction serves both, then both are transformed back.  Needs
+// Nz % 16 == 0 (matrix-core contraction).
+template <typename R>
+void op_apply2(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin0, const cx_t<R>* vin1, cx_t<R>* vout0,
+               cx_t<R>* vout1, hipStream_t s) {
+  G2G_CHECK(op.have_tensor && op.have_sigma, -4, "operator not ready: tensor or conductivity missing");
+  G2G_CHECK(op.Nz % 16 == 0, -3, "two-vector apply needs Nz % 16 == 0");
+  const int Nx2 = 2 * op.Nx, Ny2 = 2 * op.Ny, Nz3 = 3 * op.Nz;
+  if (!b.S2) {
+    G2G_HIP(hipMalloc((void**)&b.S2, sizeof(cx_t<R>) * (size_t)Nx2 * Ny2 * Nz3));
+    b.bytes += sizeof(cx_t<R>) * (size_t)Nx2 * Ny2 * Nz3;
+  }
+  Geo g = make_geo(op, b.NR);
+  const int nry = nr_rt<R>(Ny2);
+  const int chy = (Nz3 + nry - 1) / nry;
+  const cx_t<R>* vins[2] = {vin0, vin1};
+  cx_t<R>* vouts[2] = {vout0, vout1};
+  cx_t<R>* Ss[2] = {b.S, b.S2};
+  for (int r = 0; r < 2; r++) {
+    pow2_dispatch(Nx2, [&](auto LC) {
+      constexpr int LL = decltype(LC)::value;
+      hipLaunchKernelGGL((k_fwd_x<R, LL>), dim3(op.Ny, b.nchunk), dim3(NT), 0, s, vins[r], b.gsig, b.X, b.twx, g);
+    });
+    pow2_dispatch(Ny2, [&](auto LC) {
+      constexpr int LL = decltype(LC)::value;
+      hipLaunchKernelGGL((k_fwd_y<R, LL>), dim3(Nx2, chy), dim3(NT), 0, s, b.X, Ss[r], b.twy, g);
+    });
+  }
+  {
+    const size_t sm = sizeof(cx_t<R>) * (8 * (size_t)Nz3 + 2 * (size_t)op.Nz * (op.Nz + 1));
+    auto launch_mma = [&](auto kern, int nth) {
+      if (sm > 65536) {
+        G2G_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    160 * 1024));
+      }
+      hipLaunchKernelGGL(kern, dim3(op.Nx * op.Ny), dim3(nth), sm, s, b.S, b.S2, b.G, g, op.BS);
+    };
+    G2G_CHECK(sm <= 160 * 1024, -3, "two-vector contraction does not fit in LDS for this Nz");
+    if (op.Nz <= 32) launch_mma(k_contract_mma<R, 3, 6, 2>, 192);
+    else if (op.Nz == 48) launch_mma(k_contract_mma<R, 3, 16, 2>, 192);
+    else launch_mma(k_contract_mma<R, 4, 16, 2>, 256);
+  }
+  for (int r = 0; r < 2; r++) {
+    pow2_dispatch(Ny2, [&](auto LC) {
+      constexpr int LL = decltype(LC)::value;
+      hipLaunchKernelGGL((k_inv_y<R, LL>), dim3(Nx2, chy), dim3(NT), 0, s, Ss[r], b.X, b.twy, g);
+    });
+    pow2_dispatch(Nx2, [&](auto LC) {
+      constexpr int LL = decltype(LC)::value;
+      hipLaunchKernelGGL((k_inv_x<R, LL>), dim3(op.Ny, b.nchunk), dim3(NT), 0, s, b.X, vins[r], b.asiga, b.sq,
+                         b.dzn, vouts[r], b.twx, g);
+    });
+  }
+  G2G_HIP(hipGetLastError());
+}
+template void op_apply2<double>(Operator&, OpBuffers<double>&, const zd*, const zd*, zd*, zd*, hipStream_t);
+template void op_apply2<float>(Operator&, OpBuffers<float>&, const zf*, const zf*, zf*, zf*, hipStream_t);
 
 // ------------------------------------------------------ conductivity ---
 template <typename R>

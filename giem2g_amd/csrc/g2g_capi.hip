@@ -506,6 +506,70 @@ int giem2g_solve_dev(void* p, const void* Ebkg, const void* E0, const giem2g_sol
   });
 }
 
+int giem2g_solve_pair_dev(void* p, const void* Ebkg0, const void* Ebkg1, const void* E00, const void* E01,
+                          const giem2g_solver_params* pp, void* Esol0, void* Esol1, giem2g_solve_info* info2,
+                          long long* paired) {
+  return guard([&] {
+    Handle& h = *H(p);
+    Operator& op = h.op;
+    G2G_CHECK(op.have_tensor && op.have_sigma, -4, "operator not ready");
+    G2G_CHECK(Ebkg0 && Ebkg1 && Esol0 && Esol1, -2, "null vector");
+    const SolveParams prm = params_from(pp);
+    const int64_t N = op.nvec();
+    zd* work = nullptr;
+    G2G_HIP(hipMalloc(&work, sizeof(zd) * N * 4));
+    zd* rhs[2] = {work, work + N};
+    zd* guess[2] = {work + 2 * N, work + 3 * N};
+    const zd* eb[2] = {(const zd*)Ebkg0, (const zd*)Ebkg1};
+    const zd* e0[2] = {(const zd*)E00, (const zd*)E01};
+    zd* es[2] = {(zd*)Esol0, (zd*)Esol1};
+    hipEvent_t a, b;
+    G2G_HIP(hipEventCreate(&a));
+    G2G_HIP(hipEventCreate(&b));
+    PairResult r;
+    try {
+      G2G_HIP(hipEventRecord(a, op.stream));
+      for (int k = 0; k < 2; k++) {
+        solve_scale(op, rhs[k], eb[k], 0);
+        if (e0[k]) solve_scale(op, guess[k], e0[k], 1);
+        else G2G_HIP(hipMemcpyAsync(guess[k], rhs[k], sizeof(zd) * N, hipMemcpyDeviceToDevice, op.stream));
+      }
+      const zd* const cb[2] = {rhs[0], rhs[1]};
+      const zd* const cg[2] = {guess[0], guess[1]};
+      r = nested_solve_pair(op, cb, cg, es, prm);
+      for (int k = 0; k < 2; k++) solve_scale(op, es[k], es[k], 2);
+      G2G_HIP(hipEventRecord(b, op.stream));
+      G2G_HIP(hipEventSynchronize(b));
+    } catch (...) {
+      (void)hipFree(work);
+      throw;
+    }
+    float ms = 0;
+    G2G_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipFree(work);
+    if (info2) {
+      for (int k = 0; k < 2; k++) {
+        giem2g_solve_info& in = info2[k];
+        in.outer_iterations = r.r[k].iterations;
+        in.status = r.r[k].stat;
+        in.arnoldi_residual = r.r[k].bea;
+        in.true_residual = r.r[k].be;
+        in.matvecs_fp64 = r.mv64[k];
+        in.matvecs_fp32 = r.mv32[k];
+        in.precond_calls = 0;
+        in.precond_fallbacks = 0;
+        in.seconds = ms * 1e-3;
+      }
+    }
+    if (paired) {
+      paired[0] = r.paired;
+      paired[1] = r.alone;
+    }
+  });
+}
+
 int giem2g_solve(void* p, const double* Ebkg, const double* E0, const giem2g_solver_params* prm,
                  double* Esol, giem2g_solve_info* info) {
   return guard([&] {
@@ -536,6 +600,15 @@ int giem2g_last_history(void* p, double* out, int maxlen, int* len) {
     const int n = (int)h->hist.size();
     for (int i = 0; i < n && i < maxlen; i++) out[i] = h->hist[i];
     *len = n;
+  });
+}
+
+int giem2g_op_set_contraction(void* p, int mode) {
+  return guard([&] {
+    Handle* h = H(p);
+    G2G_CHECK(mode >= 0 && mode <= 2, -3, "contraction mode must be 0, 1 or 2");
+    G2G_CHECK(mode != 2 || h->op.Nz % 16 == 0, -3, "the matrix-core contraction needs Nz % 16 == 0");
+    h->op.contraction = mode;
   });
 }
 

@@ -28,6 +28,7 @@ template <typename R> struct OpBuffers {
   T2* G = nullptr;      // spectral tensor, Nx*Ny*BS
   T2* X = nullptr;      // nchunk*Ny*2Nx*NR
   T2* S = nullptr;      // 2Nx*2Ny*3Nz
+  T2* S2 = nullptr;     // second spectrum for two-vector applies (allocated on first use)
   T2* gsig = nullptr;   // Nx*Ny*Nz
   T2* asiga = nullptr;  // Nx*Ny*Nz
   R* sq = nullptr;      // Nz: sqrt(Re sigma_b)
@@ -56,6 +57,7 @@ struct Operator {
   OpBuffers<double> d64;
   OpBuffers<float> d32;
   hipStream_t stream = nullptr;
+  int contraction = 0;  // 0 automatic (matrix cores when Nz % 16 == 0), 1 vector ALU, 2 matrix cores
   Counters cnt;
   // tensor-build scratch (frequency independent weights are cached)
   double* wts = nullptr;  // 6 * 501 * Nx*Ny masked, lambda-scaled weights
@@ -74,6 +76,9 @@ template <typename R> void op_free_buffers(OpBuffers<R>& b);
 template <typename R>
 void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, hipStream_t s,
               bool timed);
+template <typename R>
+void op_apply2(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin0, const cx_t<R>* vin1, cx_t<R>* vout0,
+               cx_t<R>* vout1, hipStream_t s);
 void op_set_sigma(Operator& op, const zd* d_csiga);
 void op_make_g32(Operator& op);
 // tensor build (g2g_tensor.hip)

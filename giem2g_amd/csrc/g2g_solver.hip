@@ -17,6 +17,11 @@
 #include <vector>
 
 #include "g2g_op.hpp"
+#include <condition_variable>
+#include <exception>
+#include <mutex>
+#include <thread>
+
 #include "g2g_solver.hpp"
 
 namespace g2g {
@@ -466,21 +471,37 @@ template <typename R> struct Gmres {
 };
 
 // ------------------------------------------------------- nested solve ---
+// res = v - alpha w with the least-squares alpha = <w, v>/<w, w>; returns
+// ||res|| (res may alias w)
+template <typename T2, typename A>
+static double fit_residual(DevBlas& blas, const T2* v, const T2* w, T2* res, int64_t N, zc* alpha) {
+  zc t, ww;
+  blas.mdot<T2, A>(w, N, 1, v, &t);
+  blas.mdot<T2, A>(w, N, 1, w, &ww);
+  *alpha = ww.real() > 0 ? t / ww.real() : zc(0);
+  blas.lincomb(res, w, N, 1, alpha, false);
+  blas.bminus(res, v, N);
+  return blas.norm<T2, A>(res, N);
+}
+
 SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
                          const SolveParams& p, std::vector<double>* hist,
-                         const PrecondHook* learned) {
+                         const PrecondHook* learned, const SolveEnv* env) {
   const int64_t N = op.nvec();
+  const hipStream_t st = env ? env->stream : op.stream;
+  G2G_CHECK(!env || p.precond == 0, -3, "the learned preconditioner runs only in single solves");
   DevBlas blas;
-  blas.init(op.stream);
-  blas.dotcount = &op.cnt.dotprods;
+  blas.init(st);
+  blas.dotcount = env ? nullptr : &op.cnt.dotprods;
   Gmres<double> outer;
   outer.blas = &blas;
   outer.maxit = p.max_restart;
   outer.tol = p.tol;
   outer.alloc(N, p.outer_m, true);
-  outer.matvecs = &op.cnt.matvecs;
+  outer.matvecs = env ? env->mv64 : &op.cnt.matvecs;
   outer.hist = hist;
-  outer.A = [&](const zd* in, zd* out) { op_apply<double>(op, op.d64, in, out, op.stream, false); };
+  if (env) outer.A = env->A64;
+  else outer.A = [&](const zd* in, zd* out) { op_apply<double>(op, op.d64, in, out, op.stream, false); };
   Gmres<double> in64;
   Gmres<float> in32;
   zf *b32 = nullptr, *x32 = nullptr, *r32 = nullptr;
@@ -492,7 +513,7 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
       in64.maxit = 1;
       in64.tol = p.tol;
       in64.alloc(N, p.inner_m, false);
-      in64.matvecs = &op.cnt.matvecs;
+      in64.matvecs = env ? env->mv64 : &op.cnt.matvecs;
       in64.A = outer.A;
     } else {
       if (!op.have_g32) op_make_g32(op);
@@ -500,8 +521,9 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
       in32.maxit = 1;
       in32.tol = p.tol;
       in32.alloc(N, p.inner_m, false);
-      in32.matvecs = &op.cnt.matvecs32;
-      in32.A = [&](const zf* in, zf* out) { op_apply<float>(op, op.d32, in, out, op.stream, false); };
+      in32.matvecs = env ? env->mv32 : &op.cnt.matvecs32;
+      if (env) in32.A = env->A32;
+      else in32.A = [&](const zf* in, zf* out) { op_apply<float>(op, op.d32, in, out, op.stream, false); };
       G2G_HIP(hipMalloc(&b32, sizeof(zf) * N));
       G2G_HIP(hipMalloc(&x32, sizeof(zf) * N));
       if (p.precond == 1) G2G_HIP(hipMalloc(&r32, sizeof(zf) * N));
@@ -521,22 +543,25 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
     G2G_CHECK(p.precond == 0 || (learned && learned->apply), -4,
               "learned preconditioner requested but none is attached");
     if (p.precond == 1) {
-      // learned start + inner GMRES: c = M(v); accept it when it cuts the
-      // residual, ||v - A c|| < accept ||v|| (in the inner precision), then
-      // z = c + inner GMRES on v - A c; otherwise z = plain inner GMRES(v).
+      // learned start + inner GMRES: c = M(v), scaled by the least-squares
+      // factor alpha = <A c, v>/<A c, A c>; accepted when it cuts the
+      // residual, ||v - alpha A c|| < accept ||v|| (inner precision), then
+      // z = alpha c + inner GMRES on v - alpha A c; else plain inner GMRES.
       outer.precond = [&](const zd* v, zd* z) {
         res.precond_calls++;
         learned->apply(learned->ctx, v, z);
+        zc alpha;
         if (p.inner_precision == 0) {
           op_apply<double>(op, op.d64, z, tmp, op.stream, false);
           op.cnt.matvecs++;
-          blas.bminus(tmp, v, N);  // tmp = v - A c
-          const double rn = blas.norm<zd, double>(tmp, N), vn = blas.norm<zd, double>(v, N);
+          const double rn = fit_residual<zd, double>(blas, v, tmp, tmp, N, &alpha);
+          const double vn = blas.norm<zd, double>(v, N);
           if (!(rn < learned->accept * vn)) {
             res.precond_fallbacks++;
             in64.solve(z, v, v);
             return;
           }
+          blas.lincomb(z, z, N, 1, &alpha, false);
           in64.solve(tmp2, tmp, tmp);
           blas.add(z, tmp2, N);
         } else {
@@ -544,30 +569,33 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
           blas.convert<zd, zf>(x32, z, N);
           op_apply<float>(op, op.d32, x32, r32, op.stream, false);
           op.cnt.matvecs32++;
-          blas.bminus(r32, b32, N);  // r32 = v - A c
-          const double rn = blas.norm<zf, float>(r32, N), vn = blas.norm<zf, float>(b32, N);
+          const double rn = fit_residual<zf, float>(blas, b32, r32, r32, N, &alpha);
+          const double vn = blas.norm<zf, float>(b32, N);
           if (!(rn < learned->accept * vn)) {
             res.precond_fallbacks++;
             in32.solve(x32, b32, b32);
             blas.convert<zf, zd>(z, x32, N);
             return;
           }
+          blas.lincomb(z, z, N, 1, &alpha, false);
           in32.solve(x32, r32, r32);
           blas.convert<zf, zd>(tmp, x32, N);
           blas.add(z, tmp, N);
         }
       };
     } else if (p.precond == 2) {
-      // learned step only, z = M(v); the outer checks ||v - A z|| < accept ||v||
-      // with the A z it computes anyway and falls back to inner GMRES(v).
+      // learned step only, z = M(v); the outer checks, with the A z it
+      // computes anyway, that min_alpha ||v - alpha A z|| < accept ||v||
+      // (FGMRES does not depend on the scale of z) and otherwise replaces
+      // z by inner GMRES(v).
       outer.precond = [&](const zd* v, zd* z) {
         res.precond_calls++;
         learned->apply(learned->ctx, v, z);
       };
       outer.accept_step = [&](const zd* v, const zd* Az) {
-        blas.copy(tmp, Az, N);
-        blas.bminus(tmp, v, N);
-        const double rn = blas.norm<zd, double>(tmp, N), vn = blas.norm<zd, double>(v, N);
+        zc alpha;
+        const double rn = fit_residual<zd, double>(blas, v, Az, tmp, N, &alpha);
+        const double vn = blas.norm<zd, double>(v, N);
         if (rn < learned->accept * vn) return true;
         res.precond_fallbacks++;
         return false;
@@ -586,11 +614,131 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
       if (q) (void)hipFree(q);
     throw;
   }
-  G2G_HIP(hipStreamSynchronize(op.stream));
+  G2G_HIP(hipStreamSynchronize(st));
   outer.fini(); in64.fini(); in32.fini(); blas.fini();
   for (void* q : {(void*)b32, (void*)x32, (void*)r32, (void*)tmp, (void*)tmp2})
     if (q) (void)hipFree(q);
   return res;
+}
+
+// ------------------------------------------------------- paired solve ---
+// Two independent solves with the same operator (the two polarizations of
+// one frequency) run on two host threads, each with its own stream for the
+// Krylov BLAS.  Their matrix-vector products meet in a rendezvous: when both
+// wait for a product of the same precision, one two-vector apply serves both
+// (op_apply2: the tensor is read once for two right-hand sides); a thread
+// whose partner has finished or waits for the other precision goes alone.
+namespace {
+struct PairBatcher {
+  Operator* op = nullptr;
+  std::mutex mu;
+  std::condition_variable cv;
+  struct Req {
+    const void* in = nullptr;
+    void* out = nullptr;
+    int prec = 0;
+    bool pending = false, done = false;
+  } r[2];
+  bool active[2] = {true, true};
+  long long paired = 0, alone = 0;
+
+  void run_one(const Req& q) {
+    if (q.prec == 0) op_apply<double>(*op, op->d64, (const zd*)q.in, (zd*)q.out, op->stream, false);
+    else op_apply<float>(*op, op->d32, (const zf*)q.in, (zf*)q.out, op->stream, false);
+  }
+  void apply(int me, const void* in, void* out, int prec) {
+    std::unique_lock<std::mutex> lk(mu);
+    Req& m = r[me];
+    m.in = in;
+    m.out = out;
+    m.prec = prec;
+    m.pending = true;
+    m.done = false;
+    const int ot = 1 - me;
+    for (;;) {
+      if (m.done) {
+        m.done = false;
+        return;
+      }
+      if (r[ot].pending || !active[ot]) {
+        if (r[ot].pending && r[ot].prec == prec && op->Nz % 16 == 0) {
+          const Req& a = r[0];
+          const Req& b = r[1];
+          if (prec == 0)
+            op_apply2<double>(*op, op->d64, (const zd*)a.in, (const zd*)b.in, (zd*)a.out, (zd*)b.out, op->stream);
+          else
+            op_apply2<float>(*op, op->d32, (const zf*)a.in, (const zf*)b.in, (zf*)a.out, (zf*)b.out, op->stream);
+          paired++;
+        } else {
+          if (r[ot].pending) {
+            run_one(r[ot]);
+            alone++;
+          }
+          run_one(m);
+          alone++;
+        }
+        G2G_HIP(hipStreamSynchronize(op->stream));
+        if (r[ot].pending) {
+          r[ot].pending = false;
+          r[ot].done = true;
+        }
+        m.pending = false;
+        cv.notify_all();
+        return;
+      }
+      cv.wait(lk);
+    }
+  }
+  void finish(int me) {
+    std::lock_guard<std::mutex> lk(mu);
+    active[me] = false;
+    cv.notify_all();
+  }
+};
+}  // namespace
+
+PairResult nested_solve_pair(Operator& op, const zd* const b[2], const zd* const x0[2], zd* const xout[2],
+                             const SolveParams& p) {
+  G2G_CHECK(p.precond == 0, -3, "paired solves use the inner GMRES preconditioner only");
+  if (p.inner_precision != 0 && !op.have_g32) op_make_g32(op);
+  G2G_HIP(hipStreamSynchronize(op.stream));
+  PairBatcher pb;
+  pb.op = &op;
+  PairResult out;
+  std::exception_ptr err[2];
+  auto worker = [&](int me) {
+    hipStream_t s = nullptr;
+    try {
+      G2G_HIP(hipStreamCreate(&s));
+      SolveEnv env;
+      env.stream = s;
+      env.mv64 = &out.mv64[me];
+      env.mv32 = &out.mv32[me];
+      env.A64 = [&, me, s](const zd* in, zd* o) {
+        G2G_HIP(hipStreamSynchronize(s));
+        pb.apply(me, in, o, 0);
+      };
+      env.A32 = [&, me, s](const zf* in, zf* o) {
+        G2G_HIP(hipStreamSynchronize(s));
+        pb.apply(me, in, o, 1);
+      };
+      out.r[me] = nested_solve(op, b[me], x0[me], xout[me], p, &out.hist[me], nullptr, &env);
+    } catch (...) {
+      err[me] = std::current_exception();
+    }
+    pb.finish(me);
+    if (s) (void)hipStreamDestroy(s);
+  };
+  std::thread t0(worker, 0), t1(worker, 1);
+  t0.join();
+  t1.join();
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
+  out.paired = pb.paired;
+  out.alone = pb.alone;
+  op.cnt.matvecs += out.mv64[0] + out.mv64[1];
+  op.cnt.matvecs32 += out.mv32[0] + out.mv32[1];
+  return out;
 }
 
 void solve_scale(Operator& op, zd* out, const zd* in, int mode) {

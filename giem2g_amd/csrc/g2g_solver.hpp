@@ -1,5 +1,6 @@
 // g2g_solver.hpp -- nested FGMRES interface (see g2g_solver.hip).
 #pragma once
+#include <functional>
 #include <vector>
 
 #include "g2g_op.hpp"
@@ -27,8 +28,29 @@ struct PrecondHook {
   double accept = 1.0;  // accept the learned step when ||v - A M(v)|| < accept*||v||
 };
 
+// Where a solve runs: the stream of its Krylov BLAS, its matrix-vector
+// products and their counters (default: the operator's stream and apply).
+struct SolveEnv {
+  hipStream_t stream = nullptr;
+  std::function<void(const zd*, zd*)> A64;
+  std::function<void(const zf*, zf*)> A32;
+  int64_t* mv64 = nullptr;
+  int64_t* mv32 = nullptr;
+};
+
 SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
-                         const SolveParams& p, std::vector<double>* hist, const PrecondHook* learned);
+                         const SolveParams& p, std::vector<double>* hist, const PrecondHook* learned,
+                         const SolveEnv* env = nullptr);
+
+struct PairResult {
+  SolveResult r[2];
+  int64_t mv64[2] = {0, 0}, mv32[2] = {0, 0};
+  std::vector<double> hist[2];
+  long long paired = 0, alone = 0;  // two-vector and one-vector operator applications
+};
+// two independent solves (b[0], b[1]) whose operator products are paired
+PairResult nested_solve_pair(Operator& op, const zd* const b[2], const zd* const x0[2], zd* const xout[2],
+                             const SolveParams& p);
 // mode 0: out = sqrt(Re sb)*in (rhs); 1: out = in/asiga (warm start); 2: out = asiga*in (field)
 void solve_scale(Operator& op, zd* out, const zd* in, int mode);
 

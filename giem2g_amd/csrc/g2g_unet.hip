@@ -1,7 +1,8 @@
 // g2g_unet.hip -- the learned right preconditioner on MI355X (gfx950).
 //
 // Network: 3-level 3D U-Net, widths 16/32/64, input 8 channels (residual
-// re/im x 3, log10 contrast, one zero pad), output 6 channels; 3x3x3
+// re/im x 3, log10 contrast, log10 of the cell size in background
+// wavenumbers), output 6 channels; 3x3x3
 // convolutions with bias and ReLU, 2x2x2 average pooling, nearest
 // upsampling, concatenation [upsampled, skip], a 1x1 output convolution.
 // Volumes are channels-last [batch][z][y][x][C] in bf16.
@@ -18,7 +19,9 @@
 // gradient accumulates in FP32 through LDS tiles.  No reference counterpart:
 // GFGMRES/seq_fgmres.F90:142-152 preconditions with an inner GMRES.
 #include <cmath>
+#include <complex>
 #include <cstring>
+#include <vector>
 
 #include "g2g_op.hpp"
 #include "g2g_unet.hpp"
@@ -444,15 +447,15 @@ __global__ void k_logc(const zd* __restrict__ asiga, const zd* __restrict__ csig
 }
 
 // x[p][8] from a residual r (slots s0,s1,s2 hold physical x,y,z); bf16 or FP32
-__global__ void k_pack(const zd* __restrict__ r, int64_t ncell, int s0, int s1, int s2,
-                       const double* __restrict__ scale, const float* __restrict__ logc, bf16* xb,
-                       float* xf) {
+__global__ void k_pack(const zd* __restrict__ r, int64_t ncell, int64_t ncxy, int s0, int s1, int s2,
+                       const double* __restrict__ scale, const float* __restrict__ logc,
+                       const float* __restrict__ logk, bf16* xb, float* xf) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= ncell) return;
   const double is = scale[1];
   const zd a = r[s0 * ncell + p], b = r[s1 * ncell + p], c = r[s2 * ncell + p];
   float v[8] = {(float)(a.x * is), (float)(a.y * is), (float)(b.x * is), (float)(b.y * is),
-                (float)(c.x * is), (float)(c.y * is), logc[p], 0.f};
+                (float)(c.x * is), (float)(c.y * is), logc[p], logk[p / ncxy]};
   if (xb) {
     bf16x8 o;
 #pragma unroll
@@ -805,9 +808,17 @@ void learned_prepare(LearnedCtx& c, Operator& op, UNet& net) {
   c.ncell = op.ncell();
   if (!c.logc) {
     G2G_HIP(hipMalloc(&c.logc, sizeof(float) * c.ncell));
+    G2G_HIP(hipMalloc(&c.logk, sizeof(float) * op.Nz));
     G2G_HIP(hipMalloc(&c.scale, sizeof(double) * 2));
     G2G_HIP(hipMalloc(&c.part, sizeof(double) * 1024));
   }
+  // frequency feature per layer: log10(|k_b| dx), |k_b|^2 = omega mu0 |sigma_b|
+  std::vector<float> lk(op.Nz);
+  for (int z = 0; z < op.Nz; z++) {
+    const double kk = op.omega * (4e-7 * 3.14159265358979323846) * std::abs(std::complex<double>(op.csigb[z].x, op.csigb[z].y));
+    lk[z] = (float)std::log10(std::sqrt(kk) * op.dx);
+  }
+  G2G_HIP(hipMemcpyAsync(c.logk, lk.data(), sizeof(float) * op.Nz, hipMemcpyHostToDevice, op.stream));
   zd* d_csigb = nullptr;
   G2G_HIP(hipMalloc(&d_csigb, sizeof(zd) * op.Nz));
   G2G_HIP(hipMemcpyAsync(d_csigb, op.csigb.data(), sizeof(zd) * op.Nz, hipMemcpyHostToDevice, op.stream));
@@ -819,9 +830,9 @@ void learned_prepare(LearnedCtx& c, Operator& op, UNet& net) {
 }
 
 void learned_free(LearnedCtx& c) {
-  for (void* p : {(void*)c.logc, (void*)c.scale, (void*)c.part})
+  for (void* p : {(void*)c.logc, (void*)c.logk, (void*)c.scale, (void*)c.part})
     if (p) (void)hipFree(p);
-  c.logc = nullptr;
+  c.logc = c.logk = nullptr;
   c.scale = c.part = nullptr;
   c.ncell = 0;
 }
@@ -837,8 +848,8 @@ void learned_features(LearnedCtx& c, const zd* r, const zd* e, float* x, float* 
   Operator& op = *c.op;
   residual_scale(c, r, s);
   const int* sl = op.comp_slot;
-  hipLaunchKernelGGL(k_pack, dim3(nblk(c.ncell)), dim3(UB), 0, s, r, c.ncell, sl[0], sl[1], sl[2], c.scale,
-                     c.logc, (bf16*)nullptr, x);
+  hipLaunchKernelGGL(k_pack, dim3(nblk(c.ncell)), dim3(UB), 0, s, r, c.ncell, (int64_t)op.Nx * op.Ny, sl[0],
+                     sl[1], sl[2], c.scale, c.logc, c.logk, (bf16*)nullptr, x);
   if (e && t)
     hipLaunchKernelGGL(k_target, dim3(nblk(c.ncell)), dim3(UB), 0, s, e, op.d64.asiga, c.ncell, sl[0], sl[1],
                        sl[2], c.scale, t);
@@ -851,8 +862,8 @@ void learned_apply(LearnedCtx& c, const zd* v, zd* z, hipStream_t s) {
   unet_reserve(n, 1, op.Nx, op.Ny, op.Nz, false);
   residual_scale(c, v, s);
   const int* sl = op.comp_slot;
-  hipLaunchKernelGGL(k_pack, dim3(nblk(c.ncell)), dim3(UB), 0, s, v, c.ncell, sl[0], sl[1], sl[2], c.scale,
-                     c.logc, n.in, (float*)nullptr);
+  hipLaunchKernelGGL(k_pack, dim3(nblk(c.ncell)), dim3(UB), 0, s, v, c.ncell, (int64_t)op.Nx * op.Ny, sl[0],
+                     sl[1], sl[2], c.scale, c.logc, c.logk, n.in, (float*)nullptr);
   unet_forward(n, s);
   hipLaunchKernelGGL(k_unpack, dim3(nblk(c.ncell)), dim3(UB), 0, s, n.out, op.d64.asiga, c.ncell, sl[0], sl[1],
                      sl[2], c.scale, z);

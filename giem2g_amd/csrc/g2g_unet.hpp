@@ -59,11 +59,12 @@ void unet_adam(UNet& n, float lr, hipStream_t s);
 struct Operator;
 // Features of a residual r on the operator grid (complex128, memory slots as
 // the operator's vectors): x [vox][8] = (r/rms re/im per physical component,
-// log10(Re sa / Re sb), 0); optional target t [vox][6] = a*e/rms.
+// log10(Re sa / Re sb), log10(|k_b| dx)); optional target t [vox][6] = a*e/rms.
 struct LearnedCtx {
   UNet* net = nullptr;
   Operator* op = nullptr;
-  float* logc = nullptr;    // per cell
+  float* logc = nullptr;    // per cell: log10(Re sa / Re sb)
+  float* logk = nullptr;    // per layer: log10(|k_b| dx), |k_b|^2 = omega mu0 |sb|
   double* scale = nullptr;  // [rms, 1/rms]
   double* part = nullptr;   // reduction partials
   int64_t ncell = 0;

@@ -6,8 +6,12 @@ inner GMRES (GFGMRES/seq_fgmres.F90:142-152).  The network maps the
 residual r of the scaled system A x = b (ie_solver_module.F90:64-66) to the
 correction e = A^-1 r, which the solver uses as z = M(v) (precond mode 2)
 or as the start of the inner GMRES (mode 1).  Input channels: r / rms(r)
-re/im of x, y, z and log10(Re sa / Re sb); output: a * e / rms(r) (the field
-correction), so that z = rms * out / a.
+re/im of x, y, z, log10(Re sa / Re sb) and log10(|k_b| dx) (the frequency
+enters through the background wavenumber k_b); output: the field correction
+a * e in units of rms(r), so that z = rms * out / a.  Training targets are
+normalized per sample to unit RMS: the solver uses only the direction of
+M(v) (the outer FGMRES is invariant to its scale, and mode 1 rescales it by
+the least-squares factor).
 
 Training pairs (r_j, e_j) come from the outer FGMRES iterates x_j of the
 reference algorithm: x_0 = b, and x_j is the iterate after j outer Arnoldi
@@ -152,6 +156,7 @@ def training_pairs(op, m, pol, max_steps=5, inner_m=30, tol=1e-9, stop=1e-8):
         x = torch.empty(shape + (8,), dtype=torch.float32, device=dev)
         t = torch.empty(shape + (6,), dtype=torch.float32, device=dev)
         op.learned_features(r.contiguous(), e, x, t)
+        t /= torch.sqrt(torch.mean(t * t))
         feats.append(x)
         targs.append(t)
     del N

@@ -155,6 +155,10 @@ class IEOperator:
         2: a vin (x -> field); torch complex128 tensors on the GPU."""
         L.check(self.lib.giem2g_op_scale(self.h, mode, _ptr(vin), _ptr(vout)))
 
+    def set_contraction(self, mode):
+        """0 automatic, 1 vector-ALU kernel, 2 matrix-core (MFMA) kernel."""
+        L.check(self.lib.giem2g_op_set_contraction(self.h, mode))
+
     def set_learned(self, net, accept=1.0):
         """Attach a learned.UNet as the right preconditioner of precond modes
         1 and 2 (None detaches)."""
@@ -168,6 +172,20 @@ class IEOperator:
 
     def learned_apply(self, v, z):
         L.check(self.lib.giem2g_op_learned_apply(self.h, _ptr(v), _ptr(z)))
+
+    def solve_pair(self, Eb0, Eb1, E00=None, E01=None, outer_m=30, inner_m=30, max_restart=100,
+                   tol=1e-8, inner_precision=0):
+        """Two solves (torch complex128 tensors on the GPU) with paired
+        operator products; returns (E0, E1, info0, info1, [paired, alone])."""
+        import torch
+        p = L.SolverParams(outer_m, inner_m, max_restart, tol, inner_precision, 0, 0, 0)
+        infos = (L.SolveInfo * 2)()
+        cnt = (C.c_longlong * 2)()
+        o0, o1 = torch.empty_like(Eb0), torch.empty_like(Eb1)
+        torch.cuda.synchronize()
+        L.check(self.lib.giem2g_solve_pair_dev(self.h, _ptr(Eb0), _ptr(Eb1), _ptr(E00), _ptr(E01),
+                                               C.byref(p), _ptr(o0), _ptr(o1), infos, cnt))
+        return o0, o1, infos[0].as_dict(), infos[1].as_dict(), [cnt[0], cnt[1]]
 
     def history(self, maxlen=100000):
         buf = np.zeros(maxlen)

@@ -201,11 +201,30 @@ int giem2g_solve(void *ie_op, const double *E_bkg, const double *E0,
 int giem2g_solve_dev(void *ie_op, const void *E_bkg_dev, const void *E0_dev,
                      const giem2g_solver_params *p, void *E_sol_dev,
                      giem2g_solve_info *info);
+/* Kernel of the per-(kx,ky) tensor contraction: 0 automatic (matrix cores
+ * when Nz is a multiple of 16, else vector ALU), 1 vector ALU, 2 matrix
+ * cores (MFMA; requires Nz % 16 == 0).  Both compute the same product. */
+int giem2g_op_set_contraction(void *ie_op, int mode);
+
 /* Scalings between the field and the scaled system (ie_solver_module.F90:
  * 64-66, 85-97) on complex128 HBM vectors: mode 0 out = sqrt(Re sb)*in (the
  * right-hand side from E_bkg), 1 out = in/a (a field to the unknown x),
  * 2 out = a*in (x to the field), a = 2 sqrt(Re sb)/(sa + conj sb). */
 int giem2g_op_scale(void *ie_op, int mode, const void *in_dev, void *out_dev);
+
+/* Two independent solves with the same operator (e.g. the EY and EX
+ * polarizations of one frequency) run together: their operator products
+ * are paired so that one two-vector apply reads the tensor once for both
+ * (needs Nz % 16 == 0, otherwise the products run one by one).  HBM
+ * pointers; E0x may be NULL; info2 receives two records (seconds = wall
+ * time of the pair); paired (nullable) receives [two-vector applies,
+ * one-vector applies].  precond must be 0.  No reference counterpart: the
+ * reference solves the polarizations one after the other
+ * (giem2g.F90:346, 435). */
+int giem2g_solve_pair_dev(void *ie_op, const void *E_bkg0_dev, const void *E_bkg1_dev,
+                          const void *E00_dev, const void *E01_dev, const giem2g_solver_params *p,
+                          void *E_sol0_dev, void *E_sol1_dev, giem2g_solve_info *info2,
+                          long long *paired);
 
 /* Residual history of the last solve (outer Arnoldi residuals). */
 int giem2g_last_history(void *ie_op, double *out, int maxlen, int *len);
@@ -216,8 +235,9 @@ int giem2g_last_history(void *ie_op, double *out, int maxlen, int *len);
  * A 3-level 3D U-Net (widths 16/32/64, 3x3x3 convolutions, BF16 weights and
  * activations on MFMA, FP32 accumulation).  Volumes are channels-last FP32
  * arrays [B][Nz][Ny][Nx][C] in HBM: input C = 8 (residual re/im of x, y, z
- * divided by its RMS, log10(Re sa / Re sb), 0), output C = 6 (a*e/RMS,
- * re/im of x, y, z).  Nx, Ny, Nz must be multiples of 4. */
+ * divided by its RMS, log10(Re sa / Re sb), log10(|k_b| dx) with
+ * |k_b|^2 = omega mu0 |sb|), output C = 6 (the correction a*e, re/im of
+ * x, y, z, in the residual's units).  Nx, Ny, Nz must be multiples of 4. */
 int giem2g_unet_create(void **net);
 int giem2g_unet_destroy(void *net);
 int giem2g_unet_nparams(long long *n);

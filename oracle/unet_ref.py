@@ -2,8 +2,8 @@
 infrastructure: only tests/ use it, as the checker of the HIP U-Net kernels).
 
 The network is the 3-level 3D U-Net of the learned right preconditioner:
-widths 16/32/64, 7 input channels (residual re/im x 3 + log-contrast,
-padded with one zero channel to 8), 6 output channels, 3x3x3 convolutions
+widths 16/32/64, 8 input channels (residual re/im x 3, log-contrast and
+the log cell size in background wavenumbers), 6 output channels, 3x3x3 convolutions
 with bias and ReLU, 2x2x2 average pooling, nearest-neighbour upsampling and
 channel concatenation [upsampled, skip].  There is no counterpart in the
 reference (its preconditioner is the inner GMRES of GFGMRES/seq_fgmres.F90).

@@ -91,3 +91,28 @@ def test_nonconvergence_is_reported(c0_case):
     assert gi["status"] in (-1, 2)
     assert gi["outer_iterations"] == 2
     op.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_paired_polarizations_match_single_solves(prec):
+    """giem2g_solve_pair_dev: the EY and EX solves of one frequency with
+    paired two-vector operator products give the single solves' fields."""
+    import torch
+    from giem2g_amd import models, operator as gop
+    print("paired precision", prec, flush=True)
+    m = models.small(16, 16, 16, seed=31, freq=0.7, dz=20.0)
+    op = gop.operator_for(m)
+    Ey = torch.from_numpy(op.plane_wave(1)).cuda()
+    Ex = torch.from_numpy(op.plane_wave(0)).cuda()
+    kw = dict(outer_m=10, inner_m=10, max_restart=50, tol=1e-8, inner_precision=prec)
+    sy, iy = op.solve(Ey, **kw)
+    sx, ix = op.solve(Ex, **kw)
+    py, px, jy, jx, cnt = op.solve_pair(Ey, Ex, **kw)
+    print(iy, ix, jy, jx, cnt, flush=True)
+    assert jy["status"] == 1 and jx["status"] == 1
+    assert jy["true_residual"] < 1e-8 and jx["true_residual"] < 1e-8
+    assert rel(py, sy) < 1e-10 and rel(px, sx) < 1e-10
+    assert jy["outer_iterations"] == iy["outer_iterations"]
+    assert jx["outer_iterations"] == ix["outer_iterations"]
+    assert cnt[0] > 0
+    op.close()

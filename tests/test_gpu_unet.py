@@ -111,7 +111,8 @@ def test_learned_features(oracle):
         assert rel(x[..., 2 * c] + 1j * x[..., 2 * c + 1], rr[c] / rms) < 1e-6
         assert rel(t[..., 2 * c] + 1j * t[..., 2 * c + 1], a * ee[c] / rms) < 1e-6
     assert np.allclose(x[..., 6], np.log10(sa.real / sb.real), atol=1e-5)
-    assert np.all(x[..., 7] == 0)
+    k = np.sqrt(m.omega * 4e-7 * np.pi * np.abs(sb))  # background wavenumber |k_b|
+    assert np.allclose(x[..., 7], np.broadcast_to(np.log10(k * m.dx), x[..., 7].shape), atol=1e-5)
     op.set_learned(None)
     op.close()
     net.close()
