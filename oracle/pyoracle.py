@@ -16,10 +16,12 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libgiem2g_oracle.so")
 REF_PATH = os.path.join(HERE, "_ref", "liboracle_ref.so")
+REF_QUAD_PATH = os.path.join(HERE, "_ref", "liboracle_ref_quad.so")
 NTAPS = 501
 
 _lib = None
 _ref = None
+_ref_quad = None
 
 dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 cdp = np.ctypeslib.ndpointer(dtype=np.complex128, flags="C_CONTIGUOUS")
@@ -77,20 +79,30 @@ def ref_available():
     return os.path.exists(REF_PATH)
 
 
-def ref():
-    global _ref
+def ref(quad=False):
+    """The reference's routines built from its Fortran sources (oracle/_ref);
+    quad=True: the build with quad-precision Hankel-filter weights."""
+    global _ref, _ref_quad
+    if quad:
+        if _ref_quad is None:
+            _ref_quad = _bind(C.CDLL(REF_QUAD_PATH))
+        return _ref_quad
     if _ref is None:
-        R = C.CDLL(REF_PATH)
-        R.ref_vbt_weights.argtypes = [C.c_double] * 5 + [dp, dp]
-        R.ref_double_integral_u.argtypes = [C.c_int, dp, dp, C.c_double, C.c_int, dp,
-                                            C.c_double, dp, cdp]
-        R.ref_calc_apq.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, cdp, cdp, cdp, cdp, cdp]
-        R.ref_fgmres_dense.argtypes = [C.c_int, cdp, cdp, cdp, C.c_int, C.c_int, C.c_int, C.c_double,
-                                       cdp, C.POINTER(C.c_int), C.POINTER(C.c_int),
-                                       C.POINTER(C.c_double), C.POINTER(C.c_double),
-                                       C.POINTER(C.c_longlong)]
-        _ref = R
+        _ref = _bind(C.CDLL(REF_PATH))
     return _ref
+
+
+def _bind(R):
+    """Argument types of the harness entry points."""
+    R.ref_vbt_weights.argtypes = [C.c_double] * 5 + [dp, dp]
+    R.ref_double_integral_u.argtypes = [C.c_int, dp, dp, C.c_double, C.c_int, dp,
+                                        C.c_double, dp, cdp]
+    R.ref_calc_apq.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, cdp, cdp, cdp, cdp, cdp]
+    R.ref_fgmres_dense.argtypes = [C.c_int, cdp, cdp, cdp, C.c_int, C.c_int, C.c_int, C.c_double,
+                                   cdp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_longlong)]
+    return R
 
 
 def nthreads_default():

@@ -7,7 +7,10 @@ Run in the build container (needs oracle/_ref/liboracle_ref.so, which
 
 Every array is produced by the reference's routines (VolumeBesselTransforms,
 Calc_Double_Integral_U, Calc_APQ, GFGMRES) through oracle/ref_harness.F90;
-the inputs are small synthetic cases.  Output: tests/golden/reference_vectors.npz.
+the inputs are small synthetic cases.  With oracle/_ref/liboracle_ref_quad.so
+(`make -C oracle refquad`, the reference's VOLUME_WEIGHT_QUAD build) the
+quad-precision weights and the tensor elements built from them are added.
+Output: tests/golden/reference_vectors.npz.
 """
 import ctypes as C
 import os
@@ -39,8 +42,10 @@ TENSOR_MODELS = [([0.01, 0.1, 1.0], [1000.0, 9000.0], 1.0, np.arange(9) * 50.0, 
                  ([0.01, 0.1, 1.0], [1000.0, 9000.0], 0.3, 900.0 + np.arange(9) * 25.0, "iface")]
 
 
-def ref_tensor_element(R, sig, th, f, z, lx, ly, dx=100.0, dy=100.0):
-    """Block of the real-space tensor from the reference's Fortran routines."""
+def ref_tensor_element(W, sig, th, f, z, lx, ly, dx=100.0, dy=100.0, R=None):
+    """Block of the real-space tensor from the reference's Fortran routines:
+    Hankel weights from library W, layered kernel from R (default W)."""
+    R = W if R is None else R
     Nz = len(z) - 1
     P = Nz * (Nz + 1) // 2
     cs = np.ascontiguousarray(csig(sig, f).view(np.float64))
@@ -48,7 +53,7 @@ def ref_tensor_element(R, sig, th, f, z, lx, ly, dx=100.0, dy=100.0):
     rmin = np.hypot(dx, dy) / 2
     WT0 = np.zeros(501 * 6)
     lms = np.zeros(501)
-    R.ref_vbt_weights(rmin, lx * dx + dx / 2, ly * dy + dy / 2, dx, dy, WT0, lms)
+    W.ref_vbt_weights(rmin, lx * dx + dx / 2, ly * dy + dy / 2, dx, dy, WT0, lms)
     WT0 = WT0.reshape(6, 501).T  # D0, DXX, DXY, DX, DYY, DY
     WT = np.stack([WT0[:, 1], WT0[:, 2], WT0[:, 4], WT0[:, 3], WT0[:, 5], WT0[:, 0]], 1)
     W0 = np.zeros(6)
@@ -152,6 +157,20 @@ def main():
         out["gm%d_params" % ci] = np.array([N, mo, mi, maxit, tol])
         out["gm%d_x" % ci] = xr
         out["gm%d_info" % ci] = np.array([it.value, st.value, bea.value, be.value, nmv.value])
+    # quad-precision weights (VOLUME_WEIGHT_QUAD build) and tensor elements
+    # composed from them: the accuracy cross-check of the FP64 weights
+    if os.path.exists(po.REF_QUAD_PATH):
+        Q = po.ref(quad=True)
+        wq = []
+        for x, y, hx, hy in WEIGHT_CASES:
+            WT = np.zeros(501 * 6)
+            lms = np.zeros(501)
+            Q.ref_vbt_weights(rmin, x, y, hx, hy, WT, lms)
+            wq.append(WT.reshape(6, 501).T)
+        out["weights_quad"] = np.array(wq)
+        for sig, th, f, z, tag in TENSOR_MODELS:
+            for lx, ly in TENSOR_OFFSETS:
+                out["tensorq_%s_%d_%d" % (tag, lx, ly)] = ref_tensor_element(Q, sig, th, f, z, lx, ly, R=R)
     # tensor elements composed from the reference's own weights and kernel
     # (COMPUTE_WEIGHTS masking and the tap sum restated here, 20 lines of
     # calc_ie_tensor_module.F90:257-288)

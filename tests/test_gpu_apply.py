@@ -243,3 +243,24 @@ def test_error_paths():
     with pytest.raises(L.G2GError, match="not ready"):
         op.apply_host(np.zeros(3 * 128, np.complex128))
     op.close()
+
+
+def test_c0_gpu_tensor_against_quad_weight_tensor(golden):
+    """The GPU's FP64 tensor blocks of C0 against blocks built from the
+    reference's quad-precision Hankel weights (VOLUME_WEIGHT_QUAD) and its
+    layered kernel (tests/golden fixtures): 1e-7 per block, 1e-10 of the
+    self block."""
+    from giem2g_amd import models, operator as gop
+    m = models.c0()
+    op = gop.IEOperator.from_model(m)
+    op.set_background(m.csigb(), m.thick, m.omega)
+    op.tensor_stage(0)
+    Gr = op.get_tensor()
+    BS = op.block_size
+    q00 = golden["tensorq_c0_0_0"]
+    for lx, ly in golden["tensor_offsets"]:
+        q = golden["tensorq_c0_%d_%d" % (lx, ly)]
+        b = int(lx) + m.Nx * int(ly)
+        d = np.linalg.norm(Gr[b * BS:(b + 1) * BS] - q)
+        assert d < 1e-7 * np.linalg.norm(q) and d < 1e-10 * np.linalg.norm(q00), (lx, ly)
+    op.close()
