@@ -74,9 +74,18 @@ def cpu_baseline(m, Gspec, v, budget_s=12.0):
                       % (reps, split[0], split[1], split[2])}, out
 
 
+# learned preconditioner in the benchmark: mode 2 (network step, fallback to
+# inner GMRES) with the residual-reduction threshold below
+LEARNED_MODE = 2
+LEARNED_ACCEPT = 1.0
+
+
 def solve_bench(torch, gop, models):
     """Time to a 1e-8 FP64 true residual on the 256x256x32 random-box model
-    (0.1 Hz, EY polarization): plain FP64 nested FGMRES vs complex64 inner."""
+    (0.1 Hz, EY polarization): plain FP64 nested FGMRES, complex64 inner, and
+    complex64 inner with the learned preconditioner when its trained weights
+    (giem2g_amd/data/unet_c4.f32) are present."""
+    from giem2g_amd import learned
     m = models.c2()
     t0 = time.perf_counter()
     op = gop.IEOperator.from_model(m)
@@ -90,21 +99,35 @@ def solve_bench(torch, gop, models):
     res = {"model": "256x256x32 x 25 m, 20 random boxes 1e-3..10 S/m in 0.01 S/m host, f=0.1 Hz, EY",
            "tensor_build_s": st["t_tensor"], "tensor_fft_s": st["t_tensor_fft"], "setup_s": t_setup}
     sols = {}
-    for name, prec in (("fp64", 0), ("mixed", 1)):
+    runs = [("fp64", 0, 0), ("mixed", 1, 0)]
+    net = None
+    if os.path.exists(learned.DEFAULT_WEIGHTS):
+        net = learned.UNet.load(learned.DEFAULT_WEIGHTS)
+        op.set_learned(net, accept=LEARNED_ACCEPT)
+        runs += [("mixed+learned", 1, LEARNED_MODE)]
+    for name, prec, mode in runs:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         Es, info = op.solve(E, outer_m=30, inner_m=30, max_restart=100, tol=1e-8,
-                            inner_precision=prec)
+                            inner_precision=prec, precond=mode)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         sols[name] = Es
         res[name] = {"time_to_1e-8_s": dt, "outer_iterations": info["outer_iterations"],
                      "matvecs_fp64": info["matvecs_fp64"], "matvecs_fp32": info["matvecs_fp32"],
+                     "precond_calls": info["precond_calls"], "precond_fallbacks": info["precond_fallbacks"],
                      "true_residual": info["true_residual"], "status": info["status"]}
         log("solve", name, res[name])
-    d = torch.linalg.vector_norm(sols["mixed"] - sols["fp64"]) / torch.linalg.vector_norm(sols["fp64"])
-    res["mixed_vs_fp64_rel"] = float(d)
-    res["speedup_mixed"] = res["fp64"]["time_to_1e-8_s"] / res["mixed"]["time_to_1e-8_s"]
+    for name in sols:
+        if name != "fp64":
+            d = torch.linalg.vector_norm(sols[name] - sols["fp64"]) / torch.linalg.vector_norm(sols["fp64"])
+            res[name]["vs_fp64_rel"] = float(d)
+            res[name]["speedup_vs_fp64"] = res["fp64"]["time_to_1e-8_s"] / res[name]["time_to_1e-8_s"]
+    if net is not None:
+        res["learned"] = {"weights": os.path.relpath(learned.DEFAULT_WEIGHTS, ROOT), "mode": LEARNED_MODE,
+                          "accept": LEARNED_ACCEPT}
+        op.set_learned(None)
+        net.close()
     op.close()
     return res
 
@@ -213,7 +236,7 @@ def main():
                    "grid": [m.Nx, m.Ny, m.Nz], "precision": "complex128"},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_contract<double> (tensor contraction)",
+                     "kernel": "k_contract2<double,4> (tensor contraction, two components in flight)",
                      "algorithmic_bytes_per_launch": bytes64["contraction"],
                      "avg_launch_ms": float(stage_ms[1])},
         "kernels_ms": {"fwd_fft": float(stage_ms[0]), "contraction": float(stage_ms[1]),

@@ -353,6 +353,183 @@ __global__ __launch_bounds__(NT) void k_contract(cx_t<R>* __restrict__ S,
 #undef G2G_STASH
 }
 
+// Two-deep variant of k_contract: two tensor components in flight in
+// registers while a third is computed from LDS.
+template <typename R, int PF>  // PF prefetch registers per thread: Nz*Nz <= PF*NT
+__global__ __launch_bounds__(NT) void k_contract2(cx_t<R>* __restrict__ S,
+                                                 const cx_t<R>* __restrict__ G, Geo g,
+                                                 int64_t BS, int split_log) {
+  using T2 = cx_t<R>;
+  static_assert(PF == 4 || PF == 16, "PF must be 4 or 16");
+  extern __shared__ unsigned char smem_raw[];
+  const int Nz = g.Nz, Nz3 = g.Nz3, NZP = Nz + 1;
+  const int P = Nz * (Nz + 1) / 2;
+  const int GB = Nz * NZP;                    // one LDS component buffer
+  T2* vin = reinterpret_cast<T2*>(smem_raw);  // [4][3Nz]
+  T2* gbuf = vin + 4 * Nz3;                   // [2][GB]
+  const int b = blockIdx.x, kx = b % g.Nx, ky = b / g.Nx;
+  const int Nx2 = 2 * g.Nx, Ny2 = 2 * g.Ny;
+  // point p: bit 0 = -kx (x-Nyquist column when kx == 0), bit 1 = -ky
+  const int gNx = g.Nx, gNy = g.Ny;
+#define PX(p) (((p) & 1) ? (kx == 0 ? gNx : Nx2 - kx) : kx)
+#define PY(p) (((p) & 2) ? (ky == 0 ? gNy : Ny2 - ky) : ky)
+#define VAL(p) ((!((p) & 1) || kx > 0) && (!((p) & 2) || ky > 0))
+  const T2* blk = G + (int64_t)b * BS;
+  const T2 zero = mk<T2>(0, 0);
+#define G2G_PFDECL(q) T2 pf##q = zero, pg##q = zero;
+  G2G_FOR16(G2G_PFDECL)
+#undef G2G_PFDECL
+#define G2G_CLEN(c) ((c) < 4 ? P : Nz * Nz)
+#define G2G_CSRC(c) ((c) < 4 ? blk + (c) * P : blk + 4 * P + ((c) - 4) * Nz * Nz)
+  // loads are unconditional (index clamped into the component) so that the
+  // compiler counts outstanding loads exactly and keeps the second set in flight
+#define G2G_LDF(q) if ((q) < PF) pf##q = src_[min((int)threadIdx.x + (q) * NT, n_ - 1)];
+#define G2G_LDG(q) if ((q) < PF) pg##q = src_[min((int)threadIdx.x + (q) * NT, n_ - 1)];
+#define G2G_PREFETCH(c, SET)             \
+  {                                      \
+    const T2* src_ = G2G_CSRC(c);        \
+    const int n_ = G2G_CLEN(c);          \
+    G2G_FOR16(G2G_LD##SET)               \
+  }
+#define G2G_STX(q, r)                                      \
+  if ((q) < PF && threadIdx.x + (q) * NT < n_) {           \
+    const int e = threadIdx.x + (q) * NT;                  \
+    if (c_ < 4) dst_[e] = r##q;                            \
+    else dst_[(e / Nz) * NZP + e % Nz] = r##q;             \
+  }
+#define G2G_STF(q) G2G_STX(q, pf)
+#define G2G_STG(q) G2G_STX(q, pg)
+#define G2G_STASH(c, SET, dst)       \
+  {                                  \
+    const int c_ = (c);              \
+    T2* dst_ = (dst);                \
+    const int n_ = G2G_CLEN(c_);     \
+    G2G_FOR16(G2G_ST##SET)           \
+  }
+  G2G_PREFETCH(0, F)
+  G2G_PREFETCH(1, G)
+  for (int i = threadIdx.x; i < 4 * Nz3; i += NT) {
+    const int p = i / Nz3, e = i - p * Nz3;
+    T2 vv = zero;
+    if (VAL(p)) vv = S[((int64_t)PX(p) * Ny2 + PY(p)) * Nz3 + e];
+    vin[i] = vv;
+  }
+  G2G_STASH(0, F, gbuf)
+  __syncthreads();
+  const int split = 1 << split_log;
+  const int part = threadIdx.x & (split - 1);
+  const int i = threadIdx.x >> split_log;
+  const bool active = i < Nz;
+  // sign of point p: s = -1 for -kx (bit 0), m = -1 for -ky (bit 1)
+#define G2G_SG(p) (((p) & 1) ? R(-1) : R(1))
+#define G2G_MR(p) (((p) & 2) ? R(-1) : R(1))
+#define G2G_ACC(p) T2 ax##p = zero, ay##p = zero, az##p = zero;
+  G2G_FOR4(G2G_ACC)
+#undef G2G_ACC
+#pragma unroll
+  for (int comp = 0; comp < 6; comp++) {
+    const T2* cur = gbuf + (comp & 1) * GB;
+    // registers: F holds components 0, 2, 4 and G 1, 3, 5; component
+    // comp+2 is requested while comp is computed and comp+1 is in flight
+    if (comp + 2 < 6) {
+      if (comp & 1) G2G_PREFETCH(comp + 2, G)
+      else G2G_PREFETCH(comp + 2, F)
+    }
+    if (active) {
+#define G2G_T(p) T2 t1##p = zero, t2##p = zero;
+      G2G_FOR4(G2G_T)
+#undef G2G_T
+      if (comp < 4) {
+        const int ii = i * (i + 1) / 2;
+        const int off1 = comp == 0 ? 0 : (comp == 3 ? 2 * Nz : Nz);  // x for xx, z for zz, y otherwise
+        for (int j = part; j < Nz; j += split) {
+          const T2 sv = cur[j <= i ? j + ii : i + j * (j + 1) / 2];
+#define G2G_SYM(p)                                      \
+  {                                                     \
+    const T2* v = vin + (p) * Nz3;                      \
+    cmac(t1##p, sv, v[off1 + j]);                       \
+    if (comp == 1) cmac(t2##p, sv, v[j]);               \
+  }
+          G2G_FOR4(G2G_SYM)
+#undef G2G_SYM
+        }
+#define G2G_FOLD(p)                                                  \
+  if (comp == 0) ax##p = ax##p + t1##p;                              \
+  else if (comp == 1) {                                              \
+    ax##p = ax##p + t1##p * (G2G_SG(p) * G2G_MR(p));                 \
+    ay##p = ay##p + t2##p * (G2G_SG(p) * G2G_MR(p));                 \
+  } else if (comp == 2) ay##p = ay##p + t1##p;                       \
+  else az##p = az##p + t1##p;
+        G2G_FOR4(G2G_FOLD)
+#undef G2G_FOLD
+      } else {
+        const int offT = comp == 4 ? 0 : Nz;  // A^T acts on x (xz) or y (yz)
+        for (int j = part; j < Nz; j += split) {
+          const T2 a = cur[j * NZP + i];   // A(i,j)
+          const T2 at = cur[i * NZP + j];  // A(j,i)
+#define G2G_ASY(p)                          \
+  {                                         \
+    const T2* v = vin + (p) * Nz3;          \
+    cmac(t1##p, a, v[2 * Nz + j]);          \
+    cmac(t2##p, at, v[offT + j]);           \
+  }
+          G2G_FOR4(G2G_ASY)
+#undef G2G_ASY
+        }
+#define G2G_FOLD(p)                                   \
+  if (comp == 4) {                                    \
+    ax##p = ax##p + t1##p * G2G_SG(p);                \
+    az##p = az##p - t2##p * G2G_SG(p);                \
+  } else {                                            \
+    ay##p = ay##p + t1##p * G2G_MR(p);                \
+    az##p = az##p - t2##p * G2G_MR(p);                \
+  }
+        G2G_FOR4(G2G_FOLD)
+#undef G2G_FOLD
+      }
+    }
+    if (comp < 5) {
+      if (comp & 1) G2G_STASH(comp + 1, F, gbuf + ((comp + 1) & 1) * GB)
+      else G2G_STASH(comp + 1, G, gbuf + ((comp + 1) & 1) * GB)
+    }
+    __syncthreads();
+  }
+  for (int o = 1; o < split; o <<= 1) {
+#define G2G_RED(p)                                                       \
+  ax##p.x += __shfl_xor(ax##p.x, o); ax##p.y += __shfl_xor(ax##p.y, o);  \
+  ay##p.x += __shfl_xor(ay##p.x, o); ay##p.y += __shfl_xor(ay##p.y, o);  \
+  az##p.x += __shfl_xor(az##p.x, o); az##p.y += __shfl_xor(az##p.y, o);
+    G2G_FOR4(G2G_RED)
+#undef G2G_RED
+  }
+  if (active && part == 0) {
+#define G2G_OUT(p)                                                  \
+  {                                                                 \
+    T2* dst = S + ((int64_t)PX(p) * Ny2 + PY(p)) * Nz3;             \
+    if (!VAL(p)) ax##p = ay##p = az##p = zero;                      \
+    dst[i] = ax##p;                                                 \
+    dst[Nz + i] = ay##p;                                            \
+    dst[2 * Nz + i] = az##p;                                        \
+  }
+    G2G_FOR4(G2G_OUT)
+#undef G2G_OUT
+  }
+#undef G2G_SG
+#undef G2G_MR
+#undef PX
+#undef PY
+#undef VAL
+#undef G2G_CLEN
+#undef G2G_CSRC
+#undef G2G_LDF
+#undef G2G_LDG
+#undef G2G_PREFETCH
+#undef G2G_STX
+#undef G2G_STF
+#undef G2G_STG
+#undef G2G_STASH
+}
+
 // ------------------------------------------------------ K3 on MFMA ----
 // Same product as k_contract, as matrix-core tiles.  With the point signs
 // s (for -kx) and m (for -ky), every point uses the same 3Nz x 3Nz block
@@ -364,9 +541,8 @@ __global__ __launch_bounds__(NT) void k_contract(cx_t<R>* __restrict__ S,
 // accumulate Re(G0 v_p) and columns 4-7 Im(G0 v_p) in one
 // v_mfma_{f64,f32}_16x16x4 tile; columns 8-15 carry a second right-hand
 // side (NRHS = 2: both polarizations read the tensor once) or repeat the
-// first.  Lane l supplies
-// A[row l%16][k l/16] and B[k l/16][col l%16] and receives rows
-// 4(l/16)..4(l/16)+3 of column l%16.  Wave w owns TPW consecutive 16-row
+// first.  Lane l supplies A[row l%16][k l/16] and B[k l/16][col l%16] and
+// receives four rows of column l%16 (mma_row).  Wave w owns TPW consecutive 16-row
 // tiles of the 3Nz output rows.  The six tensor components stream through
 // two LDS buffers (symmetric ones unpacked to full Nz x Nz), with two
 // components in flight in registers while the matrix cores work on a third.
@@ -379,6 +555,12 @@ __device__ inline v4t<double>::t mma4(double a, double b, v4t<double>::t c) {
 __device__ inline v4t<float>::t mma4(float a, float b, v4t<float>::t c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
+
+// result row of accumulator element q in lane group kq = lane/16: the FP32
+// tile holds rows 4*kq + q, the FP64 tile rows kq + 4*q (both measured)
+template <typename R> __device__ inline int mma_row(int kq, int q);
+template <> __device__ inline int mma_row<float>(int kq, int q) { return 4 * kq + q; }
+template <> __device__ inline int mma_row<double>(int kq, int q) { return kq + 4 * q; }
 
 template <typename R, int NW, int PF, int NRHS>
 __global__ __launch_bounds__(64 * NW) void k_contract_mma(cx_t<R>* __restrict__ S0, cx_t<R>* __restrict__ S1,
@@ -409,8 +591,9 @@ __global__ __launch_bounds__(64 * NW) void k_contract_mma(cx_t<R>* __restrict__ 
 #undef G2G_RDECL
 #define G2G_CLEN(c) ((c) < 4 ? P : Nz * Nz)
 #define G2G_CSRC(c) ((c) < 4 ? blk + (c) * P : blk + 4 * P + ((c) - 4) * Nz * Nz)
-#define G2G_LDA(q) if ((q) < PF && tid + (q) * NTH < n_) ra##q = src_[tid + (q) * NTH];
-#define G2G_LDB(q) if ((q) < PF && tid + (q) * NTH < n_) rb##q = src_[tid + (q) * NTH];
+  // unconditional loads (index clamped into the component): exact load counts
+#define G2G_LDA(q) if ((q) < PF) ra##q = src_[min(tid + (q) * NTH, n_ - 1)];
+#define G2G_LDB(q) if ((q) < PF) rb##q = src_[min(tid + (q) * NTH, n_ - 1)];
 #define G2G_PREF(c, SET)            \
   {                                 \
     const T2* src_ = G2G_CSRC(c);   \
@@ -534,7 +717,7 @@ __global__ __launch_bounds__(64 * NW) void k_contract_mma(cx_t<R>* __restrict__ 
       const int co = (16 * t) / Nz, i0 = (16 * t) % Nz;                     \
       const R so = co == 0 ? sgx : (co == 1 ? sgy : R(1));                  \
       for (int q = 0; q < 4; q++) {                                         \
-        const int row = co * Nz + i0 + 4 * kq + q;                          \
+        const int row = co * Nz + i0 + mma_row<R>(kq, q);                   \
         vo[2 * (((part >> 1) * 4 + pnt) * Nz3 + row) + (part & 1)] = acc##k[q] * so; \
       }                                                                     \
     }                                                                       \
@@ -657,7 +840,10 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
       }
       hipLaunchKernelGGL(kern, dim3(op.Nx * op.Ny), dim3(NT), sm, s, b.S, b.G, g, op.BS, split_log);
     };
-    const bool mma = op.contraction != 1 && op.Nz % 16 == 0;
+    // single right-hand side: the vector-ALU kernel unless the matrix cores
+    // are requested (measured: equal or slower for one vector, as the
+    // tensor stream, not the arithmetic, sets the pace)
+    const bool mma = op.contraction == 2 && op.Nz % 16 == 0;
     if (mma) {
       auto launch_mma = [&](auto kern, int nth) {
         if (sm > 65536) {
@@ -669,10 +855,13 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
       if (op.Nz <= 32) launch_mma(k_contract_mma<R, 3, 6, 1>, 192);
       else if (op.Nz == 48) launch_mma(k_contract_mma<R, 3, 16, 1>, 192);
       else launch_mma(k_contract_mma<R, 4, 16, 1>, 256);
+    } else if (op.contraction == 3) {
+      if (op.Nz * op.Nz <= 4 * NT) launch(k_contract<R, 4>);
+      else launch(k_contract<R, 16>);
     } else if (op.Nz * op.Nz <= 4 * NT) {
-      launch(k_contract<R, 4>);
+      launch(k_contract2<R, 4>);
     } else {
-      launch(k_contract<R, 16>);
+      launch(k_contract2<R, 16>);
     }
   }
   if (timed) G2G_HIP(hipEventRecord(op.ev[2], s));

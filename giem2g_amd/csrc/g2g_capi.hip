@@ -606,7 +606,7 @@ int giem2g_last_history(void* p, double* out, int maxlen, int* len) {
 int giem2g_op_set_contraction(void* p, int mode) {
   return guard([&] {
     Handle* h = H(p);
-    G2G_CHECK(mode >= 0 && mode <= 2, -3, "contraction mode must be 0, 1 or 2");
+    G2G_CHECK(mode >= 0 && mode <= 3, -3, "contraction mode must be 0, 1, 2 or 3");
     G2G_CHECK(mode != 2 || h->op.Nz % 16 == 0, -3, "the matrix-core contraction needs Nz % 16 == 0");
     h->op.contraction = mode;
   });

@@ -57,7 +57,7 @@ struct Operator {
   OpBuffers<double> d64;
   OpBuffers<float> d32;
   hipStream_t stream = nullptr;
-  int contraction = 0;  // 0 automatic (matrix cores when Nz % 16 == 0), 1 vector ALU, 2 matrix cores
+  int contraction = 0;  // 0 automatic (= 1), 1 vector ALU two-deep, 2 matrix cores, 3 vector ALU one-deep
   Counters cnt;
   // tensor-build scratch (frequency independent weights are cached)
   double* wts = nullptr;  // 6 * 501 * Nx*Ny masked, lambda-scaled weights

@@ -626,8 +626,9 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
 // one frequency) run on two host threads, each with its own stream for the
 // Krylov BLAS.  Their matrix-vector products meet in a rendezvous: when both
 // wait for a product of the same precision, one two-vector apply serves both
-// (op_apply2: the tensor is read once for two right-hand sides); a thread
-// whose partner has finished or waits for the other precision goes alone.
+// (op_apply2: the tensor is read once for two right-hand sides on the matrix
+// cores); a thread whose partner has finished or waits for the other
+// precision goes alone.
 namespace {
 struct PairBatcher {
   Operator* op = nullptr;

@@ -156,7 +156,8 @@ class IEOperator:
         L.check(self.lib.giem2g_op_scale(self.h, mode, _ptr(vin), _ptr(vout)))
 
     def set_contraction(self, mode):
-        """0 automatic, 1 vector-ALU kernel, 2 matrix-core (MFMA) kernel."""
+        """0 automatic, 1 vector ALU (two components in flight), 2 matrix cores
+        (MFMA), 3 vector ALU (one component in flight)."""
         L.check(self.lib.giem2g_op_set_contraction(self.h, mode))
 
     def set_learned(self, net, accept=1.0):

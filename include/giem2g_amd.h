@@ -201,9 +201,11 @@ int giem2g_solve(void *ie_op, const double *E_bkg, const double *E0,
 int giem2g_solve_dev(void *ie_op, const void *E_bkg_dev, const void *E0_dev,
                      const giem2g_solver_params *p, void *E_sol_dev,
                      giem2g_solve_info *info);
-/* Kernel of the per-(kx,ky) tensor contraction: 0 automatic (matrix cores
- * when Nz is a multiple of 16, else vector ALU), 1 vector ALU, 2 matrix
- * cores (MFMA; requires Nz % 16 == 0).  Both compute the same product. */
+/* Kernel of the per-(kx,ky) tensor contraction of one vector: 0 automatic
+ * (= 1), 1 vector ALU with two tensor components in flight, 2 matrix cores
+ * (MFMA; requires Nz % 16 == 0), 3 vector ALU with one component in flight.
+ * All compute the same product.  Paired two-vector products
+ * (giem2g_solve_pair_dev) use the matrix cores. */
 int giem2g_op_set_contraction(void *ie_op, int mode);
 
 /* Scalings between the field and the scaled system (ie_solver_module.F90:
