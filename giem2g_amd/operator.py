@@ -160,7 +160,7 @@ class IEOperator:
         (MFMA), 3 vector ALU (one component in flight)."""
         L.check(self.lib.giem2g_op_set_contraction(self.h, mode))
 
-    def set_learned(self, net, accept=1.0):
+    def set_learned(self, net, accept=0.9):
         """Attach a learned.UNet as the right preconditioner of precond modes
         1 and 2 (None detaches)."""
         self._net = net  # keep the network alive while attached

@@ -53,7 +53,7 @@ def residual_reduction(op, pols=(0, 1), max_steps=5):
     return out
 
 
-def evaluate(net, n_eval=8, seed0=12000, accept=1.0):
+def evaluate(net, n_eval=8, seed0=12000, accept=0.9):
     import torch
     from . import models, operator as gop
     rows = []
