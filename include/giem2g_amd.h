@@ -163,12 +163,15 @@ int giem2g_plane_wave(void *ie_op, int pol, double *E_host);
  *   inner_precision: 0 = FP64 inner (reference semantics), 1 = complex64
  *                    inner operator and Krylov basis (FP32 accumulation)
  *   precond:         0 = inner GMRES (reference semantics);
- *                    1 = learned start: c = M(v), accepted when
- *                        ||v - A c|| < accept*||v|| (inner precision), then
- *                        z = c + inner GMRES on v - A c; else plain inner GMRES;
+ *                    1 = learned start: c = M(v) scaled by the least-
+ *                        squares alpha, accepted when ||v - alpha A c|| <
+ *                        accept*||v|| (inner precision), then z = alpha c +
+ *                        inner GMRES on v - alpha A c; else plain inner GMRES;
  *                    2 = learned step only: z = M(v), accepted when the
- *                        outer's own A z gives ||v - A z|| < accept*||v||,
- *                        else z = inner GMRES(v).
+ *                        outer's own A z gives min_alpha ||v - alpha A z|| <
+ *                        accept*||v||, else z = inner GMRES(v).
+ *                    accept < 1 (0.9 by default in the Python host): the
+ *                    least-squares residual never exceeds ||v||.
  *                    Modes 1 and 2 need giem2g_op_set_learned. */
 typedef struct {
   int outer_m;
