@@ -975,6 +975,77 @@ void op_sigma_to_f32(Operator& op) {
   G2G_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------ adjoint ---
+// The contraction kernel K of A = diag(r) (diag(a) - K diag(g)) satisfies
+// diag(dz) K = (diag(dz) K)^T (reciprocity of the Galerkin matrix,
+// giem2g.pdf Eq. 20), so A^T = diag(a r) - diag(g dz) K diag(r/dz) and
+// A^H v = conj(A^T conj(v)).  K u is one operator pass with the per-cell
+// pre-scale r/dz, no diagonal term and unit post-scale.
+__global__ void k_adj_pre(const zd* __restrict__ v, const double* __restrict__ sq,
+                          const double* __restrict__ dz, int64_t ncxy, int Nz, zd* __restrict__ u,
+                          zd* __restrict__ w) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nc = ncxy * Nz;
+  if (i >= 3 * nc) return;
+  const int z = (int)((i % nc) / ncxy);
+  u[i] = mkz(v[i].x, -v[i].y);  // conj(v)
+  if (i < nc) w[i] = mkz(sq[z] / dz[z], 0.0);
+}
+
+__global__ void k_adj_post(const zd* __restrict__ u, const zd* __restrict__ kv,
+                           const zd* __restrict__ asiga, const zd* __restrict__ gsig,
+                           const double* __restrict__ sq, const double* __restrict__ dz, int64_t ncxy,
+                           int Nz, zd* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nc = ncxy * Nz;
+  if (i >= 3 * nc) return;
+  const int64_t cell = i % nc;
+  const int z = (int)(cell / ncxy);
+  // kv = -K(r u / dz): A^T u = a r u + g dz kv
+  const zd t = asiga[cell] * u[i] * sq[z] + gsig[cell] * kv[i] * dz[z];
+  out[i] = mkz(t.x, -t.y);
+}
+
+__global__ void k_fill_z(zd* __restrict__ a, zd val, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = val;
+}
+
+void op_apply_adjoint(Operator& op, const zd* vin, zd* vout, hipStream_t s) {
+  G2G_CHECK(op.have_tensor && op.have_sigma, -4, "operator not ready: tensor or conductivity missing");
+  const int64_t nc = op.ncell(), nv = op.nvec();
+  if (!op.adj) {
+    G2G_HIP(hipMalloc(&op.adj, sizeof(zd) * (2 * nv + 2 * nc) + sizeof(double) * 2 * op.Nz));
+  }
+  zd* u = op.adj;
+  zd* kv = u + nv;
+  zd* w = kv + nv;
+  zd* zero_c = w + nc;
+  double* one_z = reinterpret_cast<double*>(zero_c + nc);
+  double* dzd = one_z + op.Nz;
+  {
+    std::vector<double> h(2 * op.Nz);
+    for (int z = 0; z < op.Nz; z++) {
+      h[z] = 1.0;
+      h[op.Nz + z] = op.dz[z];
+    }
+    G2G_HIP(hipMemcpyAsync(one_z, h.data(), sizeof(double) * 2 * op.Nz, hipMemcpyHostToDevice, s));
+  }
+  const unsigned nb = (unsigned)((3 * nc + 255) / 256);
+  hipLaunchKernelGGL(k_fill_z, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, s, zero_c, mkz(0.0, 0.0), nc);
+  hipLaunchKernelGGL(k_adj_pre, dim3(nb), dim3(256), 0, s, vin, op.d64.sq, dzd, (int64_t)op.Nx * op.Ny, op.Nz,
+                     u, w);
+  OpBuffers<double> b = op.d64;  // shares the tensor, spectra and twiddles
+  b.gsig = w;
+  b.asiga = zero_c;
+  b.sq = one_z;
+  op_apply<double>(op, b, u, kv, s, false);
+  hipLaunchKernelGGL(k_adj_post, dim3(nb), dim3(256), 0, s, u, kv, op.d64.asiga, op.d64.gsig, op.d64.sq, dzd,
+                     (int64_t)op.Nx * op.Ny, op.Nz, vout);
+  G2G_HIP(hipGetLastError());
+  G2G_HIP(hipStreamSynchronize(s));
+}
+
 void op_set_sigma(Operator& op, const zd* d_csiga) {
   G2G_CHECK(op.have_bkg, -4, "background not set");
   const int Nz = op.Nz;

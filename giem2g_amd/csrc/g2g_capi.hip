@@ -103,6 +103,8 @@ void op_init(Operator& op, int Nx, int Ny, int Nz, double dx, double dy, const d
 }
 
 void op_fini(Operator& op) {
+  if (op.adj) (void)hipFree(op.adj);
+  op.adj = nullptr;
   op_free_buffers<double>(op.d64);
   op_free_buffers<float>(op.d32);
   op_free_tensor_scratch(op);
@@ -714,6 +716,21 @@ int giem2g_unet_train_step(void* net, const float* x_dev, const float* t_dev, in
     unet_adam(*n, lr, n->stream);
     G2G_HIP(hipStreamSynchronize(n->stream));
     if (loss) *loss = l;
+  });
+}
+
+int giem2g_unet_adam(void* net, float lr) {
+  return guard([&] {
+    UNet* n = NET(net);
+    unet_adam(*n, lr, n->stream);
+    G2G_HIP(hipStreamSynchronize(n->stream));
+  });
+}
+
+int giem2g_op_apply_adjoint_dev(void* p, const void* in_dev, void* out_dev) {
+  return guard([&] {
+    Operator& op = H(p)->op;
+    op_apply_adjoint(op, (const zd*)in_dev, (zd*)out_dev, op.stream);
   });
 }
 

@@ -63,6 +63,7 @@ struct Operator {
   double* wts = nullptr;  // 6 * 501 * Nx*Ny masked, lambda-scaled weights
   bool have_wts = false;
   double* tabs = nullptr;  // per-lambda kernel tables
+  zd* adj = nullptr;       // adjoint-apply scratch (allocated on first use)
   size_t tabs_bytes = 0;
   hipEvent_t ev[8] = {};
 
@@ -80,6 +81,8 @@ template <typename R>
 void op_apply2(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin0, const cx_t<R>* vin1, cx_t<R>* vout0,
                cx_t<R>* vout1, hipStream_t s);
 void op_set_sigma(Operator& op, const zd* d_csiga);
+// out = A^H in (FP64), from the reciprocity of the contraction kernel
+void op_apply_adjoint(Operator& op, const zd* vin, zd* vout, hipStream_t s);
 void op_make_g32(Operator& op);
 // tensor build (g2g_tensor.hip)
 void op_build_tensor(Operator& op);

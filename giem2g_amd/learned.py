@@ -107,6 +107,10 @@ class UNet:
         L.check(self.lib.giem2g_unet_backward_dev(self.h, _ptr(dy.contiguous())))
         return self.grads()
 
+    def adam(self, lr):
+        """One Adam step on the gradient of the last backward."""
+        L.check(self.lib.giem2g_unet_adam(self.h, C.c_float(lr)))
+
     def train_step(self, x, t, lr):
         B, Z, Y, X, _ = x.shape
         loss = C.c_double()
@@ -226,4 +230,123 @@ def train(net, X, T, steps=2000, batch=8, lr=1e-3, seed=0, log=print, log_every=
         if log and (it % log_every == 0 or it == steps - 1):
             log("train: step %d loss %.4e (%.1f s)" % (it, np.mean(hist[-log_every:]),
                                                       time.perf_counter() - t0))
+    return np.array(hist)
+
+
+# --------------------------------------------------------------------------
+# Training on the residual: loss ||r - A M(r)||^2 / ||r||^2 with the
+# gradient through the operator's adjoint (giem2g_op_apply_adjoint_dev),
+# so that the network is judged by what the solver sees.
+
+class ResidualSample:
+    __slots__ = ("op", "r", "x", "rms", "rn2")
+
+    def __init__(self, op, r, x):
+        import torch
+        self.op, self.r, self.x = op, r, x
+        self.rn2 = float(torch.vdot(r, r).real)
+        self.rms = (self.rn2 / r.numel()) ** 0.5
+
+
+def residual_dataset(n_models, seed0=10000, max_steps=5, log=print):
+    """Residuals of the outer FGMRES iterates of n_models C4 models, with the
+    models' operators kept on the GPU (about 0.1 GB each).  Returns (ops,
+    samples); ops[k] = (operator, model, a) with a = the per-cell scaling
+    a = 2 sqrt(Re sb) / (sa + conj sb) on every vector slot."""
+    import torch
+    from . import models, operator as gop
+    ops, samples = [], []
+    t0 = time.perf_counter()
+    feat = UNet(seed=0)
+    for k in range(n_models):
+        m = models.c4_train(seed0 + k)
+        op = gop.operator_for(m)
+        op.set_learned(feat)
+        n = 3 * m.ncell
+        ones = torch.ones(n, dtype=torch.complex128, device="cuda")
+        a = torch.empty_like(ones)
+        op.scale(2, ones, a)
+        Eb = torch.from_numpy(op.plane_wave(k % 2)).cuda()
+        b = torch.empty_like(Eb)
+        op.scale(0, Eb, b)
+        xj = b.clone()
+        r = torch.empty_like(Eb)
+        bn = float(torch.linalg.vector_norm(b))
+        for j in range(max_steps):
+            if j > 0:
+                Ej, _ = op.solve(Eb, outer_m=j, inner_m=30, max_restart=1, tol=1e-14)
+                op.scale(1, Ej, xj)
+            op.apply_dev(xj, r)
+            torch.cuda.synchronize()
+            rj = (b - r).contiguous()
+            if float(torch.linalg.vector_norm(rj)) < 1e-8 * bn:
+                break
+            x = torch.empty((m.Nz, m.Ny, m.Nx, 8), dtype=torch.float32, device="cuda")
+            op.learned_features(rj, None, x, None)
+            samples.append(ResidualSample(len(ops), rj, x))
+        ops.append((op, m, a))
+        if log and (k % 50 == 0 or k == n_models - 1):
+            log("residual dataset: model %d/%d, %d samples, %.1f s" % (k + 1, n_models, len(samples),
+                                                                     time.perf_counter() - t0))
+    return ops, samples, feat
+
+
+def _to_vec(y, m):
+    """(Nz, Ny, Nx, 6) real channels -> complex (3*Nz*Ny*Nx) operator vector."""
+    import torch
+    w = torch.complex(y[..., 0::2].double(), y[..., 1::2].double())  # (Nz, Ny, Nx, 3)
+    return w.permute(3, 0, 1, 2).reshape(-1).contiguous()
+
+
+def _to_chan(g, m):
+    """complex operator vector -> (Nz, Ny, Nx, 6) real channels."""
+    import torch
+    w = g.reshape(3, m.Nz, m.Ny, m.Nx).permute(1, 2, 3, 0)
+    out = torch.empty(w.shape[:3] + (6,), dtype=torch.float32, device=g.device)
+    out[..., 0::2] = w.real.float()
+    out[..., 1::2] = w.imag.float()
+    return out
+
+
+def residual_loss_step(net, ops, batch, lr):
+    """One Adam step on mean_b ||r_b - A_b z_b||^2 / ||r_b||^2 with
+    z_b = rms_b * net(x_b) / a_b.  Returns the mean loss."""
+    import torch
+    X = torch.stack([s.x for s in batch]).contiguous()
+    Y = net.forward(X, keep=True)
+    dY = torch.empty_like(Y)
+    loss = 0.0
+    B = len(batch)
+    for i, s in enumerate(batch):
+        op, m, a = ops[s.op]
+        z = (s.rms * _to_vec(Y[i], m) / a).contiguous()
+        az = torch.empty_like(z)
+        op.apply_dev(z, az)
+        torch.cuda.synchronize()
+        res = (s.r - az).contiguous()
+        loss += float(torch.vdot(res, res).real) / s.rn2 / B
+        g = torch.empty_like(res)
+        op.apply_adjoint_dev(res, g)
+        torch.cuda.synchronize()
+        # dL/d conj(z) = -A^H res / ||r||^2; dL/d conj(w) = conj(rms / a) dL/d conj(z)
+        gw = torch.conj(s.rms / a) * (-g / s.rn2)
+        dY[i] = 2.0 * _to_chan(gw, m) / B
+    net.backward(dY.contiguous())
+    net.adam(lr)
+    return loss
+
+
+def train_residual(net, ops, samples, steps=2000, batch=8, lr=3e-4, seed=0, log=print, log_every=100):
+    import torch
+    g = torch.Generator()
+    g.manual_seed(seed)
+    hist = []
+    t0 = time.perf_counter()
+    for it in range(steps):
+        idx = torch.randint(0, len(samples), (batch,), generator=g).tolist()
+        lr_t = lr * (0.05 + 0.95 * 0.5 * (1 + np.cos(np.pi * it / steps)))
+        hist.append(residual_loss_step(net, ops, [samples[i] for i in idx], lr_t))
+        if log and (it % log_every == 0 or it == steps - 1):
+            log("train (residual loss): step %d loss %.4f (%.1f s)" % (it, np.mean(hist[-log_every:]),
+                                                                    time.perf_counter() - t0))
     return np.array(hist)

@@ -27,7 +27,8 @@ EXTENDED_SYMBOLS = [
     "giem2g_unet_set_params", "giem2g_unet_get_params", "giem2g_unet_get_grads",
     "giem2g_unet_forward_dev", "giem2g_unet_backward_dev", "giem2g_unet_train_step",
     "giem2g_op_set_learned", "giem2g_op_learned_features", "giem2g_op_learned_apply",
-    "giem2g_op_scale", "giem2g_op_set_contraction", "giem2g_solve_pair_dev",
+    "giem2g_op_scale", "giem2g_op_set_contraction", "giem2g_solve_pair_dev", "giem2g_unet_adam",
+    "giem2g_op_apply_adjoint_dev",
 ]
 
 
@@ -128,6 +129,8 @@ def load():
         "giem2g_op_learned_apply": (i, [vp, vp, vp]),
         "giem2g_op_scale": (i, [vp, i, vp, vp]),
         "giem2g_op_set_contraction": (i, [vp, i]),
+        "giem2g_unet_adam": (i, [vp, C.c_float]),
+        "giem2g_op_apply_adjoint_dev": (i, [vp, vp, vp]),
         "giem2g_solve_pair_dev": (i, [vp, vp, vp, vp, vp, C.POINTER(SolverParams), vp, vp,
                                       C.POINTER(SolveInfo), C.POINTER(C.c_longlong)]),
         "giem2g_calc_data_sizes": (None, [Anomaly, C.POINTER(Data)]),

@@ -107,6 +107,10 @@ class IEOperator:
         """FP64 matvec on HBM (asynchronous on the operator stream)."""
         L.check(self.lib.giem2g_op_apply_dev(self.h, _ptr(vin), _ptr(vout)))
 
+    def apply_adjoint_dev(self, vin, vout):
+        """vout = A^H vin (complex128 torch tensors on the GPU)."""
+        L.check(self.lib.giem2g_op_apply_adjoint_dev(self.h, _ptr(vin), _ptr(vout)))
+
     def apply_dev_f32(self, vin, vout):
         L.check(self.lib.giem2g_op_apply_dev_f32(self.h, _ptr(vin), _ptr(vout)))
 

@@ -1,13 +1,16 @@
 """Train the learned right preconditioner on the GPU.
 
-    python -m giem2g_amd.train_precond --models 2000 --steps 4000 --out giem2g_amd/data/unet_c4.f32
+    python -m giem2g_amd.train_precond --models 300 --steps 3000 --out giem2g_amd/data/unet_c4.f32
 
-1. Training pairs: for each C4 random-box model (64x64x16, f log-uniform in
+1. Training data: for each C4 random-box model (64x64x16, f log-uniform in
    [1e-2, 1e2] Hz, seeds 10000+k, polarization EX/EY alternating) the FP64
-   nested FGMRES is run and the residuals/corrections of its outer iterates
-   are turned into network features on the GPU (learned.training_pairs).
-2. Training: Adam on the MSE with the library's BF16-MFMA forward/backward
-   kernels, random complex phase per sample.
+   nested FGMRES is run and the residuals of its outer iterates are kept
+   (with the correction from the FP64 solution for --loss correction).
+2. Training: Adam with the library's BF16-MFMA forward/backward kernels,
+   either on the residual the solver sees, ||r - A M(r)||^2 / ||r||^2, with
+   the gradient through the operator adjoint (--loss residual, default), or
+   on the MSE to the normalized FP64 correction (--loss correction, with a
+   random complex phase per sample).
 3. Evaluation on held-out models (seeds 12000+): the residual reduction
    ||r - A M(r)|| / ||r|| of the trained network on the held-out pairs, and
    the nested solve with and without the network.
@@ -98,22 +101,37 @@ def main(argv=None):
     ap.add_argument("--eval", type=int, default=8)
     ap.add_argument("--init", default=None, help="start from this parameter file")
     ap.add_argument("--out", default=None)
+    ap.add_argument("--loss", choices=["correction", "residual"], default="residual",
+                    help="correction: MSE to the FP64 correction; residual: ||r - A M(r)||^2 "
+                         "through the operator adjoint")
     a = ap.parse_args(argv)
     import torch
     from . import learned
     assert torch.cuda.is_available()
-    t0 = time.perf_counter()
-    X, T = learned.make_dataset(a.models, max_steps=a.max_pairs, log=log)
-    t_data = time.perf_counter() - t0
-    npairs = int(X.shape[0])
-    log("dataset", tuple(X.shape), "%.1f s" % t_data)
     net = learned.UNet.load(a.init) if a.init else learned.UNet(seed=0)
     t0 = time.perf_counter()
-    hist = learned.train(net, X, T, steps=a.steps, batch=a.batch, lr=a.lr, log=log)
+    if a.loss == "correction":
+        X, T = learned.make_dataset(a.models, max_steps=a.max_pairs, log=log)
+        t_data = time.perf_counter() - t0
+        npairs = int(X.shape[0])
+        log("dataset", tuple(X.shape), "%.1f s" % t_data)
+        t0 = time.perf_counter()
+        hist = learned.train(net, X, T, steps=a.steps, batch=a.batch, lr=a.lr, log=log)
+        del X, T
+    else:
+        ops, samples, feat = learned.residual_dataset(a.models, max_steps=a.max_pairs, log=log)
+        t_data = time.perf_counter() - t0
+        npairs = len(samples)
+        t0 = time.perf_counter()
+        hist = learned.train_residual(net, ops, samples, steps=a.steps, batch=a.batch, lr=a.lr, log=log)
+        for op, _, _ in ops:
+            op.set_learned(None)
+            op.close()
+        del ops, samples
+        feat.close()
     t_train = time.perf_counter() - t0
     if a.out:
         net.save(a.out)
-    del X, T
     torch.cuda.empty_cache()
     rows = evaluate(net, a.eval) if a.eval else []
     summary = {"models": a.models, "pairs": npairs, "dataset_s": t_data,

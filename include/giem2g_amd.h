@@ -259,6 +259,12 @@ int giem2g_unet_backward_dev(void *net, const float *dy_dev);
 /* Forward, MSE loss against t [B][vox][6], backward and one Adam step. */
 int giem2g_unet_train_step(void *net, const float *x_dev, const float *t_dev, int B, int nx,
                            int ny, int nz, float lr, double *loss);
+/* One Adam step (beta 0.9, 0.999) on the gradient of the last backward. */
+int giem2g_unet_adam(void *net, float lr);
+/* out = A^H in on complex128 HBM vectors (no reference counterpart; used to
+ * train the network on the residual ||r - A M(r)||).  From the reciprocity
+ * of the contraction kernel: diag(dz) K is complex-symmetric. */
+int giem2g_op_apply_adjoint_dev(void *ie_op, const void *in_dev, void *out_dev);
 /* Attach a network to an operator (net NULL detaches); accept is the
  * residual-reduction threshold of precond modes 1 and 2. */
 int giem2g_op_set_learned(void *ie_op, void *net, double accept);

@@ -264,3 +264,41 @@ def test_c0_gpu_tensor_against_quad_weight_tensor(golden):
         d = np.linalg.norm(Gr[b * BS:(b + 1) * BS] - q)
         assert d < 1e-7 * np.linalg.norm(q) and d < 1e-10 * np.linalg.norm(q00), (lx, ly)
     op.close()
+
+
+@pytest.mark.parametrize("Nx,Ny,Nz", [(8, 8, 3), (16, 16, 16)])
+def test_adjoint_apply(oracle, Nx, Ny, Nz):
+    """giem2g_op_apply_adjoint_dev: <A^H u, v> = <u, A v> on random vectors,
+    and on a small grid A^H against the conjugate transpose of the dense
+    operator assembled from the oracle."""
+    torch = _torch()
+    from giem2g_amd import models, operator as gop
+    print("case", Nx, Ny, Nz, flush=True)
+    m = models.small(Nx, Ny, Nz, seed=5 + Nz, freq=3.0, dz=30.0)
+    m.z = np.cumsum(np.concatenate([[0.0], 20.0 + 10.0 * (np.arange(Nz) % 3)]))  # unequal layers
+    op = gop.operator_for(m)
+    n = 3 * m.ncell
+    u = torch.from_numpy(rand_vec(n, 1)).cuda()
+    v = torch.from_numpy(rand_vec(n, 2)).cuda()
+    ahu = torch.empty_like(u)
+    av = torch.empty_like(v)
+    op.apply_adjoint_dev(u, ahu)
+    op.apply_dev(v, av)
+    torch.cuda.synchronize()
+    lhs, rhs = complex(torch.vdot(ahu, v)), complex(torch.vdot(u, av))
+    assert abs(lhs - rhs) < 1e-12 * abs(rhs), (lhs, rhs)
+    if n <= 600:
+        om = oracle_model(oracle, m)
+        Gs = om.tensor_fft(om.tensor_realspace())
+        A = np.zeros((n, n), complex)
+        for j in range(n):
+            e = np.zeros(n, complex)
+            e[j] = 1
+            A[:, j] = om.apply(Gs, m.csiga(), e)
+        w = rand_vec(n, 3)
+        wt = torch.from_numpy(w).cuda()
+        out = torch.empty_like(wt)
+        op.apply_adjoint_dev(wt, out)
+        torch.cuda.synchronize()
+        assert rel(out, A.conj().T @ w) < 1e-11
+    op.close()
