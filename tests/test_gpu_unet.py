@@ -129,7 +129,7 @@ def test_learned_modes_converge_with_untrained_net(mode, prec):
     E = op.plane_wave(1)
     E0, i0 = op.solve(E, outer_m=10, inner_m=10, max_restart=50, tol=1e-8)
     net = learned.UNet(seed=2)
-    op.set_learned(net, accept=1.0)
+    op.set_learned(net, accept=0.9)
     E1, i1 = op.solve(E, outer_m=10, inner_m=10, max_restart=50, tol=1e-8, inner_precision=prec,
                       precond=mode)
     print(i0, i1, flush=True)
