@@ -24,16 +24,15 @@ namespace g2g {
 
 constexpr int NT = 256;  // threads per block for every apply kernel
 
+// Rows per FFT block: about 32 KB of LDS per block (four blocks per CU),
+// between 1 and 64 rows.  Measured at 512-point rows: 64 KB blocks (two per
+// CU) were 17-30% slower.
 template <typename R> constexpr int nr_for(int L) {
-  if constexpr (sizeof(R) == 8)
-    return L <= 64 ? 32 : (L <= 128 ? 16 : (L <= 512 ? 8 : 4));
-  else
-    return L <= 64 ? 64 : (L <= 128 ? 32 : (L <= 512 ? 16 : 8));
+  constexpr int kBytes = 32 * 1024;
+  const int n = kBytes / (L * 2 * (int)sizeof(R));
+  return n < 1 ? 1 : (n > 64 ? 64 : n);
 }
-template <typename R> int nr_rt(int L) {
-  if (sizeof(R) == 8) return L <= 64 ? 32 : (L <= 128 ? 16 : (L <= 512 ? 8 : 4));
-  return L <= 64 ? 64 : (L <= 128 ? 32 : (L <= 512 ? 16 : 8));
-}
+template <typename R> int nr_rt(int L) { return nr_for<R>(L); }
 
 struct Geo {
   int Nx, Ny, Nz, Nz3;
