@@ -183,6 +183,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--c3", action="store_true",
                     help="also run the 128x128x64 multi-frequency paired-polarization case")
+    ap.add_argument("--c3-freqs", type=int, default=16, help="number of C3 frequencies (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (HBM counter calibration)")
     a = ap.parse_args()
@@ -262,7 +263,7 @@ def main():
     if not a.no_solve:
         line["solve"] = solve_bench(torch, gop, models)
     if a.c3:
-        line["c3"] = c3_bench(torch, gop, models)
+        line["c3"] = c3_bench(torch, gop, models, nfreq=a.c3_freqs)
     print(json.dumps(line), flush=True)
     return 0
 
