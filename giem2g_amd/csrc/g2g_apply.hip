@@ -354,12 +354,12 @@ __global__ __launch_bounds__(NT) void k_contract(cx_t<R>* __restrict__ S,
 
 // Two-deep variant of k_contract: two tensor components in flight in
 // registers while a third is computed from LDS.
-template <typename R, int PF>  // PF prefetch registers per thread: Nz*Nz <= PF*NT
-__global__ __launch_bounds__(NT) void k_contract2(cx_t<R>* __restrict__ S,
+template <typename R, int PF, int NTC>  // NTC threads; Nz*Nz <= PF*NTC
+__global__ __launch_bounds__(NTC) void k_contract2(cx_t<R>* __restrict__ S,
                                                  const cx_t<R>* __restrict__ G, Geo g,
                                                  int64_t BS, int split_log) {
   using T2 = cx_t<R>;
-  static_assert(PF == 4 || PF == 16, "PF must be 4 or 16");
+  static_assert(PF == 4 || PF == 8 || PF == 16, "PF must be 4, 8 or 16");
   extern __shared__ unsigned char smem_raw[];
   const int Nz = g.Nz, Nz3 = g.Nz3, NZP = Nz + 1;
   const int P = Nz * (Nz + 1) / 2;
@@ -382,8 +382,8 @@ __global__ __launch_bounds__(NT) void k_contract2(cx_t<R>* __restrict__ S,
 #define G2G_CSRC(c) ((c) < 4 ? blk + (c) * P : blk + 4 * P + ((c) - 4) * Nz * Nz)
   // loads are unconditional (index clamped into the component) so that the
   // compiler counts outstanding loads exactly and keeps the second set in flight
-#define G2G_LDF(q) if ((q) < PF) pf##q = src_[min((int)threadIdx.x + (q) * NT, n_ - 1)];
-#define G2G_LDG(q) if ((q) < PF) pg##q = src_[min((int)threadIdx.x + (q) * NT, n_ - 1)];
+#define G2G_LDF(q) if ((q) < PF) pf##q = src_[min((int)threadIdx.x + (q) * NTC, n_ - 1)];
+#define G2G_LDG(q) if ((q) < PF) pg##q = src_[min((int)threadIdx.x + (q) * NTC, n_ - 1)];
 #define G2G_PREFETCH(c, SET)             \
   {                                      \
     const T2* src_ = G2G_CSRC(c);        \
@@ -391,8 +391,8 @@ __global__ __launch_bounds__(NT) void k_contract2(cx_t<R>* __restrict__ S,
     G2G_FOR16(G2G_LD##SET)               \
   }
 #define G2G_STX(q, r)                                      \
-  if ((q) < PF && threadIdx.x + (q) * NT < n_) {           \
-    const int e = threadIdx.x + (q) * NT;                  \
+  if ((q) < PF && threadIdx.x + (q) * NTC < n_) {           \
+    const int e = threadIdx.x + (q) * NTC;                  \
     if (c_ < 4) dst_[e] = r##q;                            \
     else dst_[(e / Nz) * NZP + e % Nz] = r##q;             \
   }
@@ -407,7 +407,7 @@ __global__ __launch_bounds__(NT) void k_contract2(cx_t<R>* __restrict__ S,
   }
   G2G_PREFETCH(0, F)
   G2G_PREFETCH(1, G)
-  for (int i = threadIdx.x; i < 4 * Nz3; i += NT) {
+  for (int i = threadIdx.x; i < 4 * Nz3; i += NTC) {
     const int p = i / Nz3, e = i - p * Nz3;
     T2 vv = zero;
     if (VAL(p)) vv = S[((int64_t)PX(p) * Ny2 + PY(p)) * Nz3 + e];
@@ -858,9 +858,18 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
       if (op.Nz * op.Nz <= 4 * NT) launch(k_contract<R, 4>);
       else launch(k_contract<R, 16>);
     } else if (op.Nz * op.Nz <= 4 * NT) {
-      launch(k_contract2<R, 4>);
+      launch(k_contract2<R, 4, NT>);
     } else {
-      launch(k_contract2<R, 16>);
+      // Nz > 32: 512 threads (8 lanes per receiver row at Nz = 64) keep twice
+      // the waves per CU that the LDS budget (one or two blocks) allows
+      int sl2 = 0;
+      while ((op.Nz << (sl2 + 1)) <= 2 * NT) sl2++;
+      auto kern = k_contract2<R, 8, 2 * NT>;
+      if (sm > 65536) {
+        G2G_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    160 * 1024));
+      }
+      hipLaunchKernelGGL(kern, dim3(op.Nx * op.Ny), dim3(2 * NT), sm, s, b.S, b.G, g, op.BS, sl2);
     }
   }
   if (timed) G2G_HIP(hipEventRecord(op.ev[2], s));
