@@ -853,7 +853,7 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
       };
       if (op.Nz <= 32) launch_mma(k_contract_mma<R, 3, 6, 1>, 192);
       else if (op.Nz == 48) launch_mma(k_contract_mma<R, 3, 16, 1>, 192);
-      else launch_mma(k_contract_mma<R, 4, 16, 1>, 256);
+      else launch_mma(k_contract_mma<R, 6, 11, 1>, 384);
     } else if (op.contraction == 3) {
       if (op.Nz * op.Nz <= 4 * NT) launch(k_contract<R, 4>);
       else launch(k_contract<R, 16>);
@@ -929,7 +929,7 @@ void op_apply2(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin0, const cx_t<R>
     G2G_CHECK(sm <= 160 * 1024, -3, "two-vector contraction does not fit in LDS for this Nz");
     if (op.Nz <= 32) launch_mma(k_contract_mma<R, 3, 6, 2>, 192);
     else if (op.Nz == 48) launch_mma(k_contract_mma<R, 3, 16, 2>, 192);
-    else launch_mma(k_contract_mma<R, 4, 16, 2>, 256);
+    else launch_mma(k_contract_mma<R, 6, 11, 2>, 384);  // 12 tiles, two per wave
   }
   for (int r = 0; r < 2; r++) {
     pow2_dispatch(Ny2, [&](auto LC) {
