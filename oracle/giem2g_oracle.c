@@ -1503,3 +1503,600 @@ void or_solve(const or_model *m, const double *Gspec, const double *csiga_d,
   free(rhs);
   free(guess);
 }
+
+/* ======================================================================== */
+/* Receiver recalculation: the continuation of the IE solution to receivers */
+/* (Continuation_Function_Module.F90, calc_rc_tensor_module.F90,            */
+/* rc_kernel_hankel_module.F90, sources_module.F90:26-66).                  */
+/* ======================================================================== */
+
+/* PrepareRecvs (data_types_module.F90:124-152): receiver layer (from the
+ * depth as given), the anomaly cell index, and depths strictly inside an
+ * anomaly cell moved up to that cell's top border. */
+void or_prepare_recvs(const or_model *m, int Nr, const double *depth,
+                      double *zr, int *layer, int *cell) {
+  int r, Iz, Nz = m->Nz;
+  for (r = 0; r < Nr; r++) {
+    double z = depth[r];
+    int l = get_layer(m, z);
+    layer[r] = l < 1 ? 1 : l;
+    zr[r] = z;
+    if (z <= m->z[0]) {
+      cell[r] = 0;
+    } else if (z >= m->z[Nz]) {
+      cell[r] = Nz + 1;
+    } else {
+      cell[r] = -1;
+      for (Iz = 1; Iz <= Nz; Iz++) {
+        if (fabs(z / m->z[Iz] - 1.0) < 1e-7) {
+          cell[r] = Iz;
+          break;
+        } else if (z > m->z[Iz - 1] && z < m->z[Iz]) {
+          zr[r] = m->z[Iz - 1];
+          cell[r] = Iz - 1;
+          break;
+        }
+      }
+    }
+  }
+}
+
+/* PrepareExps2 (VolumeBesselTransforms.F90:342-371): the two edges of the
+ * source rectangle seen from the point, in both directions. */
+static void vb_prepare_exps2(double phi, double alpha, double beta, vb_edt *e) {
+  int i, k;
+  double cphi = cos(phi), sphi = sin(phi);
+  for (i = 0; i < VB_N; i++) {
+    double x[2], y[2];
+    x[0] = e->explt[i] * cphi * 0.5;
+    x[1] = x[0] + alpha * e->explt[i] * 0.5;
+    y[0] = e->explt[i] * sphi * 0.5;
+    y[1] = y[0] + beta * e->explt[i] * 0.5;
+    for (k = 0; k < 2; k++) {
+      e->xy[0][i][k] = x[k];
+      e->xy2[0][i][k] = x[k] * x[k];
+      e->xexp2[0][i][k] = exp(-(x[k] * x[k]));
+      e->xerf[0][i][k] = erf(x[k]);
+      e->xy[1][i][k] = y[k];
+      e->xy2[1][i][k] = y[k] * y[k];
+      e->xexp2[1][i][k] = exp(-(y[k] * y[k]));
+      e->xerf[1][i][k] = erf(y[k]);
+    }
+  }
+}
+
+/* first difference f(2)-f(1) of the SINGLE_INT_* routines */
+#define D1(f) ((f)[1] - (f)[0])
+
+/* SINGLE_INT_D0 (:553-567) */
+static void vb_single_int_d0(const vb_edt *e, int i, int j, double *R) {
+  const double *x = e->xy[j][i], *x2 = e->xy2[j][i];
+  double f[2], a, b, c;
+  int k;
+  for (k = 0; k < 2; k++) f[k] = e->xerf[j][i][k] * sqrt(PI);
+  a = D1(f);
+  R[0] = a * 0.5;
+  for (k = 0; k < 2; k++) f[k] = x[k] * e->xexp2[j][i][k];
+  b = D1(f);
+  R[1] = a - 2.0 * b;
+  for (k = 0; k < 2; k++) f[k] = f[k] * x2[k];
+  c = D1(f);
+  R[2] = 6.0 * a - 12.0 * b - 8.0 * c;
+}
+
+/* SINGLE_INT_D1 (:569-585) */
+static void vb_single_int_d1(const vb_edt *e, int i, int j, double *R) {
+  const double *x2 = e->xy2[j][i];
+  double f[2];
+  int k;
+  for (k = 0; k < 2; k++) f[k] = e->xexp2[j][i][k];
+  R[0] = D1(f) / 2.0;
+  for (k = 0; k < 2; k++) f[k] = f[k] * x2[k];
+  R[1] = D1(f) * 2.0;
+  for (k = 0; k < 2; k++) f[k] = f[k] * x2[k];
+  R[2] = D1(f) * 8.0;
+}
+
+/* SINGLE_INT_D2 (:587-603) */
+static void vb_single_int_d2(const vb_edt *e, int i, int j, double *R) {
+  const double *x = e->xy[j][i], *x2 = e->xy2[j][i];
+  double f[2], a, b, c;
+  int k;
+  for (k = 0; k < 2; k++) f[k] = x[k] * e->xexp2[j][i][k];
+  a = D1(f);
+  R[0] = -a * 0.5;
+  for (k = 0; k < 2; k++) f[k] = f[k] * x2[k];
+  b = D1(f);
+  R[1] = 2.0 * (a - b);
+  for (k = 0; k < 2; k++) f[k] = f[k] * x2[k];
+  c = D1(f);
+  R[2] = 16.0 * b - 8.0 * c;
+}
+#undef D1
+
+/* outfunc_int2_all (:407-434) */
+static void vb_outfunc_int2(const vb_edt *e, int i, double *R) {
+  double fx[3][3], fy[3][3];
+  vb_single_int_d0(e, i, 0, fx[0]);
+  vb_single_int_d1(e, i, 0, fx[1]);
+  vb_single_int_d2(e, i, 0, fx[2]);
+  vb_single_int_d0(e, i, 1, fy[0]);
+  vb_single_int_d1(e, i, 1, fy[1]);
+  vb_single_int_d2(e, i, 1, fy[2]);
+#define CMB(a, b) (fx[a][2] * fy[b][0] + fx[a][0] * fy[b][2] + 2.0 * fx[a][1] * fy[b][1])
+  R[0] = CMB(0, 0) / e->explt[i];
+  R[1] = CMB(2, 0) * e->explt[i];
+  R[2] = CMB(1, 1) * e->explt[i];
+  R[3] = CMB(1, 0);
+  R[4] = CMB(0, 2) * e->explt[i];
+  R[5] = CMB(0, 1);
+#undef CMB
+}
+
+/* VBTransformWeightsAllSingleInt (:198-229) + CalcWeights22 (:231-262):
+ * weights of a point receiver at offset (x,y) from the source rectangle's
+ * corner; WT (K,i) column-major in the order D0,DXX,DXY,DX,DYY,DY. */
+void or_vbt_weights_single(double x, double y, double hx, double hy,
+                           double rmin, double *WT) {
+  double lms[OR_NTAPS];
+  cplx f1[VB_N];
+  cplx *g = (cplx *)malloc(sizeof(cplx) * VB_N * 6);
+  vb_edt *e = (vb_edt *)malloc(sizeof(vb_edt));
+  double rho = sqrt(x * x + y * y);
+  /* scalings of :222-228: D0 times rho; DXX, DXY, DYY over rho */
+  static const int rscale[6] = {+1, -1, -1, 0, -1, 0};
+  int i, j;
+  vb_init(rmin, lms, f1);
+  vb_calc_exp_lt(log(rho) - log(rmin), e);
+  vb_prepare_exps2(atan2(y, x), hx / rho, hy / rho, e);
+  for (i = 0; i < VB_N; i += 2) {
+    double R[6];
+    vb_outfunc_int2(e, i, R);
+    for (j = 0; j < 6; j++) g[j * VB_N + i] = R[j];
+    vb_outfunc_int2(e, i + 1, R);
+    for (j = 0; j < 6; j++) g[j * VB_N + i + 1] = -R[j];
+  }
+  for (j = 0; j < 6; j++) {
+    cplx *h = g + j * VB_N;
+    fft_c(h, VB_N, VB_FWD);
+    for (i = 0; i < VB_N; i += 2) {
+      h[i] = h[i] / f1[i];
+      h[i + 1] = -h[i + 1] / f1[i + 1];
+    }
+    fft_c(h, VB_N, VB_BWD);
+    for (i = 1; i < VB_N; i += 2) {
+      h[i - 1] = h[i - 1] / (double)VB_N;
+      h[i] = -h[i] / (double)VB_N;
+    }
+    for (i = OR_NFIRST; i <= OR_NLAST; i++) {
+      double v = creal(h[i]);
+      WT[(i - OR_NFIRST) + OR_NTAPS * j] = rscale[j] > 0 ? v * rho : (rscale[j] < 0 ? v / rho : v);
+    }
+  }
+  free(e);
+  free(g);
+}
+
+/* RC weight order (rc_kernel_hankel_module.F90:30-36) is the IE order. */
+/* E entries REXX..REZZ and H entries RHXX..RHZY (const_module.F90:112-130) */
+enum { REXX = 0, REXY, REXZ, REYY, REYZ, REZX, REZY, REZZ };
+enum { RHXX = 0, RHXY, RHXZ, RHYX, RHYZ, RHZX, RHZY };
+
+/* Calc_Integral_U (rc_kernel_hankel_module.F90:56-150) with Calc_Recv_Part
+ * (187-230), Calc_Ftb (232-254) and Calc_G_not_match (152-186).  GE is
+ * complex (8,Nz), GH complex (7,Nz), both accumulated. */
+static void rc_integral_U(const or_model *m, double zrecv, int rl, int N,
+                          double lms, const double *WT, cplx *GE, cplx *GH) {
+  int Nz = m->Nz, Nl = m->Nl, I, l, k;
+  cplx Arr[OR_MAXL + 1][2], p[OR_MAXL + 1][2], q[OR_MAXL + 1][2], eta[OR_MAXL + 1];
+  cplx(*A)[OR_MAXL + 1][2] = calloc(OR_MAXL + 1, sizeof(*A));
+  cplx *expz = malloc(sizeof(cplx) * (Nz + 1));
+  cplx(*pez)[2] = malloc(sizeof(cplx) * 2 * (Nz + 1));
+  cplx(*qez)[2] = malloc(sizeof(cplx) * 2 * (Nz + 1));
+  cplx(*w1)[2] = malloc(sizeof(cplx) * 2 * (Nz + 1));
+  cplx(*w2)[2] = malloc(sizeof(cplx) * 2 * (Nz + 1));
+  cplx(*Ftb)[3] = calloc((size_t)3 * (Nz + 1), sizeof(cplx));
+  cplx e, eta_r, eta_zr, eta_dr, iwm = m->iwm;
+  double lm2 = lms * lms;
+  int N1, N2, Iz;
+  calc_apq(m, lms, Arr, p, q, eta, A);
+  for (Iz = 1; Iz <= Nz; Iz++) expz[Iz] = cexp(-eta[m->lnum[Iz]] * (m->z[Iz] - m->z[Iz - 1]));
+  /* Calc_Recv_Part */
+  eta_r = eta[rl];
+  eta_zr = eta_r * zrecv;
+  eta_dr = rl > 1 ? eta_r * m->depth[rl - 1] * 2.0 : 0.0;
+  N1 = N + 1 > 1 ? N + 1 : 1;
+  N2 = N < Nz ? N : Nz;
+  for (I = N1; I <= Nz; I++) {
+    cplx e1, e2;
+    l = m->lnum[I];
+    e1 = cexp(-m->z[I - 1] * eta[l] + eta_zr);
+    e2 = cexp(-m->z[I - 1] * eta[l] - eta_zr + eta_dr);
+    w1[I][0] = e2 * p[rl][0] + e1;
+    w1[I][1] = (-e2 * p[rl][0] + e1) * eta_r;
+    w2[I][0] = e2 * p[rl][1] + e1;
+    w2[I][1] = (-e2 * p[rl][1] + e1) * eta_r;
+  }
+  for (I = 1; I <= N2; I++) {
+    cplx e1, e2;
+    l = m->lnum[I];
+    e1 = cexp(m->z[I] * eta[l] - eta_zr);
+    /* below the last layer eta_dr is HUGE (:219): the reflected term is 0 */
+    e2 = rl < Nl ? cexp(m->z[I] * eta[l] + eta_zr - eta_r * m->depth[rl] * 2.0) : 0.0;
+    w1[I][0] = e2 * q[rl][0] + e1;
+    w1[I][1] = (e2 * q[rl][0] - e1) * eta_r;
+    w2[I][0] = e2 * q[rl][1] + e1;
+    w2[I][1] = (e2 * q[rl][1] - e1) * eta_r;
+  }
+  /* Calc_pexpz_All / Calc_qexpz_All (apq_module.F90:112-177) */
+  l = m->lnum[0];
+  if (l > 1)
+    e = cexp(-2.0 * eta[l] * (m->z[0] - m->depth[l - 1]));
+  else if (l == 1)
+    e = cexp(-2.0 * eta[1] * (m->z[0]));
+  else
+    e = 1.0;
+  for (I = 1; I <= Nz; I++) {
+    if (l != m->lnum[I]) e = 1.0;
+    l = m->lnum[I];
+    for (k = 0; k < 2; k++) pez[I][k] = e * p[l][k];
+    e = e * expz[I] * expz[I];
+  }
+  l = m->lnum[Nz];
+  e = (l != Nl) ? cexp(-2.0 * eta[l] * (m->depth[l] - m->z[Nz])) : 0.0;
+  for (I = Nz; I >= 1; I--) {
+    if (l != m->lnum[I]) e = 1.0;
+    l = m->lnum[I];
+    for (k = 0; k < 2; k++) qez[I][k] = e * q[l][k];
+    e = e * expz[I] * expz[I];
+  }
+  /* Calc_Ftb: cells above the receiver use p (sign -1), below it q (+1) */
+  for (I = 1; I <= Nz; I++) {
+    int above = (N >= 1 && N <= Nz) ? (I <= N) : (N > Nz);
+    cplx(*pq)[2] = above ? pez : qez;
+    double s = above ? -1.0 : 1.0;
+    cplx a, a1, e1[2], e2[2];
+    l = m->lnum[I];
+    a = 1.0 - expz[I];
+    a1 = a / eta[l];
+    for (k = 0; k < 2; k++) {
+      e1[k] = 1.0 + pq[I][k] * expz[I];
+      e2[k] = (1.0 - pq[I][k] * expz[I]) * s;
+    }
+    Ftb[I][0] = a1 * e1[0];
+    Ftb[I][1] = a1 * e1[1];
+    Ftb[I][2] = a * e2[1];
+  }
+  /* Calc_G_not_match and the accumulation (:116-145) */
+  for (I = 1; I <= Nz; I++) {
+    cplx a1, a2, f[3], v1[2], v2[2], sr = m->csigma[rl];
+    cplx G1, G3DZ, G2DZ, G2DZ0, G2G2DZZ, G3, G1DZ, G2, G11;
+    cplx *ge = GE + 8 * (I - 1), *gh = GH + 7 * (I - 1);
+    l = m->lnum[I];
+    a1 = A[rl][l][0];
+    a2 = A[rl][l][1];
+    for (k = 0; k < 3; k++) f[k] = Ftb[I][k];
+    v1[0] = w1[I][0];
+    v1[1] = w1[I][1];
+    v2[0] = w2[I][0];
+    v2[1] = w2[I][1];
+    G1 = a1 * f[0] * v1[0];
+    G3DZ = (a2 * f[2] * v2[1]) / sr;
+    G2DZ = (a2 * f[1] * v2[1]) / sr;
+    G2DZ0 = (a2 * f[2] * v2[0]) / sr;
+    G2G2DZZ = lm2 * (a2 * f[1] * v2[0]) / sr;
+    G3 = (a2 * f[2] * v2[0] - a1 * f[0] * v1[1]);
+    G1DZ = a1 * f[0] * v1[1];
+    G2 = a2 * f[1] * v2[0];
+    G11 = G1 * iwm;
+    ge[REXX] = ge[REXX] + G11 * WT[IE_D0] + (G3DZ + G11) * WT[IE_DXX];
+    ge[REXY] = ge[REXY] + (G3DZ + G11) * WT[IE_DXY];
+    ge[REXZ] = ge[REXZ] + G2DZ * WT[IE_DX];
+    ge[REYY] = ge[REYY] + G11 * WT[IE_D0] + (G3DZ + G11) * WT[IE_DYY];
+    ge[REYZ] = ge[REYZ] + G2DZ * WT[IE_DY];
+    ge[REZX] = ge[REZX] + G2DZ0 * WT[IE_DX];
+    ge[REZY] = ge[REZY] + G2DZ0 * WT[IE_DY];
+    ge[REZZ] = ge[REZZ] + G2G2DZZ * WT[IE_D0];
+    gh[RHXX] = gh[RHXX] + G3 * WT[IE_DXY];
+    gh[RHXY] = gh[RHXY] + G3 * WT[IE_DYY] - G1DZ * WT[IE_D0];
+    gh[RHXZ] = gh[RHXZ] + G2 * WT[IE_DY];
+    gh[RHYX] = gh[RHYX] + G1DZ * WT[IE_D0] - G3 * WT[IE_DXX];
+    gh[RHYZ] = gh[RHYZ] - G2 * WT[IE_DX];
+    gh[RHZX] = gh[RHZX] - G1 * WT[IE_DY];
+    gh[RHZY] = gh[RHZY] + G1 * WT[IE_DX];
+  }
+  free(A);
+  free(expz);
+  free(pez);
+  free(qez);
+  free(w1);
+  free(w2);
+  free(Ftb);
+}
+
+void or_rc_integral_U(const or_model *m, double zrecv, int layer, int cell,
+                      double lm, const double *W, double *GE, double *GH) {
+  rc_integral_U(m, zrecv, layer, cell, lm, W, (cplx *)GE, (cplx *)GH);
+}
+
+int64_t or_rc_block_size(int Nz, int Nr) { return 15 * (int64_t)Nr * Nz; }
+
+/* CalcRecalculationGreenTensor3dElement (calc_rc_tensor_module.F90:164-230)
+ * for one lateral offset (lx, ly) (cells, receiver minus source) and every
+ * receiver.  blk: 15*Nr*Nz complex, entry e (E: 0..7, H: 8..14), receiver
+ * r, source layer z at (e*Nr + r)*Nz + z. */
+void or_rc_tensor_element(const or_model *m, int Nr, const double *xs,
+                          const double *ys, const double *zr, const int *layer,
+                          const int *cell, int lx, int ly, double *blk_d) {
+  int Nz = m->Nz, r, K, c, iz;
+  double rmin = sqrt(m->dx * m->dx + m->dy * m->dy) / 2.0; /* :67 */
+  double lms[OR_NTAPS], WT0[OR_NTAPS * 6], WT[OR_NTAPS * 6];
+  cplx *GE = malloc(sizeof(cplx) * 8 * Nz), *GH = malloc(sizeof(cplx) * 7 * Nz);
+  cplx *blk = (cplx *)blk_d;
+  or_vbt_lms(rmin, lms);
+  for (r = 0; r < Nr; r++) {
+    double x = lx * m->dx - m->dx / 2.0 + xs[r]; /* :191-192 */
+    double y = ly * m->dy - m->dy / 2.0 + ys[r];
+    double W0 = 0;
+    or_vbt_weights_single(x, y, m->dx, m->dy, rmin, WT0);
+    /* column map :196-201 (WT0 order D0,DXX,DXY,DX,DYY,DY) */
+    memcpy(WT + OR_NTAPS * IE_D0, WT0 + OR_NTAPS * 0, sizeof(double) * OR_NTAPS);
+    memcpy(WT + OR_NTAPS * IE_DXX, WT0 + OR_NTAPS * 1, sizeof(double) * OR_NTAPS);
+    memcpy(WT + OR_NTAPS * IE_DXY, WT0 + OR_NTAPS * 2, sizeof(double) * OR_NTAPS);
+    memcpy(WT + OR_NTAPS * IE_DX, WT0 + OR_NTAPS * 3, sizeof(double) * OR_NTAPS);
+    memcpy(WT + OR_NTAPS * IE_DYY, WT0 + OR_NTAPS * 4, sizeof(double) * OR_NTAPS);
+    memcpy(WT + OR_NTAPS * IE_DY, WT0 + OR_NTAPS * 5, sizeof(double) * OR_NTAPS);
+    /* :202 W0 = MAXVAL(ABS(WT)): one scalar over all taps and columns */
+    for (K = 0; K < OR_NTAPS * 6; K++)
+      if (fabs(WT[K]) > W0) W0 = fabs(WT[K]);
+    memset(GE, 0, sizeof(cplx) * 8 * Nz);
+    memset(GH, 0, sizeof(cplx) * 7 * Nz);
+    for (K = 0; K < OR_NTAPS; K++) {
+      double Wm = 0, W[6], lm;
+      for (c = 0; c < 6; c++) {
+        double t = fabs(WT[K + OR_NTAPS * c] / W0);
+        if (c == 0 || t > Wm) Wm = t;
+      }
+      if (Wm > WT_THRESHOLD) {
+        lm = lms[K];
+        W[IE_DXX] = WT[K + OR_NTAPS * IE_DXX] / lm;
+        W[IE_DYY] = WT[K + OR_NTAPS * IE_DYY] / lm;
+        W[IE_DXY] = WT[K + OR_NTAPS * IE_DXY] / lm;
+        W[IE_DX] = WT[K + OR_NTAPS * IE_DX] * lm;
+        W[IE_DY] = WT[K + OR_NTAPS * IE_DY] * lm;
+        W[IE_D0] = WT[K + OR_NTAPS * IE_D0] * lm;
+        rc_integral_U(m, zr[r], layer[r], cell[r], lm, W, GE, GH);
+      }
+    }
+    /* :223-224 transpose and scale by 1/(4 pi) */
+    for (iz = 0; iz < Nz; iz++) {
+      for (c = 0; c < 8; c++) blk[((int64_t)c * Nr + r) * Nz + iz] = GE[c + 8 * iz] / PI / 4.0;
+      for (c = 0; c < 7; c++) blk[((int64_t)(8 + c) * Nr + r) * Nz + iz] = GH[c + 7 * iz] / PI / 4.0;
+    }
+  }
+  free(GE);
+  free(GH);
+}
+
+/* CalcRecalculationGreenTensor (:37-162) on one rank: every point of the
+ * 2Nx x 2Ny offset grid, p = ix*2Ny + iy with lx = ix (ix < Nx) or
+ * ix - 2Nx (ix > Nx), the same for y; the ix = Nx and iy = Ny offsets are
+ * zero (:70-79, :110-113). */
+void or_rc_tensor_realspace(const or_model *m, int Nr, const double *xs,
+                            const double *ys, const double *zr,
+                            const int *layer, const int *cell, double *Greal,
+                            int nthreads) {
+  int Nx = m->Nx, Ny = m->Ny;
+  int64_t BS = or_rc_block_size(m->Nz, Nr), p, NP = 4 * (int64_t)Nx * Ny;
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+  for (p = 0; p < NP; p++) {
+    int ix = (int)(p / (2 * Ny)), iy = (int)(p % (2 * Ny));
+    double *blk = Greal + 2 * BS * p;
+    if (ix == Nx || iy == Ny) {
+      memset(blk, 0, sizeof(cplx) * BS);
+      continue;
+    }
+    or_rc_tensor_element(m, Nr, xs, ys, zr, layer, cell, ix < Nx ? ix : ix - 2 * Nx,
+                         iy < Ny ? iy : iy - 2 * Ny, blk);
+  }
+}
+
+/* CalcFFTofRCTensor (Continuation_Function_Module.F90:349-383): forward 2-D
+ * DFT (exp(-i)) of every tensor column over the 2Nx x 2Ny grid. */
+void or_rc_tensor_fft(const or_model *m, int Nr, const double *Greal_d,
+                      double *Gspec_d, int nthreads) {
+  int Nx2 = 2 * m->Nx, Ny2 = 2 * m->Ny;
+  int64_t BS = or_rc_block_size(m->Nz, Nr), e;
+  const cplx *Greal = (const cplx *)Greal_d;
+  cplx *Gspec = (cplx *)Gspec_d;
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+  for (e = 0; e < BS; e++) {
+    cplx *F = malloc(sizeof(cplx) * Nx2 * Ny2), *col = malloc(sizeof(cplx) * Nx2);
+    int ix, iy;
+    for (ix = 0; ix < Nx2; ix++)
+      for (iy = 0; iy < Ny2; iy++) F[iy + Ny2 * ix] = Greal[e + BS * (iy + (int64_t)Ny2 * ix)];
+    for (ix = 0; ix < Nx2; ix++) fft_c(F + Ny2 * ix, Ny2, -1);
+    for (iy = 0; iy < Ny2; iy++) {
+      for (ix = 0; ix < Nx2; ix++) col[ix] = F[iy + Ny2 * ix];
+      fft_c(col, Nx2, -1);
+      for (ix = 0; ix < Nx2; ix++) Gspec[e + BS * (iy + (int64_t)Ny2 * ix)] = col[ix];
+    }
+    free(F);
+    free(col);
+  }
+}
+
+/* The 17 block products of APPLY_RC_E_OP / APPLY_RC_H_OP
+ * (Continuation_Function_Module.F90:408-481): (entry, input component,
+ * output component 0..5 = Ex,Ey,Ez,Hx,Hy,Hz, sign).  RHYY is RHXX with
+ * alpha = -1 (:468, const_module.F90:127). */
+static const int RC_TERMS[17][4] = {
+    {REXX, 0, 0, 1}, {REXY, 1, 0, 1}, {REXZ, 2, 0, 1},
+    {REXY, 0, 1, 1}, {REYY, 1, 1, 1}, {REYZ, 2, 1, 1},
+    {REZX, 0, 2, 1}, {REZY, 1, 2, 1}, {REZZ, 2, 2, 1},
+    {8 + RHXX, 0, 3, 1}, {8 + RHXY, 1, 3, 1}, {8 + RHXZ, 2, 3, 1},
+    {8 + RHYX, 0, 4, 1}, {8 + RHXX, 1, 4, -1}, {8 + RHYZ, 2, 4, 1},
+    {8 + RHZX, 0, 5, 1}, {8 + RHZY, 1, 5, 1}};
+
+/* LoadEint (:241-265): j = (sigma_a - sigma_b) * E, component-major. */
+static cplx *rc_current(const or_model *m, const cplx *csiga, const cplx *E) {
+  int Nx = m->Nx, Ny = m->Ny, Nz = m->Nz, x, y, z, c;
+  int64_t nc = (int64_t)Nx * Ny * Nz;
+  cplx *J = malloc(sizeof(cplx) * 3 * nc);
+  for (c = 0; c < 3; c++)
+    for (z = 0; z < Nz; z++) {
+      cplx csigb;
+      double sq;
+      layer_sig(m, z, &csigb, &sq);
+      for (y = 0; y < Ny; y++)
+        for (x = 0; x < Nx; x++) {
+          int64_t ci = x + (int64_t)Nx * (y + (int64_t)Ny * z);
+          J[ci + nc * c] = (csiga[ci] - csigb) * E[ci + nc * c];
+        }
+    }
+  return J;
+}
+
+/* ReCalculation (:198-240): Ea, Ha complex (Nr,3,Nx,Ny) Fortran order. */
+void or_rc_apply(const or_model *m, int Nr, const double *Gspec_d,
+                 const double *csiga_d, const double *Eint_d, double *Ea_d,
+                 double *Ha_d, int nthreads) {
+  int Nx = m->Nx, Ny = m->Ny, Nz = m->Nz, Nx2 = 2 * Nx, Ny2 = 2 * Ny;
+  int64_t NS = (int64_t)Nx2 * Ny2, nc = (int64_t)Nx * Ny * Nz, s, p;
+  int64_t BS = or_rc_block_size(Nz, Nr);
+  const cplx *Gspec = (const cplx *)Gspec_d;
+  cplx *J = rc_current(m, (const cplx *)csiga_d, (const cplx *)Eint_d);
+  cplx *F = calloc((size_t)NS * 3 * Nz, sizeof(cplx)); /* [c*Nz+z][ix][iy] */
+  cplx *O = calloc((size_t)NS * 6 * Nr, sizeof(cplx)); /* [c'*Nr+r][ix][iy] */
+  cplx *Ea = (cplx *)Ea_d, *Ha = (cplx *)Ha_d;
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+  for (s = 0; s < 3 * Nz; s++) {
+    int c = (int)(s / Nz), z = (int)(s % Nz), x, y;
+    cplx *A = F + NS * s, *col = malloc(sizeof(cplx) * Nx2);
+    for (x = 0; x < Nx; x++)
+      for (y = 0; y < Ny; y++)
+        A[y + (int64_t)Ny2 * x] = J[x + (int64_t)Nx * (y + (int64_t)Ny * z) + nc * c];
+    for (x = 0; x < Nx; x++) fft_c(A + Ny2 * x, Ny2, -1);
+    for (y = 0; y < Ny2; y++) {
+      for (x = 0; x < Nx2; x++) col[x] = A[y + (int64_t)Ny2 * x];
+      fft_c(col, Nx2, -1);
+      for (x = 0; x < Nx2; x++) A[y + (int64_t)Ny2 * x] = col[x];
+    }
+    free(col);
+  }
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (p = 0; p < NS; p++) {
+    const cplx *blk = Gspec + BS * p;
+    int t, r, z;
+    for (t = 0; t < 17; t++) {
+      int e = RC_TERMS[t][0], ci = RC_TERMS[t][1], co = RC_TERMS[t][2];
+      double sg = RC_TERMS[t][3];
+      for (r = 0; r < Nr; r++) {
+        cplx acc = 0;
+        for (z = 0; z < Nz; z++) acc += blk[((int64_t)e * Nr + r) * Nz + z] * F[p + NS * (ci * Nz + z)];
+        O[p + NS * (co * Nr + r)] += sg * acc;
+      }
+    }
+  }
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+  for (s = 0; s < 6 * Nr; s++) {
+    int co = (int)(s / Nr), r = (int)(s % Nr), x, y;
+    cplx *A = O + NS * s, *col = malloc(sizeof(cplx) * Nx2);
+    for (x = 0; x < Nx2; x++) fft_c(A + Ny2 * x, Ny2, +1);
+    for (y = 0; y < Ny; y++) {
+      for (x = 0; x < Nx2; x++) col[x] = A[y + (int64_t)Ny2 * x];
+      fft_c(col, Nx2, +1);
+      for (x = 0; x < Nx2; x++) A[y + (int64_t)Ny2 * x] = col[x];
+    }
+    for (y = 0; y < Ny; y++)
+      for (x = 0; x < Nx; x++) {
+        cplx v = A[y + (int64_t)Ny2 * x] / (4.0 * Nx * Ny); /* RC_OP_FFTW_BWD :347 */
+        int64_t o = r + (int64_t)Nr * ((co % 3) + 3 * (x + (int64_t)Nx * y));
+        if (co < 3)
+          Ea[o] = v;
+        else
+          Ha[o] = v;
+      }
+    free(col);
+  }
+  free(J);
+  free(F);
+  free(O);
+}
+
+/* Direct sum of the convolution the FFT path computes: no FFT, no padding. */
+void or_rc_apply_bruteforce(const or_model *m, int Nr, const double *Greal_d,
+                            const double *csiga_d, const double *Eint_d,
+                            double *Ea_d, double *Ha_d, int nthreads) {
+  int Nx = m->Nx, Ny = m->Ny, Nz = m->Nz;
+  int64_t nc = (int64_t)Nx * Ny * Nz, BS = or_rc_block_size(Nz, Nr), q;
+  const cplx *Greal = (const cplx *)Greal_d;
+  cplx *J = rc_current(m, (const cplx *)csiga_d, (const cplx *)Eint_d);
+  cplx *Ea = (cplx *)Ea_d, *Ha = (cplx *)Ha_d;
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+  for (q = 0; q < (int64_t)Nx * Ny; q++) {
+    int xr = (int)(q % Nx), yr = (int)(q / Nx), xs, ys, t, r, z;
+    cplx acc[6 * 64];
+    for (t = 0; t < 6 * Nr; t++) acc[t] = 0;
+    for (ys = 0; ys < Ny; ys++)
+      for (xs = 0; xs < Nx; xs++) {
+        int lx = xr - xs, ly = yr - ys;
+        int ix = lx >= 0 ? lx : lx + 2 * Nx, iy = ly >= 0 ? ly : ly + 2 * Ny;
+        const cplx *blk = Greal + BS * (iy + (int64_t)2 * Ny * ix);
+        for (t = 0; t < 17; t++) {
+          int e = RC_TERMS[t][0], ci = RC_TERMS[t][1], co = RC_TERMS[t][2];
+          double sg = RC_TERMS[t][3];
+          for (r = 0; r < Nr; r++)
+            for (z = 0; z < Nz; z++)
+              acc[co * Nr + r] += sg * blk[((int64_t)e * Nr + r) * Nz + z] *
+                                  J[xs + (int64_t)Nx * (ys + (int64_t)Ny * z) + nc * ci];
+        }
+      }
+    for (t = 0; t < 6 * Nr; t++) {
+      int co = t / Nr, r = t % Nr;
+      int64_t o = r + (int64_t)Nr * ((co % 3) + 3 * (xr + (int64_t)Nx * yr));
+      if (co < 3)
+        Ea[o] = acc[t];
+      else
+        Ha[o] = acc[t];
+    }
+  }
+  free(J);
+}
+
+/* PlaneWave (sources_module.F90:26-66): the normal field at receiver depths
+ * for one polarization (0 = EX, 1 = EY).  F0 complex (Nr,6), components
+ * Ex,Ey,Ez,Hx,Hy,Hz. */
+void or_plane_wave(const or_model *m, int pol, int Nr, const double *zrecv,
+                   double *F0_d) {
+  int Nl = m->Nl, r, l;
+  cplx Arr[OR_MAXL + 1][2], p[OR_MAXL + 1][2], q[OR_MAXL + 1][2], eta[OR_MAXL + 1], t1;
+  cplx(*A)[OR_MAXL + 1][2] = calloc(OR_MAXL + 1, sizeof(*A));
+  cplx *F0 = (cplx *)F0_d;
+  memset(F0, 0, sizeof(cplx) * 6 * Nr);
+  calc_apq(m, 0.0, Arr, p, q, eta, A);
+  t1 = Nl > 1 ? 1.0 + q[1][0] * cexp(-m->depth[1] * eta[1] * 2.0) : 1.0;
+  t1 = A[1][1][0] * t1;
+  for (r = 0; r < Nr; r++) {
+    cplx t2, e1, e2, G0, G1;
+    l = get_layer(m, zrecv[r]);
+    l = l < 1 ? 1 : l;
+    t2 = A[l][1][0] / t1;
+    e1 = cexp(-eta[l] * zrecv[r]);
+    e2 = l < Nl ? q[l][0] * cexp(-eta[l] * (2.0 * m->depth[l] - zrecv[r])) : 0.0;
+    G0 = t2 * (e1 + e2);
+    G1 = -t2 * (e1 - e2) * eta[l] / m->iwm;
+    if (pol == 0) {
+      F0[r + Nr * 0] = G0;
+      F0[r + Nr * 4] = G1;
+    } else {
+      F0[r + Nr * 1] = G0;
+      F0[r + Nr * 3] = -G1;
+    }
+  }
+  free(A);
+}

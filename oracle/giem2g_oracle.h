@@ -105,6 +105,44 @@ void or_solve(const or_model *m, const double *Gspec, const double *csiga,
               int m_in, int maxit, double tol, double *Esol,
               or_gmres_info *info, int nthreads);
 
+/* Receiver recalculation (Continuation_Function_Module.F90,
+ * calc_rc_tensor_module.F90, rc_kernel_hankel_module.F90).
+ *   receivers     : xs, ys lateral shifts (m) from the cell centres, zr depths
+ *                   after or_prepare_recvs, layer = background layer,
+ *                   cell = anomaly cell index (0 above, Nz+1 below)
+ *   RC tensor     : per offset point p = ix*2Ny + iy of the 2Nx x 2Ny grid a
+ *                   block of 15*Nr*Nz complex; entry e (0..7 = REXX, REXY,
+ *                   REXZ, REYY, REYZ, REZX, REZY, REZZ; 8..14 = RHXX, RHXY,
+ *                   RHXZ, RHYX, RHYZ, RHZX, RHZY), receiver r, source layer z
+ *                   at (e*Nr + r)*Nz + z
+ *   Ea, Ha        : complex (Nr,3,Nx,Ny) Fortran order                       */
+void or_prepare_recvs(const or_model *m, int Nr, const double *depth,
+                      double *zr, int *layer, int *cell);
+void or_vbt_weights_single(double x, double y, double hx, double hy,
+                           double rmin, double *WT /*501*6 (K,i), D0,DXX,DXY,DX,DYY,DY*/);
+void or_rc_integral_U(const or_model *m, double zrecv, int layer, int cell,
+                      double lm, const double *W /*6 IE order*/,
+                      double *GE /*complex 8*Nz*/, double *GH /*complex 7*Nz*/);
+int64_t or_rc_block_size(int Nz, int Nr);
+void or_rc_tensor_element(const or_model *m, int Nr, const double *xs,
+                          const double *ys, const double *zr, const int *layer,
+                          const int *cell, int lx, int ly, double *blk);
+void or_rc_tensor_realspace(const or_model *m, int Nr, const double *xs,
+                            const double *ys, const double *zr,
+                            const int *layer, const int *cell, double *Greal,
+                            int nthreads);
+void or_rc_tensor_fft(const or_model *m, int Nr, const double *Greal,
+                      double *Gspec, int nthreads);
+void or_rc_apply(const or_model *m, int Nr, const double *Gspec,
+                 const double *csiga, const double *Eint, double *Ea,
+                 double *Ha, int nthreads);
+void or_rc_apply_bruteforce(const or_model *m, int Nr, const double *Greal,
+                            const double *csiga, const double *Eint,
+                            double *Ea, double *Ha, int nthreads);
+/* PlaneWave (sources_module.F90:26-66); F0 complex (Nr,6) Ex,Ey,Ez,Hx,Hy,Hz */
+void or_plane_wave(const or_model *m, int pol, int Nr, const double *zrecv,
+                   double *F0);
+
 /* Radix-2 complex FFT, length n (power of two), in place; sign -1 forward. */
 void or_fft(double *a, int n, int sign);
 

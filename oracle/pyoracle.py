@@ -71,6 +71,20 @@ def lib():
         L.or_solve.argtypes = [C.c_void_p, cdp, cdp, cdp, C.c_void_p, C.c_int, C.c_int, C.c_int,
                                C.c_double, cdp, C.POINTER(GmresInfo), C.c_int]
         L.or_fft.argtypes = [cdp, C.c_int, C.c_int]
+        ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        L.or_prepare_recvs.argtypes = [C.c_void_p, C.c_int, dp, dp, ip, ip]
+        L.or_vbt_weights_single.argtypes = [C.c_double] * 5 + [dp]
+        L.or_rc_integral_U.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_double, dp,
+                                       cdp, cdp]
+        L.or_rc_block_size.restype = C.c_int64
+        L.or_rc_block_size.argtypes = [C.c_int, C.c_int]
+        L.or_rc_tensor_element.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, ip, ip, C.c_int,
+                                           C.c_int, cdp]
+        L.or_rc_tensor_realspace.argtypes = [C.c_void_p, C.c_int, dp, dp, dp, ip, ip, cdp, C.c_int]
+        L.or_rc_tensor_fft.argtypes = [C.c_void_p, C.c_int, cdp, cdp, C.c_int]
+        L.or_rc_apply.argtypes = [C.c_void_p, C.c_int, cdp, cdp, cdp, cdp, cdp, C.c_int]
+        L.or_rc_apply_bruteforce.argtypes = [C.c_void_p, C.c_int, cdp, cdp, cdp, cdp, cdp, C.c_int]
+        L.or_plane_wave.argtypes = [C.c_void_p, C.c_int, C.c_int, dp, cdp]
         _lib = L
     return _lib
 
@@ -98,6 +112,13 @@ def _bind(R):
     R.ref_double_integral_u.argtypes = [C.c_int, dp, dp, C.c_double, C.c_int, dp,
                                         C.c_double, dp, cdp]
     R.ref_calc_apq.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, cdp, cdp, cdp, cdp, cdp]
+    if hasattr(R, "ref_vbt_weights_single"):
+        R.ref_vbt_weights_single.argtypes = [C.c_double] * 5 + [dp, dp]
+        R.ref_rc_integral_u.argtypes = [C.c_int, dp, dp, C.c_double, C.c_int, dp, C.c_double,
+                                        C.c_double, dp, cdp, cdp]
+        ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        R.ref_prepare_recvs.argtypes = [C.c_int, dp, dp, C.c_double, C.c_int, dp, C.c_int, dp,
+                                        dp, ip, ip]
     R.ref_fgmres_dense.argtypes = [C.c_int, cdp, cdp, cdp, C.c_int, C.c_int, C.c_int, C.c_double,
                                    cdp, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                    C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -204,6 +225,94 @@ class Model:
                        np.ascontiguousarray(Ebkg, np.complex128), e0, m_out, m_in, maxit, tol,
                        Esol, C.byref(info), nthreads or nthreads_default())
         return Esol, info
+
+
+class Receivers:
+    """Receivers of the continuation (RECEIVER_TYPE, data_types_module.F90:43-47)
+    after PrepareRecvs on a model: lateral shifts (m) from the cell centres,
+    depths, background layer and anomaly cell index."""
+
+    def __init__(self, model, xshift, yshift, depth):
+        self.Nr = len(depth)
+        self.xs = np.ascontiguousarray(xshift, np.float64)
+        self.ys = np.ascontiguousarray(yshift, np.float64)
+        self.depth = np.ascontiguousarray(depth, np.float64)
+        self.zr = np.zeros(self.Nr)
+        self.layer = np.zeros(self.Nr, np.int32)
+        self.cell = np.zeros(self.Nr, np.int32)
+        lib().or_prepare_recvs(model.ptr, self.Nr, self.depth, self.zr, self.layer, self.cell)
+        self.model = model
+
+    def args(self):
+        return (self.Nr, self.xs, self.ys, self.zr, self.layer, self.cell)
+
+    @property
+    def block_size(self):
+        return int(lib().or_rc_block_size(self.model.Nz, self.Nr))
+
+    def tensor_element(self, lx, ly):
+        out = np.zeros(self.block_size, np.complex128)
+        lib().or_rc_tensor_element(self.model.ptr, *self.args(), lx, ly, out)
+        return out
+
+    def tensor_realspace(self, nthreads=None):
+        m = self.model
+        G = np.zeros(4 * m.Nx * m.Ny * self.block_size, np.complex128)
+        lib().or_rc_tensor_realspace(m.ptr, *self.args(), G, nthreads or nthreads_default())
+        return G
+
+    def tensor_fft(self, Greal, nthreads=None):
+        G = np.zeros_like(Greal)
+        lib().or_rc_tensor_fft(self.model.ptr, self.Nr, np.ascontiguousarray(Greal), G,
+                               nthreads or nthreads_default())
+        return G
+
+    def _fields(self):
+        m = self.model
+        n = self.Nr * 3 * m.Nx * m.Ny
+        return np.zeros(n, np.complex128), np.zeros(n, np.complex128)
+
+    def apply(self, Gspec, csiga, E, nthreads=None):
+        Ea, Ha = self._fields()
+        lib().or_rc_apply(self.model.ptr, self.Nr, Gspec, np.ascontiguousarray(csiga, np.complex128),
+                          np.ascontiguousarray(E, np.complex128), Ea, Ha,
+                          nthreads or nthreads_default())
+        return Ea, Ha
+
+    def apply_bruteforce(self, Greal, csiga, E, nthreads=None):
+        Ea, Ha = self._fields()
+        lib().or_rc_apply_bruteforce(self.model.ptr, self.Nr, Greal,
+                                     np.ascontiguousarray(csiga, np.complex128),
+                                     np.ascontiguousarray(E, np.complex128), Ea, Ha,
+                                     nthreads or nthreads_default())
+        return Ea, Ha
+
+    def plane_wave(self, pol):
+        """Normal field at the prepared receiver depths, as the reference
+        driver evaluates it (giem2g.F90:281-283, 416)."""
+        return plane_wave(self.model, pol, self.zr)
+
+
+def plane_wave(model, pol, depths):
+    """PlaneWave at the given depths: complex (Nr, 6) Ex,Ey,Ez,Hx,Hy,Hz."""
+    depths = np.ascontiguousarray(depths, np.float64)
+    F = np.zeros(6 * len(depths), np.complex128)
+    lib().or_plane_wave(model.ptr, pol, len(depths), depths, F)
+    return F.reshape(6, len(depths)).T.copy()
+
+
+def rc_integral_U(model, zr, layer, cell, lm, W):
+    GE = np.zeros(8 * model.Nz, np.complex128)
+    GH = np.zeros(7 * model.Nz, np.complex128)
+    lib().or_rc_integral_U(model.ptr, zr, layer, cell, lm, np.ascontiguousarray(W, np.float64),
+                           GE, GH)
+    return GE.reshape(model.Nz, 8).T.copy(), GH.reshape(model.Nz, 7).T.copy()
+
+
+def vbt_weights_single(x, y, hx, hy, rmin):
+    WT = np.zeros(NTAPS * 6)
+    lib().or_vbt_weights_single(x, y, hx, hy, rmin, WT)
+    return WT.reshape(6, NTAPS).T.copy()
 
 
 def vbt_weights(x, y, hx, hy, rmin):

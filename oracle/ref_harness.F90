@@ -11,6 +11,7 @@ MODULE REF_HARNESS
   USE DATA_TYPES_MODULE
   USE APQ_MODULE
   USE IE_Kernel_Image_Module
+  USE RC_Kernel_Image_Module, ONLY: Calc_Integral_U
   USE VolumeBesselTransforms
   USE IntegralCodes
   USE FGMRES_INTERFACES
@@ -64,6 +65,66 @@ CONTAINS
     REAL(C_DOUBLE), INTENT(OUT) :: WT(Nfirst:Nlast, 6), lms(Nfirst:Nlast)
     CALL VBTransformInit(rmin, lms)
     CALL VBTransformWeightsAllDoubleInt(x, y, hx, hy, WT)
+  END SUBROUTINE
+
+  SUBROUTINE ref_vbt_weights_single(rmin, x, y, hx, hy, WT, lms) BIND(C, NAME='ref_vbt_weights_single')
+    REAL(C_DOUBLE), VALUE, INTENT(IN) :: rmin, x, y, hx, hy
+    REAL(C_DOUBLE), INTENT(OUT) :: WT(Nfirst:Nlast, 6), lms(Nfirst:Nlast)
+    CALL VBTransformInit(rmin, lms)
+    CALL VBTransformWeightsAllSingleInt(x, y, hx, hy, WT)
+  END SUBROUTINE
+
+  ! PrepareRecvs on Nr receivers at the given depths
+  SUBROUTINE ref_prepare_recvs(Nl, csig, thick, omega, Nz, z, Nr, depth, zr, layer, cell) &
+      BIND(C, NAME='ref_prepare_recvs')
+    INTEGER(C_INT), VALUE, INTENT(IN) :: Nl, Nz, Nr
+    REAL(C_DOUBLE), INTENT(IN) :: csig(2*(Nl+1)), thick(*), z(Nz+1), depth(Nr)
+    REAL(C_DOUBLE), VALUE, INTENT(IN) :: omega
+    REAL(C_DOUBLE), INTENT(OUT) :: zr(Nr)
+    INTEGER(C_INT), INTENT(OUT) :: layer(Nr), cell(Nr)
+    TYPE(BKG_DATA_TYPE) :: bkg
+    TYPE(ANOMALY_TYPE) :: anomaly
+    TYPE(RECEIVER_TYPE), POINTER :: recvs(:)
+    INTEGER :: I
+    CALL make_bkg(Nl, csig, thick, omega, bkg)
+    CALL make_anomaly(Nz, z, bkg, anomaly)
+    ALLOCATE(recvs(Nr))
+    DO I = 1, Nr
+      recvs(I)%zrecv = depth(I)
+      recvs(I)%x_shift = 0
+      recvs(I)%y_shift = 0
+    ENDDO
+    CALL PrepareRecvs(recvs, anomaly, bkg)
+    DO I = 1, Nr
+      zr(I) = recvs(I)%zrecv
+      layer(I) = recvs(I)%recv_layer
+      cell(I) = recvs(I)%anom_cell
+    ENDDO
+    DEALLOCATE(recvs)
+  END SUBROUTINE
+
+  ! Calc_Integral_U of the receiver continuation for one receiver (depth as
+  ! given, prepared by PrepareRecvs) and one lambda
+  SUBROUTINE ref_rc_integral_u(Nl, csig, thick, omega, Nz, z, depth, lm, W, GE, GH) &
+      BIND(C, NAME='ref_rc_integral_u')
+    INTEGER(C_INT), VALUE, INTENT(IN) :: Nl, Nz
+    REAL(C_DOUBLE), INTENT(IN) :: csig(2*(Nl+1)), thick(*), z(Nz+1), W(6)
+    REAL(C_DOUBLE), VALUE, INTENT(IN) :: omega, depth, lm
+    COMPLEX(C_DOUBLE_COMPLEX), INTENT(INOUT) :: GE(8, Nz), GH(7, Nz)
+    TYPE(BKG_DATA_TYPE) :: bkg
+    TYPE(ANOMALY_TYPE) :: anomaly
+    TYPE(RECEIVER_TYPE), POINTER :: recvs(:)
+    REAL(REALPARM) :: ctime(2)
+    CALL make_bkg(Nl, csig, thick, omega, bkg)
+    CALL make_anomaly(Nz, z, bkg, anomaly)
+    ALLOCATE(recvs(1))
+    recvs(1)%zrecv = depth
+    recvs(1)%x_shift = 0
+    recvs(1)%y_shift = 0
+    CALL PrepareRecvs(recvs, anomaly, bkg)
+    ctime = 0
+    CALL Calc_Integral_U(anomaly, bkg, recvs(1), lm, W, GE, GH, ctime)
+    DEALLOCATE(recvs)
   END SUBROUTINE
 
   SUBROUTINE ref_double_integral_u(Nl, csig, thick, omega, Nz, z, lm, W, G) &

@@ -6,7 +6,8 @@ Run in the build container (needs oracle/_ref/liboracle_ref.so, which
     python tests/golden/make_golden.py
 
 Every array is produced by the reference's routines (VolumeBesselTransforms,
-Calc_Double_Integral_U, Calc_APQ, GFGMRES) through oracle/ref_harness.F90;
+Calc_Double_Integral_U, Calc_Integral_U of the receiver continuation,
+PrepareRecvs, Calc_APQ, GFGMRES) through oracle/ref_harness.F90;
 the inputs are small synthetic cases.  With oracle/_ref/liboracle_ref_quad.so
 (`make -C oracle refquad`, the reference's VOLUME_WEIGHT_QUAD build) the
 quad-precision weights and the tensor elements built from them are added.
@@ -100,6 +101,51 @@ def ref_tensor_element(W, sig, th, f, z, lx, ly, dx=100.0, dy=100.0, R=None):
     return blk
 
 
+# receiver continuation (calc_rc_tensor_module.F90, rc_kernel_hankel_module.F90)
+RC_WEIGHT_CASES = np.array([[-50.0, -50.0, 100.0, 100.0], [150.0, -250.0, 100.0, 100.0],
+                            [3150.0, 50.0, 100.0, 50.0], [-1234.5, 20000.0, 312.5, 312.5],
+                            [0.3, -50.0, 100.0, 100.0]])
+RC_DEPTHS = np.array([0.0, 25.0, 50.0, 200.0, 430.0, 950.0, 1000.0, 1050.0, 1500.0, 12000.0,
+                      50.000001, 899.0])
+RC_OFFSETS = np.array([[0, 0], [1, 0], [-1, 2], [5, -7], [31, 31], [-12, 2]])
+RC_RECVS = {"c0": np.array([[0.0, 0.0, 0.0], [30.0, -40.0, 130.0], [-45.0, 10.0, 600.0]]),
+            "iface": np.array([[0.0, 0.0, 0.0], [30.0, -40.0, 950.0], [-45.0, 10.0, 1300.0]])}
+
+
+def ref_rc_tensor_element(R, sig, th, f, z, lx, ly, recvs, dx=100.0, dy=100.0):
+    """Block of the real-space receiver tensor from the reference's Fortran
+    routines (single-integral weights and Calc_Integral_U), composed with the
+    masking and tap sum of calc_rc_tensor_module.F90:191-224.  Layout
+    [entry 0..14][receiver][z]."""
+    Nz = len(z) - 1
+    Nr = len(recvs)
+    cs = np.ascontiguousarray(csig(sig, f).view(np.float64))
+    thd = np.array(list(th) + [0.0])
+    rmin = np.hypot(dx, dy) / 2
+    blk = np.zeros((15, Nr, Nz), np.complex128)
+    for r, (xs, ys, depth) in enumerate(recvs):
+        WT0 = np.zeros(501 * 6)
+        lms = np.zeros(501)
+        R.ref_vbt_weights_single(rmin, lx * dx - dx / 2 + xs, ly * dy - dy / 2 + ys, dx, dy, WT0, lms)
+        WT0 = WT0.reshape(6, 501).T  # D0, DXX, DXY, DX, DYY, DY
+        WT = np.stack([WT0[:, 1], WT0[:, 2], WT0[:, 4], WT0[:, 3], WT0[:, 5], WT0[:, 0]], 1)
+        W0 = np.abs(WT).max()
+        GE = np.zeros(8 * Nz, np.complex128)
+        GH = np.zeros(7 * Nz, np.complex128)
+        for K in range(501):
+            if np.max(np.abs(WT[K] / W0)) <= 1e-12:
+                continue
+            lm = lms[K]
+            W = WT[K].copy()
+            W[[0, 1, 2]] /= lm
+            W[[3, 4, 5]] *= lm
+            R.ref_rc_integral_u(len(sig), cs, thd, 2 * np.pi * f, Nz, np.ascontiguousarray(z), depth,
+                                lm, W, GE, GH)
+        blk[:8, r] = GE.reshape(Nz, 8).T / np.pi / 4
+        blk[8:, r] = GH.reshape(Nz, 7).T / np.pi / 4
+    return blk.reshape(-1)
+
+
 def csig(sig, f):
     w = 2 * np.pi * f
     return np.array([1e-12 - 1j * w * EPS0] + [s - 1j * w * EPS0 for s in sig])
@@ -140,6 +186,47 @@ def main():
         A = np.zeros(2 * Nl * Nl, np.complex128)
         R.ref_calc_apq(Nl, csd, thd, 2 * np.pi * f, 0.0, Arr, p, q, eta, A)
         out["apq_%d" % bi] = np.concatenate([Arr, p, q, eta, A])
+    # receiver continuation: single-integral weights, PrepareRecvs and the
+    # receiver kernel Calc_Integral_U
+    wr = []
+    for x, y, hx, hy in RC_WEIGHT_CASES:
+        WT = np.zeros(501 * 6)
+        lms = np.zeros(501)
+        R.ref_vbt_weights_single(np.hypot(hx, hy) / 2, x, y, hx, hy, WT, lms)
+        wr.append(WT.reshape(6, 501).T)
+    out["rc_weight_cases"] = RC_WEIGHT_CASES
+    out["rc_weights"] = np.array(wr)
+    out["rc_depths"] = RC_DEPTHS
+    for bi, (sig, th, f) in enumerate(BKGS):
+        cs = csig(sig, f)
+        csd = np.ascontiguousarray(cs.view(np.float64))
+        thd = np.array(th + [0.0])
+        Nl = len(sig)
+        for zi, z in enumerate(ZS):
+            Nz = len(z) - 1
+            nd = len(RC_DEPTHS)
+            zr = np.zeros(nd)
+            lay = np.zeros(nd, np.int32)
+            cell = np.zeros(nd, np.int32)
+            R.ref_prepare_recvs(Nl, csd, thd, 2 * np.pi * f, Nz, np.ascontiguousarray(z), nd, RC_DEPTHS,
+                                zr, lay, cell)
+            out["rc_recv_%d_%d" % (bi, zi)] = np.stack([zr, lay, cell])
+            U = np.zeros((nd, len(LAMBDAS), 15, Nz), np.complex128)
+            for di, depth in enumerate(RC_DEPTHS):
+                for li, lm in enumerate(LAMBDAS):
+                    GE = np.zeros(8 * Nz, np.complex128)
+                    GH = np.zeros(7 * Nz, np.complex128)
+                    R.ref_rc_integral_u(Nl, csd, thd, 2 * np.pi * f, Nz, np.ascontiguousarray(z), depth,
+                                        lm, W6, GE, GH)
+                    U[di, li, :8] = GE.reshape(Nz, 8).T
+                    U[di, li, 8:] = GH.reshape(Nz, 7).T
+            out["rcU_%d_%d" % (bi, zi)] = U
+    out["rc_offsets"] = RC_OFFSETS
+    for sig, th, f, z, tag in TENSOR_MODELS:
+        out["rc_recvs_%s" % tag] = RC_RECVS[tag]
+        for lx, ly in RC_OFFSETS:
+            out["rctensor_%s_%d_%d" % (tag, lx, ly)] = ref_rc_tensor_element(R, sig, th, f, z, lx, ly,
+                                                                             RC_RECVS[tag])
     # GFGMRES on dense complex systems
     rng = np.random.Generator(np.random.PCG64(7))
     cases = [(60, 10, 5, 50, 1e-10), (120, 8, 4, 100, 1e-9), (50, 30, 30, 100, 1e-8),
