@@ -86,7 +86,10 @@ __global__ __launch_bounds__(NT) void k_fwd_x(const cx_t<R>* __restrict__ v,
 }
 
 // ---------------------------------------------------------------- K2 ----
-template <typename R, int L>
+// FULL: every point of the 2Nx x 2Ny spectrum (the receiver continuation);
+// otherwise the x- and y-Nyquist lines, which the IE operator never uses,
+// are skipped.
+template <typename R, int L, bool FULL = false>
 __global__ __launch_bounds__(NT) void k_fwd_y(const cx_t<R>* __restrict__ X,
                                               cx_t<R>* __restrict__ S,
                                               const cx_t<R>* __restrict__ twg, Geo g) {
@@ -96,7 +99,7 @@ __global__ __launch_bounds__(NT) void k_fwd_y(const cx_t<R>* __restrict__ X,
   __shared__ T2 buf[NR * LS];
   __shared__ T2 tw[L];
   const int kx = blockIdx.x, chunk = blockIdx.y, Ny = g.Ny;
-  if (kx == g.Nx) return;  // x-Nyquist column is never used
+  if (!FULL && kx == g.Nx) return;  // x-Nyquist column is never used
   load_tw<T2, L>(tw, twg);
   for (int i = threadIdx.x; i < NR * Ny; i += NT) {
     const int yy = i / NR, s = i - yy * NR, cz = chunk * NR + s;
@@ -107,12 +110,12 @@ __global__ __launch_bounds__(NT) void k_fwd_y(const cx_t<R>* __restrict__ X,
   lds_fft<T2, L, NR, LS, NT, -1>(buf, tw);
   for (int i = threadIdx.x; i < NR * L; i += NT) {
     const int ky = i / NR, s = i - ky * NR, cz = chunk * NR + s;
-    if (ky != Ny && cz < g.Nz3) S[((int64_t)kx * L + ky) * g.Nz3 + cz] = buf[s * LS + ky];
+    if ((FULL || ky != Ny) && cz < g.Nz3) S[((int64_t)kx * L + ky) * g.Nz3 + cz] = buf[s * LS + ky];
   }
 }
 
 // ---------------------------------------------------------------- K4 ----
-template <typename R, int L>
+template <typename R, int L, bool FULL = false>
 __global__ __launch_bounds__(NT) void k_inv_y(const cx_t<R>* __restrict__ S,
                                               cx_t<R>* __restrict__ X,
                                               const cx_t<R>* __restrict__ twg, Geo g) {
@@ -122,7 +125,7 @@ __global__ __launch_bounds__(NT) void k_inv_y(const cx_t<R>* __restrict__ S,
   __shared__ T2 buf[NR * LS];
   __shared__ T2 tw[L];
   const int kx = blockIdx.x, chunk = blockIdx.y, Ny = g.Ny;
-  if (kx == g.Nx) {  // x-Nyquist: zero (InitialTransposeRepack, Distributed_FFT.F90:723-724)
+  if (!FULL && kx == g.Nx) {  // x-Nyquist: zero (InitialTransposeRepack, Distributed_FFT.F90:723-724)
     for (int i = threadIdx.x; i < NR * Ny; i += NT) {
       const int yy = i / NR, s = i - yy * NR, cz = chunk * NR + s;
       if (cz < g.Nz3) X[xidx(g, cz, yy, kx)] = mk<T2>(0, 0);
@@ -946,6 +949,220 @@ void op_apply2(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin0, const cx_t<R>
 }
 template void op_apply2<double>(Operator&, OpBuffers<double>&, const zd*, const zd*, zd*, zd*, hipStream_t);
 template void op_apply2<float>(Operator&, OpBuffers<float>&, const zf*, const zf*, zf*, zf*, hipStream_t);
+
+// ============================================================ receivers ==
+// Continuation of the solution to receivers (ReCalculation, APPLY_RC_E_OP,
+// APPLY_RC_H_OP, Continuation_Function_Module.F90:198-481): j = dsig * E is
+// transformed once (the reference transforms it once for E and again for
+// H); one contraction gives the six field components of every receiver; one
+// inverse transform of 6 Nr rows brings them back.
+//   K1 k_fwd_x         : j = dsig * E, zero pad, x-FFT (the IE kernel)
+//   K2 k_fwd_y<FULL>   : y-FFT, the whole 2Nx x 2Ny spectrum
+//   K3 k_rc_contract   : per (point, receiver) the 17 block products
+//                        out_(c', r) = sum_z G_e(z, r) j_c(z) (six outputs)
+//   K4 k_inv_y<FULL>   : inverse y-FFT of the 6 Nr output rows, y < Ny
+//   K5 k_rc_inv_x      : inverse x-FFT, x < Nx, 1/(4 Nx Ny), (Nr,3,Nx,Ny) store
+
+// One P-lane segment per (point, receiver) pair, lane = source layer z;
+// partial products summed over the segment with lane exchanges.  All lanes
+// stay active through the exchanges (loads and stores are predicated).
+template <int P>
+__global__ __launch_bounds__(NT) void k_rc_contract(const zd* __restrict__ S, const zd* __restrict__ G,
+                                                    zd* __restrict__ So, int64_t npairs, int Nr, int Nz) {
+  const int lane = threadIdx.x & (P - 1);
+  const int64_t q = (int64_t)blockIdx.x * (NT / P) + threadIdx.x / P;
+  const bool pair_ok = q < npairs;
+  const bool ok = pair_ok && lane < Nz;
+  const int64_t pt = pair_ok ? q / Nr : 0;
+  const int r = pair_ok ? (int)(q - pt * Nr) : 0;
+  const zd zero = mkz(0, 0);
+  const zd* s = S + pt * 3 * Nz + lane;
+  const zd jx = ok ? s[0] : zero, jy = ok ? s[Nz] : zero, jz = ok ? s[2 * Nz] : zero;
+  const zd* g = G + (pt * 15 * Nr + r) * Nz + lane;  // entry e at g[e * Nr * Nz]
+  const int64_t es = (int64_t)Nr * Nz;
+#define GE(e) (ok ? g[(e) * es] : zero)
+  const zd rexx = GE(0), rexy = GE(1), rexz = GE(2), reyy = GE(3), reyz = GE(4), rezx = GE(5),
+           rezy = GE(6), rezz = GE(7), rhxx = GE(8), rhxy = GE(9), rhxz = GE(10), rhyx = GE(11),
+           rhyz = GE(12), rhzx = GE(13), rhzy = GE(14);
+#undef GE
+  // APPLY_RC_E_OP (:420-434) and APPLY_RC_H_OP (:461-473); RHYY = -RHXX (:468)
+  zd o[6];
+  o[0] = rexx * jx + rexy * jy + rexz * jz;
+  o[1] = rexy * jx + reyy * jy + reyz * jz;
+  o[2] = rezx * jx + rezy * jy + rezz * jz;
+  o[3] = rhxx * jx + rhxy * jy + rhxz * jz;
+  o[4] = rhyx * jx - rhxx * jy + rhyz * jz;
+  o[5] = rhzx * jx + rhzy * jy;
+#pragma unroll
+  for (int c = 0; c < 6; c++)
+#pragma unroll
+    for (int off = P / 2; off > 0; off >>= 1) {
+      o[c].x += __shfl_xor(o[c].x, off);
+      o[c].y += __shfl_xor(o[c].y, off);
+    }
+  if (pair_ok && lane == 0) {
+    zd* d = So + pt * 6 * Nr + r;
+#pragma unroll
+    for (int c = 0; c < 6; c++) d[c * Nr] = o[c];
+  }
+}
+
+// Inverse x-FFT of the output rows (row = component*Nr + receiver), keep
+// x < Nx, scale (RC_OP_FFTW_BWD, :344-348) and store Ea, Ha (Nr,3,Nx,Ny).
+template <int L>
+__global__ __launch_bounds__(NT) void k_rc_inv_x(const zd* __restrict__ X, zd* __restrict__ Ea,
+                                                 zd* __restrict__ Ha, const zd* __restrict__ twg, Geo g,
+                                                 int Nr, double scale) {
+  constexpr int NR = nr_for<double>(L);
+  constexpr int LS = L + 1;
+  __shared__ zd buf[NR * LS];
+  __shared__ zd tw[L];
+  const int y = blockIdx.x, chunk = blockIdx.y, Nx = g.Nx;
+  load_tw<zd, L>(tw, twg);
+  const zd* src = X + (((int64_t)chunk * g.Ny + y) * L) * NR;
+  for (int i = threadIdx.x; i < NR * L; i += NT) {
+    const int kx = i / NR, s = i - kx * NR;
+    buf[s * LS + kx] = src[i];
+  }
+  __syncthreads();
+  lds_fft<zd, L, NR, LS, NT, +1>(buf, tw);
+  for (int i = threadIdx.x; i < NR * Nx; i += NT) {
+    const int s = i / Nx, x = i - s * Nx, row = chunk * NR + s;
+    if (row < g.Nz3) {
+      const int c = row / Nr, r = row - c * Nr;
+      const int64_t o = r + (int64_t)Nr * ((c % 3) + 3 * (x + (int64_t)Nx * y));
+      (c < 3 ? Ea : Ha)[o] = buf[s * LS + x] * scale;
+    }
+  }
+}
+
+// LoadEint (:255): dsig = sigma_a - sigma_b
+__global__ void k_dsig(const zd* __restrict__ csiga, const zd* __restrict__ csigb, zd* __restrict__ dsig,
+                       int64_t ncxy, int Nz) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncxy * Nz) return;
+  const int z = (int)(i / ncxy);
+  dsig[i] = csiga[i] - csigb[z];
+}
+
+void rc_init(RcOperator& rc, Operator& op, int Nr, const double* xs, const double* ys, const double* depth) {
+  G2G_CHECK(Nr >= 1 && Nr <= 64, -3, "Nr must be in [1,64]");
+  for (int r = 0; r < Nr; r++) {
+    G2G_CHECK(std::isfinite(xs[r]) && std::isfinite(ys[r]) && std::isfinite(depth[r]) && depth[r] >= 0, -3,
+              "receiver shifts must be finite and depths non-negative");
+    // the weights need a nonzero distance to the source cell corner
+    // (VBTransformWeightsAllSingleInt takes log(rho))
+    const double fx = xs[r] / op.dx - 0.5, fy = ys[r] / op.dy - 0.5;
+    const bool cx = std::fabs(fx - std::nearbyint(fx)) < 1e-12 && std::fabs(std::nearbyint(fx)) < op.Nx;
+    const bool cy = std::fabs(fy - std::nearbyint(fy)) < 1e-12 && std::fabs(std::nearbyint(fy)) < op.Ny;
+    G2G_CHECK(!(cx && cy), -3, "receiver on a cell corner: the filter weights are singular there");
+  }
+  rc.op = &op;
+  rc.Nr = Nr;
+  rc.xs.assign(xs, xs + Nr);
+  rc.ys.assign(ys, ys + Nr);
+  rc.depth.assign(depth, depth + Nr);
+  const int Nx2 = 2 * op.Nx, Ny2 = 2 * op.Ny;
+  rc.NP = (int64_t)Nx2 * Ny2;
+  rc.BS = 15 * (int64_t)Nr * op.Nz;
+  rc.NR = nr_rt<double>(Nx2);
+  rc.nchunk_f = (3 * op.Nz + rc.NR - 1) / rc.NR;
+  rc.nchunk_i = (6 * Nr + rc.NR - 1) / rc.NR;
+  auto A = [&](zd** p, size_t n) {
+    G2G_HIP(hipMalloc((void**)p, n * sizeof(zd)));
+    rc.bytes += n * sizeof(zd);
+  };
+  A(&rc.G, (size_t)rc.NP * rc.BS);
+  A(&rc.dsig, (size_t)op.ncell());
+  A(&rc.Xf, (size_t)rc.nchunk_f * rc.NR * op.Ny * Nx2);
+  A(&rc.S, (size_t)rc.NP * 3 * op.Nz);
+  A(&rc.So, (size_t)rc.NP * 6 * Nr);
+  A(&rc.Xi, (size_t)rc.nchunk_i * rc.NR * op.Ny * Nx2);
+}
+
+void rc_fini(RcOperator& rc) {
+  for (zd** p : {&rc.G, &rc.dsig, &rc.Xf, &rc.S, &rc.So, &rc.Xi, &rc.tabs}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+  if (rc.wts) (void)hipFree(rc.wts);
+  rc.wts = nullptr;
+}
+
+// FinalizeRCOperator (:168-196): the conductivity of the current frequency
+void rc_set_sigma(RcOperator& rc, const zd* d_csiga) {
+  Operator& op = *rc.op;
+  G2G_CHECK(op.have_bkg, -4, "background not set");
+  zd* d_csigb;
+  G2G_HIP(hipMalloc(&d_csigb, sizeof(zd) * op.Nz));
+  G2G_HIP(hipMemcpy(d_csigb, op.csigb.data(), sizeof(zd) * op.Nz, hipMemcpyHostToDevice));
+  const int64_t n = op.ncell();
+  hipLaunchKernelGGL(k_dsig, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, op.stream, d_csiga, d_csigb,
+                     rc.dsig, (int64_t)op.Nx * op.Ny, op.Nz);
+  G2G_HIP(hipGetLastError());
+  G2G_HIP(hipStreamSynchronize(op.stream));
+  (void)hipFree(d_csigb);
+  rc.have_sigma = true;
+  rc.sigma_version = op.bkg_version;
+}
+
+void rc_apply(RcOperator& rc, const zd* E, zd* Ea, zd* Ha, hipStream_t s) {
+  Operator& op = *rc.op;
+  G2G_CHECK(rc.have_tensor && rc.spectral && rc.tensor_version == op.bkg_version, -4,
+            "receiver tensor not built and transformed for the current frequency");
+  G2G_CHECK(rc.have_sigma && rc.sigma_version == op.bkg_version, -4,
+            "receiver conductivity not set for the current frequency");
+  const int Nx2 = 2 * op.Nx, Ny2 = 2 * op.Ny, Nr = rc.Nr;
+  Geo gf = make_geo(op, rc.NR);
+  Geo gi = gf;
+  gi.Nz3 = 6 * Nr;
+  const int nry = nr_rt<double>(Ny2);
+  const zd* twx = op.d64.twx;
+  const zd* twy = op.d64.twy;
+  G2G_HIP(hipEventRecord(op.ev[4], s));
+  pow2_dispatch(Nx2, [&](auto LC) {
+    constexpr int LL = decltype(LC)::value;
+    hipLaunchKernelGGL((k_fwd_x<double, LL>), dim3(op.Ny, rc.nchunk_f), dim3(NT), 0, s, E, rc.dsig, rc.Xf,
+                       twx, gf);
+  });
+  pow2_dispatch(Ny2, [&](auto LC) {
+    constexpr int LL = decltype(LC)::value;
+    hipLaunchKernelGGL((k_fwd_y<double, LL, true>), dim3(Nx2, (3 * op.Nz + nry - 1) / nry), dim3(NT), 0, s,
+                       rc.Xf, rc.S, twy, gf);
+  });
+  {
+    int P = 1;
+    while (P < op.Nz) P <<= 1;
+    const int64_t npairs = rc.NP * Nr;
+    const unsigned blocks = (unsigned)((npairs + NT / P - 1) / (NT / P));
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(blocks), dim3(NT), 0, s, rc.S, rc.G, rc.So, npairs, Nr, op.Nz);
+    };
+    switch (P) {
+      case 1: go(k_rc_contract<1>); break;
+      case 2: go(k_rc_contract<2>); break;
+      case 4: go(k_rc_contract<4>); break;
+      case 8: go(k_rc_contract<8>); break;
+      case 16: go(k_rc_contract<16>); break;
+      case 32: go(k_rc_contract<32>); break;
+      default: go(k_rc_contract<64>); break;
+    }
+  }
+  G2G_HIP(hipEventRecord(op.ev[5], s));
+  pow2_dispatch(Ny2, [&](auto LC) {
+    constexpr int LL = decltype(LC)::value;
+    hipLaunchKernelGGL((k_inv_y<double, LL, true>), dim3(Nx2, (6 * Nr + nry - 1) / nry), dim3(NT), 0, s,
+                       rc.So, rc.Xi, twy, gi);
+  });
+  pow2_dispatch(Nx2, [&](auto LC) {
+    constexpr int LL = decltype(LC)::value;
+    hipLaunchKernelGGL((k_rc_inv_x<LL>), dim3(op.Ny, rc.nchunk_i), dim3(NT), 0, s, rc.Xi, Ea, Ha, twx, gi, Nr,
+                       1.0 / (4.0 * op.Nx * op.Ny));
+  });
+  G2G_HIP(hipEventRecord(op.ev[6], s));
+  G2G_HIP(hipGetLastError());
+  rc.applies++;
+}
 
 // ------------------------------------------------------ conductivity ---
 template <typename R>

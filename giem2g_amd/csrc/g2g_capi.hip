@@ -67,18 +67,28 @@ Handle* H(void* p) {
   return static_cast<Handle*>(p);
 }
 
-// GetLayerSingle (data_types_module.F90:75-103)
-int get_layer(const Operator& op, double z) {
-  if (std::fabs(z) < 1e-15) return 0;
-  if (z < 0) return -1;
-  if (op.Nl == 1) return 1;
-  std::vector<double> depth(op.Nl + 1, 0.0);
-  for (int i = 1; i <= op.Nl - 1; i++) depth[i] = depth[i - 1] + op.thick[i];
-  if (z > depth[op.Nl - 1]) return op.Nl;
-  int l = 1;
-  while (z > depth[l]) l++;
-  return l;
+struct RcHandle {
+  RcOperator rc;
+};
+
+RcOperator& RC(void* p) {
+  G2G_CHECK(p != nullptr, -2, "null receiver-operator handle");
+  return static_cast<RcHandle*>(p)->rc;
 }
+
+// a host array through a temporary device buffer
+struct DevCopy {
+  zd* d = nullptr;
+  DevCopy(const double* h, size_t n) {
+    G2G_HIP(hipMalloc(&d, sizeof(zd) * n));
+    if (h) G2G_HIP(hipMemcpy(d, h, sizeof(zd) * n, hipMemcpyHostToDevice));
+  }
+  ~DevCopy() {
+    if (d) (void)hipFree(d);
+  }
+};
+
+int get_layer(const Operator& op, double z) { return op_layer(op, z); }
 
 void op_init(Operator& op, int Nx, int Ny, int Nz, double dx, double dy, const double* z) {
   G2G_CHECK(Nx >= 8 && Ny >= 8 && is_pow2(Nx) && is_pow2(Ny) && Nx <= 512 && Ny <= 512, -3,
@@ -133,6 +143,7 @@ void set_background(Operator& op, int Nl, const double* csigb, const double* thi
     op.sqsigb[iz] = std::sqrt(op.csigma[l].x);  // PreparePreconditioner (calc_ie_tensor_module.F90:303)
   }
   op.have_bkg = true;
+  op.bkg_version++;
   op.have_tensor = false;
   op.have_g32 = false;
 }
@@ -192,11 +203,18 @@ void apply_host(Operator& op, const double* in, double* out) {
   (void)hipFree(b);
 }
 
-// Calc_Apq with the A matrix at lambda = 0 (apq_module.F90:230-266) and
-// PlaneWaveIntegral (sources_module.F90:67-101).
-void plane_wave(const Operator& op, int pol, double* Eh) {
+// The normal (1-D) field of a unit plane wave: Calc_Apq at lambda = 0 with
+// the TE column A(l,1,1) of the layer-coupling matrix (apq_module.F90:
+// 230-266) and the surface normalisation t1 (sources_module.F90:36-42, 77-83).
+struct PlaneWaveBasis {
+  int Nl = 0;
+  std::vector<zc> eta, q0, A1;
+  std::vector<double> depth;
+  zc t1;
+};
+
+PlaneWaveBasis plane_wave_basis(const Operator& op) {
   G2G_CHECK(op.have_bkg, -4, "background not set");
-  G2G_CHECK(pol == 0 || pol == 1, -3, "pol must be 0 (EX) or 1 (EY)");
   const int Nl = op.Nl;
   std::vector<zc> cs(Nl + 1), k2(Nl + 1), eta(Nl + 1), el(Nl + 1);
   std::vector<double> depth(Nl + 1, 0.0);
@@ -261,6 +279,24 @@ void plane_wave(const Operator& op, int pol, double* Eh) {
   }
   zc t1 = Nl > 1 ? 1.0 + q[1][0] * std::exp(-depth[1] * eta[1] * 2.0) : zc(1.0);
   t1 = A1[1] * t1;
+  PlaneWaveBasis b;
+  b.Nl = Nl;
+  b.eta = eta;
+  b.A1 = A1;
+  b.depth = depth;
+  b.t1 = t1;
+  b.q0.resize(Nl + 1);
+  for (int i = 1; i <= Nl; i++) b.q0[i] = q[i][0];
+  return b;
+}
+
+// PlaneWaveIntegral (sources_module.F90:67-101): cell averages over the anomaly.
+void plane_wave(const Operator& op, int pol, double* Eh) {
+  G2G_CHECK(pol == 0 || pol == 1, -3, "pol must be 0 (EX) or 1 (EY)");
+  const PlaneWaveBasis b = plane_wave_basis(op);
+  const int Nl = b.Nl;
+  const std::vector<zc>& eta = b.eta;
+  const std::vector<double>& depth = b.depth;
   const int64_t nxy = (int64_t)op.Nx * op.Ny, nc = op.ncell();
   std::memset(Eh, 0, sizeof(double) * 2 * 3 * nc);
   const int slot = op.comp_slot[pol];
@@ -270,16 +306,39 @@ void plane_wave(const Operator& op, int pol, double* Eh) {
     const zc e1 = std::exp(-eta[l] * op.z[Iz - 1]), e2 = std::exp(-eta[l] * op.z[Iz]);
     zc e3 = 0, e4 = 0;
     if (l < Nl) {
-      e3 = q[l][0] * std::exp(-eta[l] * (2.0 * depth[l] - op.z[Iz - 1]));
-      e4 = q[l][0] * std::exp(-eta[l] * (2.0 * depth[l] - op.z[Iz]));
+      e3 = b.q0[l] * std::exp(-eta[l] * (2.0 * depth[l] - op.z[Iz - 1]));
+      e4 = b.q0[l] * std::exp(-eta[l] * (2.0 * depth[l] - op.z[Iz]));
     }
-    zc G = (e1 - e2 + e4 - e3) * A1[l] / t1;
+    zc G = (e1 - e2 + e4 - e3) * b.A1[l] / b.t1;
     G = G / dz / eta[l];
     double* dst = Eh + 2 * (slot * nc + (int64_t)(Iz - 1) * nxy);
     for (int64_t i = 0; i < nxy; i++) {
       dst[2 * i] = G.real();
       dst[2 * i + 1] = G.imag();
     }
+  }
+}
+
+// PlaneWave (sources_module.F90:26-66): the field at receiver depths, F0
+// complex (Nr,6) with components Ex,Ey,Ez,Hx,Hy,Hz.
+void plane_wave_at(const Operator& op, int pol, int Nr, const double* zr, double* F0) {
+  G2G_CHECK(pol == 0 || pol == 1, -3, "pol must be 0 (EX) or 1 (EY)");
+  const PlaneWaveBasis b = plane_wave_basis(op);
+  const zc iwm(0.0, MU0 * op.omega);
+  std::memset(F0, 0, sizeof(double) * 2 * 6 * Nr);
+  for (int r = 0; r < Nr; r++) {
+    const int l = std::max(get_layer(op, zr[r]), 1);
+    const zc t2 = b.A1[l] / b.t1;
+    const zc e1 = std::exp(-b.eta[l] * zr[r]);
+    const zc e2 = l < b.Nl ? b.q0[l] * std::exp(-b.eta[l] * (2.0 * b.depth[l] - zr[r])) : zc(0.0);
+    const zc G0 = t2 * (e1 + e2);
+    const zc G1 = -t2 * (e1 - e2) * b.eta[l] / iwm;
+    const int ce = pol == 0 ? 0 : 1, ch = pol == 0 ? 4 : 3;
+    const zc gh = pol == 0 ? G1 : -G1;
+    F0[2 * (r + Nr * ce)] = G0.real();
+    F0[2 * (r + Nr * ce) + 1] = G0.imag();
+    F0[2 * (r + Nr * ch)] = gh.real();
+    F0[2 * (r + Nr * ch) + 1] = gh.imag();
   }
 }
 
@@ -840,6 +899,125 @@ void giem2g_print_stats(void* p) {
 void giem2g_delete_ie_operator(void* p) {
   const int rc = giem2g_op_destroy(p);
   if (rc) log_msg(std::string("giem2g_delete_ie_operator: ") + g_last_error);
+}
+
+// ---- receiver continuation (Continuation_Function_Module.F90) ----
+int giem2g_rc_create(void* ie_op, int Nr, const double* xshift, const double* yshift, const double* depth,
+                     void** out) {
+  return guard([&] {
+    G2G_CHECK(out != nullptr && xshift && yshift && depth, -2, "null argument");
+    auto h = std::make_unique<RcHandle>();
+    try {
+      rc_init(h->rc, H(ie_op)->op, Nr, xshift, yshift, depth);
+    } catch (...) {
+      rc_fini(h->rc);
+      throw;
+    }
+    *out = h.release();
+  });
+}
+
+int giem2g_rc_destroy(void* p) {
+  return guard([&] {
+    if (!p) return;
+    RcHandle* h = static_cast<RcHandle*>(p);
+    rc_fini(h->rc);
+    delete h;
+  });
+}
+
+int giem2g_rc_calc_kernel(void* p) {
+  return guard([&] { rc_build_tensor(RC(p)); });
+}
+
+int giem2g_rc_calc_fft_of_kernel(void* p) {
+  return guard([&] { rc_tensor_fft(RC(p)); });
+}
+
+int giem2g_rc_get_tensor(void* p, double* G) {
+  return guard([&] {
+    RcOperator& rc = RC(p);
+    G2G_CHECK(rc.have_tensor, -4, "receiver tensor not built");
+    G2G_HIP(hipMemcpy(G, rc.G, sizeof(zd) * rc.NP * rc.BS, hipMemcpyDeviceToHost));
+  });
+}
+
+int giem2g_rc_put_tensor(void* p, const double* G, int spectral) {
+  return guard([&] {
+    RcOperator& rc = RC(p);
+    G2G_HIP(hipMemcpy(rc.G, G, sizeof(zd) * rc.NP * rc.BS, hipMemcpyHostToDevice));
+    rc_prepare_receivers(rc);
+    rc.have_tensor = true;
+    rc.spectral = spectral != 0;
+    rc.tensor_version = rc.op->bkg_version;
+  });
+}
+
+int giem2g_rc_receivers(void* p, double* zr, int* layer, int* cell) {
+  return guard([&] {
+    RcOperator& rc = RC(p);
+    rc_prepare_receivers(rc);
+    for (int r = 0; r < rc.Nr; r++) {
+      if (zr) zr[r] = rc.zr[r];
+      if (layer) layer[r] = rc.layer[r];
+      if (cell) cell[r] = rc.cell[r];
+    }
+  });
+}
+
+int giem2g_rc_set_anomaly_conductivity(void* p, const double* csiga) {
+  return guard([&] {
+    RcOperator& rc = RC(p);
+    DevCopy d(csiga, (size_t)rc.op->ncell());
+    rc_set_sigma(rc, d.d);
+  });
+}
+
+int giem2g_rc_set_anomaly_conductivity_dev(void* p, const void* csiga) {
+  return guard([&] { rc_set_sigma(RC(p), static_cast<const zd*>(csiga)); });
+}
+
+int giem2g_rc_recalculate_dev(void* p, const void* Eint, void* Ea, void* Ha) {
+  return guard([&] {
+    RcOperator& rc = RC(p);
+    rc_apply(rc, static_cast<const zd*>(Eint), static_cast<zd*>(Ea), static_cast<zd*>(Ha), rc.op->stream);
+  });
+}
+
+int giem2g_rc_recalculate(void* p, const double* Eint, double* Ea, double* Ha) {
+  return guard([&] {
+    RcOperator& rc = RC(p);
+    const Operator& op = *rc.op;
+    const size_t no = (size_t)rc.Nr * 3 * op.Nx * op.Ny;
+    DevCopy e(Eint, (size_t)op.nvec()), a(nullptr, no), h(nullptr, no);
+    rc_apply(rc, e.d, a.d, h.d, op.stream);
+    G2G_HIP(hipMemcpy(Ea, a.d, sizeof(zd) * no, hipMemcpyDeviceToHost));
+    G2G_HIP(hipMemcpy(Ha, h.d, sizeof(zd) * no, hipMemcpyDeviceToHost));
+  });
+}
+
+int giem2g_rc_plane_wave(void* p, int pol, double* F0) {
+  return guard([&] {
+    RcOperator& rc = RC(p);
+    rc_prepare_receivers(rc);
+    plane_wave_at(*rc.op, pol, rc.Nr, rc.zr.data(), F0);
+  });
+}
+
+int giem2g_rc_stats(void* p, double* o) {
+  return guard([&] {
+    RcOperator& rc = RC(p);
+    Operator& op = *rc.op;
+    float ms[3] = {0, 0, 0};
+    if (rc.applies > 0) {
+      G2G_HIP(hipEventSynchronize(op.ev[6]));
+      G2G_HIP(hipEventElapsedTime(&ms[0], op.ev[4], op.ev[6]));
+      G2G_HIP(hipEventElapsedTime(&ms[1], op.ev[4], op.ev[5]));
+      G2G_HIP(hipEventElapsedTime(&ms[2], op.ev[5], op.ev[6]));
+    }
+    const double v[7] = {rc.t_tensor, rc.t_fft, ms[0], ms[1], ms[2], (double)rc.applies, (double)rc.bytes};
+    std::memcpy(o, v, sizeof v);
+  });
 }
 
 }  // extern "C"

@@ -11,6 +11,7 @@
 //   - FFT work buffers: X (x-transformed rows, tiled by NR row chunks) and
 //     S (full 2Nx x 2Ny spectrum, [kx][ky][c*Nz+z]).
 #pragma once
+#include <cmath>
 #include <vector>
 
 #include "g2g_common.hpp"
@@ -53,6 +54,7 @@ struct Operator {
   std::vector<zd> csigb;   // per anomaly layer
   std::vector<double> sqsigb;
   bool have_bkg = false, have_tensor = false, have_sigma = false, have_g32 = false;
+  int bkg_version = 0;  // incremented by every background (frequency) change
   int64_t BS = 0;
   OpBuffers<double> d64;
   OpBuffers<float> d32;
@@ -71,6 +73,47 @@ struct Operator {
   int64_t nvec() const { return 3 * ncell(); }
 };
 
+// Continuation of the IE solution to receivers (reference: RC_OPERATOR,
+// Continuation_Function_Module.F90:46-80, single rank).  It shares the IE
+// operator's grid, background and stream.  Tensor layout: per point
+// p = ix*2Ny + iy of the full 2Nx x 2Ny grid (real-space offsets or spectrum)
+// a block of BS = 15*Nr*Nz complex, entry e (0..7 = REXX, REXY, REXZ, REYY,
+// REYZ, REZX, REZY, REZZ; 8..14 = RHXX, RHXY, RHXZ, RHYX, RHYZ, RHZX, RHZY),
+// receiver r, source layer z at (e*Nr + r)*Nz + z.
+struct RcOperator {
+  Operator* op = nullptr;
+  int Nr = 0;
+  std::vector<double> xs, ys, depth;  // as given (lateral shifts from the cell centres)
+  std::vector<double> zr;             // after PrepareRecvs
+  std::vector<int> layer, cell;
+  int64_t BS = 0, NP = 0;
+  zd* G = nullptr;        // tensor, NP*BS
+  double* wts = nullptr;  // masked, lambda-scaled weights [r][6][501][NP] (frequency independent)
+  zd* tabs = nullptr;     // per-tap kernel tables of one receiver [501][13][Nz]
+  zd* dsig = nullptr;     // sigma_a - sigma_b per cell
+  zd *Xf = nullptr, *S = nullptr, *So = nullptr, *Xi = nullptr;  // FFT work buffers
+  int NR = 0, nchunk_f = 0, nchunk_i = 0;
+  bool have_wts = false, have_tensor = false, spectral = false, have_sigma = false;
+  int tensor_version = -1, sigma_version = -1;  // background version each was made for
+  double t_tensor = 0, t_fft = 0, t_apply = 0;
+  int64_t applies = 0;
+  size_t bytes = 0;
+};
+
+// GetLayerSingle (data_types_module.F90:75-103): background layer of depth z
+// (0 at the surface, -1 in the air) for the operator's current background.
+inline int op_layer(const Operator& op, double z) {
+  if (std::fabs(z) < 1e-15) return 0;
+  if (z < 0) return -1;
+  if (op.Nl == 1) return 1;
+  std::vector<double> depth(op.Nl + 1, 0.0);
+  for (int i = 1; i <= op.Nl - 1; i++) depth[i] = depth[i - 1] + op.thick[i];
+  if (z > depth[op.Nl - 1]) return op.Nl;
+  int l = 1;
+  while (z > depth[l]) l++;
+  return l;
+}
+
 // apply kernels (g2g_apply.hip)
 template <typename R> void op_alloc_buffers(Operator& op, OpBuffers<R>& b, bool with_tensor);
 template <typename R> void op_free_buffers(OpBuffers<R>& b);
@@ -88,5 +131,14 @@ void op_make_g32(Operator& op);
 void op_build_tensor(Operator& op);
 void op_tensor_fft(Operator& op, zd* G);
 void op_free_tensor_scratch(Operator& op);
+// receiver continuation
+void rc_init(RcOperator& rc, Operator& op, int Nr, const double* xs, const double* ys,
+             const double* depth);                                     // g2g_apply.hip
+void rc_fini(RcOperator& rc);                                          // g2g_apply.hip
+void rc_prepare_receivers(RcOperator& rc);                             // g2g_tensor.hip
+void rc_build_tensor(RcOperator& rc);                                  // g2g_tensor.hip
+void rc_tensor_fft(RcOperator& rc);                                    // g2g_tensor.hip
+void rc_set_sigma(RcOperator& rc, const zd* d_csiga);                  // g2g_apply.hip
+void rc_apply(RcOperator& rc, const zd* E, zd* Ea, zd* Ha, hipStream_t s);  // g2g_apply.hip
 
 }  // namespace g2g

@@ -267,6 +267,90 @@ __device__ inline zd cexp_d(zd z) {
 __device__ inline zd operator/(zd a, zd b) { return cdiv(a, b); }
 __device__ inline zd rdiv(double a, zd b) { return cdiv(mkz(a, 0.0), b); }
 
+// Calc_Apq without the A matrix (apq_module.F90:230-249): eta, el, p, q and
+// Arr of the layered background for one lambda; arrays indexed by layer
+// 1..Nl.  cs, k2: csigma and k^2 (0..Nl); td: (thickness, depth) per layer.
+__device__ inline void layer_apq(double lm, int Nl, const zd* sh_cs, const zd* sh_k2, const zd* sh_td,
+                                 zd* eta, zd* el, zd (*p)[2], zd (*q)[2], zd (*Arr)[2]) {
+  for (int i = 1; i <= Nl; i++) eta[i] = csqrt_d(cz2(lm * lm - sh_k2[i].x, -sh_k2[i].y));
+  for (int i = 1; i <= Nl - 1; i++) el[i] = cexp_d(eta[i] * (-2.0 * sh_td[i].x));
+  el[Nl] = cz2(0, 0);
+  // Calc_p
+  {
+    zd eta0 = csqrt_d(cz2(lm * lm - sh_k2[0].x, -sh_k2[0].y));
+    zd ceta = eta0 / eta[1];
+    p[1][0] = cz2(1.0 - ceta.x, -ceta.y) / cadd(ceta, 1.0);
+    zd cg = sh_cs[0] / sh_cs[1];
+    p[1][1] = (cg - ceta) / (cg + ceta);
+    for (int i = 2; i <= Nl; i++) {
+      zd a0 = eta[i - 1] / eta[i];
+      zd a1 = a0 * sh_cs[i] / sh_cs[i - 1];
+      for (int k = 0; k < 2; k++) {
+        zd w = el[i - 1] * p[i - 1][k];
+        w = cadd(w, -1.0) / cadd(w, 1.0);
+        w = w * (k == 0 ? a0 : a1);
+        p[i][k] = cadd(w, 1.0) / cz2(1.0 - w.x, -w.y);
+      }
+    }
+  }
+  // Calc_q
+  {
+    q[Nl][0] = q[Nl][1] = cz2(0, 0);
+    if (Nl > 1) {
+      zd a0 = eta[Nl] / eta[Nl - 1];
+      zd a1 = a0 * sh_cs[Nl - 1] / sh_cs[Nl];
+      q[Nl - 1][0] = cz2(1.0 - a0.x, -a0.y) / cadd(a0, 1.0);
+      q[Nl - 1][1] = cz2(1.0 - a1.x, -a1.y) / cadd(a1, 1.0);
+      for (int i = Nl - 2; i >= 1; i--) {
+        zd b0 = eta[i + 1] / eta[i];
+        zd b1 = b0 * sh_cs[i] / sh_cs[i + 1];
+        for (int k = 0; k < 2; k++) {
+          zd w = el[i + 1] * q[i + 1][k];
+          w = cadd(w, -1.0) / cadd(w, 1.0);
+          w = w * (k == 0 ? b0 : b1);
+          q[i][k] = cadd(w, 1.0) / cz2(1.0 - w.x, -w.y);
+        }
+      }
+    }
+  }
+  for (int i = 1; i <= Nl; i++)
+    for (int k = 0; k < 2; k++)
+      Arr[i][k] = rdiv(1.0, eta[i]) / cz2(1.0 - (p[i][k] * q[i][k] * el[i]).x,
+                                           -(p[i][k] * q[i][k] * el[i]).y);
+}
+
+// expz per cell, Calc_pexpz_All and Calc_qexpz_All (apq_module.F90:112-177);
+// pez, qez at [2*I + k].
+__device__ inline void layer_pqexpz(int Nz, int Nl, const int* lnumg, const double* zg,
+                                    const double* dzg, const zd* sh_td, const zd* eta,
+                                    const zd (*p)[2], const zd (*q)[2], zd* expz, zd* pez, zd* qez) {
+  for (int I = 1; I <= Nz; I++) expz[I] = cexp_d(eta[lnumg[I]] * (-dzg[I]));
+  // Calc_pexpz_All
+  int l = lnumg[0];
+  zd e;
+  if (l > 1) e = cexp_d(eta[l] * (-2.0 * (zg[0] - sh_td[l - 1].y)));
+  else if (l == 1) e = cexp_d(eta[1] * (-2.0 * zg[0]));
+  else e = cz2(1, 0);
+  for (int I = 1; I <= Nz; I++) {
+    if (l != lnumg[I]) e = cz2(1, 0);
+    l = lnumg[I];
+    pez[2 * I] = e * p[l][0];
+    pez[2 * I + 1] = e * p[l][1];
+    e = e * expz[I] * expz[I];
+  }
+  // Calc_qexpz_All
+  l = lnumg[Nz];
+  if (l != Nl) e = cexp_d(eta[l] * (-2.0 * (sh_td[l].y - zg[Nz])));
+  else e = cz2(0, 0);
+  for (int I = Nz; I >= 1; I--) {
+    if (l != lnumg[I]) e = cz2(1, 0);
+    l = lnumg[I];
+    qez[2 * I] = e * q[l][0];
+    qez[2 * I + 1] = e * q[l][1];
+    e = e * expz[I] * expz[I];
+  }
+}
+
 // tables per tap K: a[P], b[P], c[P] packed upper (i<=j at i + j(j+1)/2), d[Nz*Nz] column-major
 __global__ __launch_bounds__(64) void k_lambda(zd* __restrict__ tabs,
                                                const double* __restrict__ lms,
@@ -301,78 +385,9 @@ __global__ __launch_bounds__(64) void k_lambda(zd* __restrict__ tabs,
   zd* tc = tb + P;
   zd* td = tc + P;
   if (threadIdx.x == 0) {
-    // Calc_Apq (apq_module.F90:230-249)
     zd el[65];
-    for (int i = 1; i <= Nl; i++) eta[i] = csqrt_d(cz2(lm * lm - sh_k2[i].x, -sh_k2[i].y));
-    for (int i = 1; i <= Nl - 1; i++) el[i] = cexp_d(eta[i] * (-2.0 * sh_td[i].x));
-    el[Nl] = cz2(0, 0);
-    // Calc_p
-    {
-      zd eta0 = csqrt_d(cz2(lm * lm - sh_k2[0].x, -sh_k2[0].y));
-      zd ceta = eta0 / eta[1];
-      p[1][0] = cz2(1.0 - ceta.x, -ceta.y) / cadd(ceta, 1.0);
-      zd cg = sh_cs[0] / sh_cs[1];
-      p[1][1] = (cg - ceta) / (cg + ceta);
-      for (int i = 2; i <= Nl; i++) {
-        zd a0 = eta[i - 1] / eta[i];
-        zd a1 = a0 * sh_cs[i] / sh_cs[i - 1];
-        for (int k = 0; k < 2; k++) {
-          zd w = el[i - 1] * p[i - 1][k];
-          w = cadd(w, -1.0) / cadd(w, 1.0);
-          w = w * (k == 0 ? a0 : a1);
-          p[i][k] = cadd(w, 1.0) / cz2(1.0 - w.x, -w.y);
-        }
-      }
-    }
-    // Calc_q
-    {
-      q[Nl][0] = q[Nl][1] = cz2(0, 0);
-      if (Nl > 1) {
-        zd a0 = eta[Nl] / eta[Nl - 1];
-        zd a1 = a0 * sh_cs[Nl - 1] / sh_cs[Nl];
-        q[Nl - 1][0] = cz2(1.0 - a0.x, -a0.y) / cadd(a0, 1.0);
-        q[Nl - 1][1] = cz2(1.0 - a1.x, -a1.y) / cadd(a1, 1.0);
-        for (int i = Nl - 2; i >= 1; i--) {
-          zd b0 = eta[i + 1] / eta[i];
-          zd b1 = b0 * sh_cs[i] / sh_cs[i + 1];
-          for (int k = 0; k < 2; k++) {
-            zd w = el[i + 1] * q[i + 1][k];
-            w = cadd(w, -1.0) / cadd(w, 1.0);
-            w = w * (k == 0 ? b0 : b1);
-            q[i][k] = cadd(w, 1.0) / cz2(1.0 - w.x, -w.y);
-          }
-        }
-      }
-    }
-    for (int i = 1; i <= Nl; i++)
-      for (int k = 0; k < 2; k++)
-        Arr[i][k] = rdiv(1.0, eta[i]) / cz2(1.0 - (p[i][k] * q[i][k] * el[i]).x,
-                                             -(p[i][k] * q[i][k] * el[i]).y);
-    for (int I = 1; I <= Nz; I++) expz[I] = cexp_d(eta[lnumg[I]] * (-dzg[I]));
-    // Calc_pexpz_All
-    int l = lnumg[0];
-    zd e;
-    if (l > 1) e = cexp_d(eta[l] * (-2.0 * (zg[0] - sh_td[l - 1].y)));
-    else if (l == 1) e = cexp_d(eta[1] * (-2.0 * zg[0]));
-    else e = cz2(1, 0);
-    for (int I = 1; I <= Nz; I++) {
-      if (l != lnumg[I]) e = cz2(1, 0);
-      l = lnumg[I];
-      pez[2 * I] = e * p[l][0];
-      pez[2 * I + 1] = e * p[l][1];
-      e = e * expz[I] * expz[I];
-    }
-    // Calc_qexpz_All
-    l = lnumg[Nz];
-    if (l != Nl) e = cexp_d(eta[l] * (-2.0 * (sh_td[l].y - zg[Nz])));
-    else e = cz2(0, 0);
-    for (int I = Nz; I >= 1; I--) {
-      if (l != lnumg[I]) e = cz2(1, 0);
-      l = lnumg[I];
-      qez[2 * I] = e * q[l][0];
-      qez[2 * I + 1] = e * q[l][1];
-      e = e * expz[I] * expz[I];
-    }
+    layer_apq(lm, Nl, sh_cs, sh_k2, sh_td, eta, el, p, q, Arr);
+    layer_pqexpz(Nz, Nl, lnumg, zg, dzg, sh_td, eta, p, q, expz, pez, qez);
   }
   __syncthreads();
   for (int I = 1 + threadIdx.x; I <= Nz; I += blockDim.x) {
@@ -721,6 +736,486 @@ void op_build_tensor(Operator& op) {
   (void)hipFree(d_dz);
   (void)hipFree(d_z);
   (void)hipFree(d_ln);
+}
+
+// ====================================================================== //
+// Receiver continuation tensor (CalcRecalculationGreenTensor,              //
+// calc_rc_tensor_module.F90:37-230, with VBTransformWeightsAllSingleInt,   //
+// VolumeBesselTransforms.F90:198-229, and Calc_Integral_U,                 //
+// rc_kernel_hankel_module.F90:56-254).  The same factorisation as above:   //
+// per-(offset, receiver) weights, per-(tap, receiver) weight-free tables   //
+// of the 13 kernel functions of (lambda, z), and their weighted tap sums   //
+// on k_wgemm.  Then the full 2-D FFT of every tensor column               //
+// (CalcFFTofRCTensor, Continuation_Function_Module.F90:349-383).          //
+// ====================================================================== //
+
+// outfunc_int2_all for one knot (VolumeBesselTransforms.F90:342-371, 407-434, 553-603)
+__device__ inline void outfunc2(double explt, double cphi, double sphi, double alpha, double beta,
+                                double* R) {
+  double f[2][3][3];  // [x/y][D0,D1,D2][1..3]
+#pragma unroll
+  for (int J = 0; J < 2; J++) {
+    const double c = J == 0 ? cphi : sphi, h = J == 0 ? alpha : beta;
+    double x[2], x2[2], ex[2], er[2], t[2];
+    x[0] = explt * c * 0.5;
+    x[1] = x[0] + h * explt * 0.5;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      x2[k] = x[k] * x[k];
+      ex[k] = exp(-x2[k]);
+      er[k] = erf(x[k]);
+    }
+    // SINGLE_INT_D0
+    const double a0 = er[1] * sqrt(PI_D) - er[0] * sqrt(PI_D);
+    for (int k = 0; k < 2; k++) t[k] = x[k] * ex[k];
+    double b = t[1] - t[0];
+    for (int k = 0; k < 2; k++) t[k] = t[k] * x2[k];
+    double cc = t[1] - t[0];
+    f[J][0][0] = a0 * 0.5;
+    f[J][0][1] = a0 - 2.0 * b;
+    f[J][0][2] = 6.0 * a0 - 12.0 * b - 8.0 * cc;
+    // SINGLE_INT_D1
+    for (int k = 0; k < 2; k++) t[k] = ex[k];
+    f[J][1][0] = (t[1] - t[0]) / 2.0;
+    for (int k = 0; k < 2; k++) t[k] = t[k] * x2[k];
+    f[J][1][1] = (t[1] - t[0]) * 2.0;
+    for (int k = 0; k < 2; k++) t[k] = t[k] * x2[k];
+    f[J][1][2] = (t[1] - t[0]) * 8.0;
+    // SINGLE_INT_D2
+    for (int k = 0; k < 2; k++) t[k] = x[k] * ex[k];
+    const double a2 = t[1] - t[0];
+    for (int k = 0; k < 2; k++) t[k] = t[k] * x2[k];
+    b = t[1] - t[0];
+    for (int k = 0; k < 2; k++) t[k] = t[k] * x2[k];
+    cc = t[1] - t[0];
+    f[J][2][0] = -a2 * 0.5;
+    f[J][2][1] = 2.0 * (a2 - b);
+    f[J][2][2] = 16.0 * b - 8.0 * cc;
+  }
+#define CMB(a, b) (f[0][a][2] * f[1][b][0] + f[0][a][0] * f[1][b][2] + 2.0 * f[0][a][1] * f[1][b][1])
+  R[0] = CMB(0, 0) / explt;
+  R[1] = CMB(2, 0) * explt;
+  R[2] = CMB(1, 1) * explt;
+  R[3] = CMB(1, 0);
+  R[4] = CMB(0, 2) * explt;
+  R[5] = CMB(0, 1);
+#undef CMB
+}
+
+struct RcWParams {
+  int Nx, Ny;
+  double dx, dy, xs, ys, log_rmin;
+};
+
+// One block per offset point p = ix*2Ny + iy of one receiver.  Output
+// wts[c][K][p] in IE order, masked (1e-12 of the largest weight of the
+// offset, calc_rc_tensor_module.F90:202-212) and lambda-scaled (:214-219);
+// zero on the ix = Nx and iy = Ny offsets.
+__global__ __launch_bounds__(256) void k_rc_weights(double* __restrict__ wts,
+                                                    const zd* __restrict__ f1,
+                                                    const zd* __restrict__ twg,
+                                                    const double* __restrict__ lms, RcWParams P) {
+  constexpr int NRW = 3, LS = VBN + 1;
+  __shared__ zd buf[NRW * LS];
+  __shared__ zd tw[VBN];
+  __shared__ double wt[6][NTAPS];
+  __shared__ double red[8];
+  const int Ny2 = 2 * P.Ny;
+  const int64_t NP = (int64_t)4 * P.Nx * P.Ny;
+  const int p = blockIdx.x, ix = p / Ny2, iy = p - ix * Ny2;
+  if (ix == P.Nx || iy == P.Ny) {
+    for (int i = threadIdx.x; i < 6 * NTAPS; i += 256) wts[(int64_t)i * NP + p] = 0.0;
+    return;
+  }
+  const int lx = ix < P.Nx ? ix : ix - 2 * P.Nx, ly = iy < P.Ny ? iy : iy - Ny2;
+  const double x = lx * P.dx - P.dx / 2.0 + P.xs, y = ly * P.dy - P.dy / 2.0 + P.ys;
+  const double rho = sqrt(x * x + y * y);
+  const double alpha = P.dx / rho, beta = P.dy / rho, phi = atan2(y, x);
+  const double cphi = cos(phi), sphi = sin(phi);
+  const double s = log(rho) - P.log_rmin;
+  for (int i = threadIdx.x; i < VBN; i += 256) tw[i] = twg[i];
+  for (int half = 0; half < 2; half++) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < VBN; i += 256) {
+      const int i0 = i & ~1;
+      double y2 = (double)(i0 - 512) * VB_YSTEP + VB_P + s;
+      if (i & 1) y2 = y2 + VB_YSTEP;
+      const double explt = exp(y2);
+      double R[6];
+      outfunc2(explt, cphi, sphi, alpha, beta, R);
+      const double sg = (i & 1) ? -1.0 : 1.0;
+      for (int c = 0; c < NRW; c++) buf[c * LS + i] = mkz(sg * R[half * 3 + c], 0.0);
+    }
+    __syncthreads();
+    lds_fft<zd, VBN, NRW, LS, 256, +1>(buf, tw);
+    for (int i = threadIdx.x; i < NRW * VBN; i += 256) {
+      const int c = i / VBN, J = i - c * VBN;
+      const zd h = cdiv(buf[c * LS + J], f1[J]);
+      buf[c * LS + J] = (J & 1) ? -h : h;
+    }
+    __syncthreads();
+    lds_fft<zd, VBN, NRW, LS, 256, -1>(buf, tw);
+    for (int i = threadIdx.x; i < NRW * NTAPS; i += 256) {
+      const int c = i / NTAPS, K = i - c * NTAPS, J = K + NFIRST;
+      double h = buf[c * LS + J].x / (double)VBN;
+      if (J & 1) h = -h;
+      const int col = half * 3 + c;  // reference output order D0,DXX,DXY,DX,DYY,DY
+      // scalings of VolumeBesselTransforms.F90:222-228
+      if (col == 0) h = h * rho;
+      else if (col == 1 || col == 2 || col == 4) h = h / rho;
+      const int ie = col == 0 ? IE_D0 : col == 1 ? IE_DXX : col == 2 ? IE_DXY
+                   : col == 3 ? IE_DX : col == 4 ? IE_DYY : IE_DY;
+      wt[ie][K] = h;
+    }
+  }
+  __syncthreads();
+  double m = 0;
+  for (int i = threadIdx.x; i < 6 * NTAPS; i += 256) m = fmax(m, fabs(wt[i / NTAPS][i % NTAPS]));
+  const double W0 = block_max(m, red);
+  for (int K = threadIdx.x; K < NTAPS; K += 256) {
+    double Wm = 0;
+    for (int c = 0; c < 6; c++) Wm = fmax(Wm, fabs(wt[c][K] / W0));
+    const bool on = Wm > 1e-12;
+    const double lm = lms[K];
+    double o[6];
+    o[IE_DXX] = wt[IE_DXX][K] / lm;
+    o[IE_DYY] = wt[IE_DYY][K] / lm;
+    o[IE_DXY] = wt[IE_DXY][K] / lm;
+    o[IE_DX] = wt[IE_DX][K] * lm;
+    o[IE_DY] = wt[IE_DY][K] * lm;
+    o[IE_D0] = wt[IE_D0][K] * lm;
+    for (int c = 0; c < 6; c++) wts[((int64_t)c * NTAPS + K) * NP + p] = on ? o[c] : 0.0;
+  }
+}
+
+// kernel functions of one receiver per tap, table [K][f][z]
+enum { RF_G11 = 0, RF_G3DZ_G11, RF_G2DZ, RF_G2DZ0, RF_G2G2DZZ, RF_G3, RF_G1DZ, RF_G2, RF_G1,
+       RF_NG1DZ, RF_NG3, RF_NG2, RF_NG1, RF_COUNT };
+
+struct RcLParams {
+  int Nl, Nz, rl, cell;
+  double zr;
+  zd iwm;
+  const zd* bk;  // [csigma 0..Nl | k2 0..Nl | (thick, depth) 0..Nl]
+};
+
+// One block per tap K: Calc_Apq with the receiver row of A, Calc_Recv_Part,
+// Calc_Ftb and Calc_G_not_match for every source layer (one lane each).
+__global__ __launch_bounds__(64) void k_rc_lambda(zd* __restrict__ tabs,
+                                                  const double* __restrict__ lms,
+                                                  const double* __restrict__ dzg,
+                                                  const double* __restrict__ zg,
+                                                  const int* __restrict__ lnumg, RcLParams L) {
+  const int Nz = L.Nz, Nl = L.Nl, rl = L.rl, N = L.cell;
+  __shared__ zd eta[65], p[65][2], q[65][2], Arr[65][2], wlay[65][2], Arl[65][2];
+  __shared__ zd sh_cs[65], sh_k2[65], sh_td[65];
+  __shared__ zd expz[65], pez[130], qez[130];
+  for (int i = threadIdx.x; i <= Nl; i += blockDim.x) {
+    sh_cs[i] = L.bk[i];
+    sh_k2[i] = L.bk[Nl + 1 + i];
+    sh_td[i] = L.bk[2 * (Nl + 1) + i];
+  }
+  __syncthreads();
+  const int K = blockIdx.x;
+  const double lm = lms[K];
+  if (threadIdx.x == 0) {
+    zd el[65];
+    layer_apq(lm, Nl, sh_cs, sh_k2, sh_td, eta, el, p, q, Arr);
+    // the receiver row A(rl, l) of the layer-coupling matrix (apq_module.F90:250-264)
+    for (int i = 1; i <= Nl - 1; i++) {
+      zd tmp = sh_k2[i] - sh_k2[i + 1];
+      tmp = tmp * sh_td[i].y;
+      tmp = tmp / (eta[i + 1] + eta[i]);
+      const zd et = cexp_d(tmp);
+      for (int k = 0; k < 2; k++) wlay[i][k] = cadd(p[i + 1][k], 1.0) / cadd(p[i][k] * el[i], 1.0) * et;
+    }
+    for (int l = 1; l <= Nl; l++) {
+      const int hi = l > rl ? l : rl, lo = l > rl ? rl : l;
+      zd a0 = Arr[hi][0], a1 = Arr[hi][1];
+      for (int j = hi - 1; j >= lo; j--) {
+        a0 = a0 * wlay[j][0];
+        a1 = a1 * wlay[j][1];
+      }
+      Arl[l][0] = a0;
+      Arl[l][1] = rl > l ? a1 * sh_cs[rl] / sh_cs[l] : a1;
+    }
+    layer_pqexpz(Nz, Nl, lnumg, zg, dzg, sh_td, eta, p, q, expz, pez, qez);
+  }
+  __syncthreads();
+  zd* T = tabs + (int64_t)K * RF_COUNT * Nz;
+  const zd eta_r = eta[rl], eta_zr = eta_r * L.zr, sr = sh_cs[rl];
+  for (int I = 1 + threadIdx.x; I <= Nz; I += blockDim.x) {
+    const int l = lnumg[I];
+    const zd ez = expz[I], et = eta[l];
+    const bool above = I <= N;  // cell above the receiver
+    // Calc_Recv_Part (rc_kernel_hankel_module.F90:187-230)
+    zd w1[2], w2[2];
+    if (!above) {
+      const zd eta_dr = rl > 1 ? eta_r * (sh_td[rl - 1].y * 2.0) : cz2(0, 0);
+      const zd e1 = cexp_d(et * (-zg[I - 1]) + eta_zr);
+      const zd e2 = cexp_d(et * (-zg[I - 1]) - eta_zr + eta_dr);
+      w1[0] = e2 * p[rl][0] + e1;
+      w1[1] = (e1 - e2 * p[rl][0]) * eta_r;
+      w2[0] = e2 * p[rl][1] + e1;
+      w2[1] = (e1 - e2 * p[rl][1]) * eta_r;
+    } else {
+      const zd e1 = cexp_d(et * zg[I] - eta_zr);
+      // below the last layer the reflected term vanishes (eta_dr = HUGE, :219)
+      const zd e2 = rl < Nl ? cexp_d(et * zg[I] + eta_zr - eta_r * (sh_td[rl].y * 2.0)) : cz2(0, 0);
+      w1[0] = e2 * q[rl][0] + e1;
+      w1[1] = (e2 * q[rl][0] - e1) * eta_r;
+      w2[0] = e2 * q[rl][1] + e1;
+      w2[1] = (e2 * q[rl][1] - e1) * eta_r;
+    }
+    // Calc_Ftb (:232-254): p above the receiver with sign -1, q below with +1
+    const zd* pq = above ? pez : qez;
+    const double sgn = above ? -1.0 : 1.0;
+    const zd a = cz2(1.0 - ez.x, -ez.y), a1 = a / et;
+    const zd F0 = a1 * cadd(pq[2 * I] * ez, 1.0);
+    const zd F1 = a1 * cadd(pq[2 * I + 1] * ez, 1.0);
+    const zd pe = pq[2 * I + 1] * ez;
+    const zd F2 = a * (cz2(1.0 - pe.x, -pe.y) * sgn);
+    // Calc_G_not_match (:152-186)
+    const zd A1 = Arl[l][0], A2 = Arl[l][1];
+    const zd G1 = A1 * F0 * w1[0];
+    const zd G3DZ = (A2 * F2 * w2[1]) / sr;
+    const zd G2DZ = (A2 * F1 * w2[1]) / sr;
+    const zd G2DZ0 = (A2 * F2 * w2[0]) / sr;
+    const zd G2G2DZZ = (A2 * F1 * w2[0]) * (lm * lm) / sr;
+    const zd G3 = A2 * F2 * w2[0] - A1 * F0 * w1[1];
+    const zd G1DZ = A1 * F0 * w1[1];
+    const zd G2 = A2 * F1 * w2[0];
+    const zd G11 = G1 * L.iwm;
+    const int z = I - 1;
+    T[RF_G11 * Nz + z] = G11;
+    T[RF_G3DZ_G11 * Nz + z] = G3DZ + G11;
+    T[RF_G2DZ * Nz + z] = G2DZ;
+    T[RF_G2DZ0 * Nz + z] = G2DZ0;
+    T[RF_G2G2DZZ * Nz + z] = G2G2DZZ;
+    T[RF_G3 * Nz + z] = G3;
+    T[RF_G1DZ * Nz + z] = G1DZ;
+    T[RF_G2 * Nz + z] = G2;
+    T[RF_G1 * Nz + z] = G1;
+    T[RF_NG1DZ * Nz + z] = -G1DZ;
+    T[RF_NG3 * Nz + z] = -G3;
+    T[RF_NG2 * Nz + z] = -G2;
+    T[RF_NG1 * Nz + z] = -G1;
+  }
+}
+
+// In-place forward FFT of every tensor column along y for a fixed ix
+// (AXIS 0) or along x for a fixed iy (AXIS 1), points p = ix*Ny2 + iy.
+template <int L, int AXIS>
+__global__ __launch_bounds__(256) void k_rc_tfft(zd* __restrict__ G, const zd* __restrict__ twg, int Ny2,
+                                                 int64_t BS) {
+  constexpr int NE = ne_for<L>(), LS = L + 1;
+  __shared__ zd buf[NE * LS];
+  __shared__ zd tw[L];
+  const int fixed = blockIdx.x;
+  const int64_t e0 = (int64_t)blockIdx.y * NE;
+  for (int i = threadIdx.x; i < L; i += 256) tw[i] = twg[i];
+  auto gidx = [&](int l) -> int64_t {
+    return AXIS == 0 ? ((int64_t)fixed * Ny2 + l) * BS : ((int64_t)l * Ny2 + fixed) * BS;
+  };
+  for (int i = threadIdx.x; i < NE * L; i += 256) {
+    const int l = i / NE, s = i - l * NE;
+    buf[s * LS + l] = (e0 + s < BS) ? G[gidx(l) + e0 + s] : mkz(0, 0);
+  }
+  __syncthreads();
+  lds_fft<zd, L, NE, LS, 256, -1>(buf, tw);
+  for (int i = threadIdx.x; i < NE * L; i += 256) {
+    const int k = i / NE, s = i - k * NE;
+    if (e0 + s < BS) G[gidx(k) + e0 + s] = buf[s * LS + k];
+  }
+}
+
+// PrepareRecvs (data_types_module.F90:124-152) against the current background
+void rc_prepare_receivers(RcOperator& rc) {
+  const Operator& op = *rc.op;
+  G2G_CHECK(op.have_bkg, -4, "background not set");
+  const int Nz = op.Nz;
+  rc.zr.resize(rc.Nr);
+  rc.layer.resize(rc.Nr);
+  rc.cell.resize(rc.Nr);
+  for (int r = 0; r < rc.Nr; r++) {
+    const double z = rc.depth[r];
+    rc.layer[r] = std::max(op_layer(op, z), 1);
+    rc.zr[r] = z;
+    if (z <= op.z[0]) {
+      rc.cell[r] = 0;
+    } else if (z >= op.z[Nz]) {
+      rc.cell[r] = Nz + 1;
+    } else {
+      for (int Iz = 1; Iz <= Nz; Iz++) {
+        if (std::fabs(z / op.z[Iz] - 1.0) < 1e-7) {
+          rc.cell[r] = Iz;
+          break;
+        } else if (z > op.z[Iz - 1] && z < op.z[Iz]) {
+          rc.zr[r] = op.z[Iz - 1];
+          rc.cell[r] = Iz - 1;
+          break;
+        }
+      }
+    }
+  }
+}
+
+void rc_build_tensor(RcOperator& rc) {
+  Operator& op = *rc.op;
+  G2G_CHECK(op.have_bkg, -4, "background not set");
+  G2G_CHECK(op.Nl <= 64 && op.Nz <= 64, -3, "tensor build supports Nl <= 64 and Nz <= 64");
+  rc_prepare_receivers(rc);
+  const int Nz = op.Nz, Nr = rc.Nr;
+  const int64_t NP = rc.NP, BS = rc.BS, WS = (int64_t)6 * NTAPS * NP;  // weights per receiver
+  const double rmin = std::sqrt(op.dx * op.dx + op.dy * op.dy) / 2.0;  // :67
+  std::vector<double> lms(NTAPS);
+  for (int j = 0; j < NTAPS; j++) lms[j] = std::exp(VB_XI + (double)(j + NFIRST - 512) * VB_YSTEP) / rmin;
+  double* d_lms;
+  G2G_HIP(hipMalloc(&d_lms, sizeof(double) * NTAPS));
+  G2G_HIP(hipMemcpy(d_lms, lms.data(), sizeof(double) * NTAPS, hipMemcpyHostToDevice));
+  hipEvent_t e0, e1;
+  G2G_HIP(hipEventCreate(&e0));
+  G2G_HIP(hipEventCreate(&e1));
+  G2G_HIP(hipEventRecord(e0, op.stream));
+  // The weights do not depend on frequency: all receivers' weights are kept
+  // when they fit in 8 GB, else one receiver's are recomputed per build.
+  const bool cache = (double)Nr * WS * sizeof(double) <= 8.0 * (1 << 30);
+  zd *tw = nullptr, *f1 = nullptr;
+  if (!rc.have_wts) {
+    make_tw64(VBN, -1, &tw);
+    G2G_HIP(hipMalloc(&f1, sizeof(zd) * VBN));
+    hipLaunchKernelGGL(k_f1, dim3(1), dim3(256), 0, op.stream, f1, tw);
+    if (!rc.wts) {
+      G2G_HIP(hipMalloc(&rc.wts, sizeof(double) * WS * (cache ? Nr : 1)));
+      rc.bytes += sizeof(double) * WS * (cache ? Nr : 1);
+    }
+  }
+  auto weights = [&](int r, double* dst) {
+    RcWParams wp{op.Nx, op.Ny, op.dx, op.dy, rc.xs[r], rc.ys[r], std::log(rmin)};
+    hipLaunchKernelGGL(k_rc_weights, dim3((unsigned)NP), dim3(256), 0, op.stream, dst, f1, tw, d_lms, wp);
+  };
+  if (!rc.have_wts && cache)
+    for (int r = 0; r < Nr; r++) weights(r, rc.wts + r * WS);
+  // background arrays, as for the IE tensor
+  const double mw = 4.0 * PI_D * 1e-7 * op.omega;
+  std::vector<zd> bk(3 * (op.Nl + 1), mkz(0, 0));
+  for (int i = 0; i <= op.Nl; i++) {
+    bk[i] = op.csigma[i];
+    bk[op.Nl + 1 + i] = mkz(-op.csigma[i].y * mw, op.csigma[i].x * mw);
+  }
+  {
+    double depth = 0;
+    for (int i = 1; i <= op.Nl - 1; i++) {
+      depth += op.thick[i];
+      bk[2 * (op.Nl + 1) + i] = mkz(op.thick[i], depth);
+    }
+  }
+  zd* d_bk;
+  G2G_HIP(hipMalloc(&d_bk, sizeof(zd) * bk.size()));
+  G2G_HIP(hipMemcpy(d_bk, bk.data(), sizeof(zd) * bk.size(), hipMemcpyHostToDevice));
+  if (!rc.tabs) {
+    G2G_HIP(hipMalloc(&rc.tabs, sizeof(zd) * NTAPS * RF_COUNT * Nz));
+    rc.bytes += sizeof(zd) * NTAPS * RF_COUNT * Nz;
+  }
+  double *d_dz, *d_z;
+  int* d_ln;
+  std::vector<double> dz1(Nz + 1, 0.0);
+  for (int i = 1; i <= Nz; i++) dz1[i] = op.dz[i - 1];
+  G2G_HIP(hipMalloc(&d_dz, sizeof(double) * (Nz + 1)));
+  G2G_HIP(hipMalloc(&d_z, sizeof(double) * (Nz + 1)));
+  G2G_HIP(hipMalloc(&d_ln, sizeof(int) * (Nz + 1)));
+  G2G_HIP(hipMemcpy(d_dz, dz1.data(), sizeof(double) * (Nz + 1), hipMemcpyHostToDevice));
+  G2G_HIP(hipMemcpy(d_z, op.z.data(), sizeof(double) * (Nz + 1), hipMemcpyHostToDevice));
+  G2G_HIP(hipMemcpy(d_ln, op.lnum.data(), sizeof(int) * (Nz + 1), hipMemcpyHostToDevice));
+  const double scale = 1.0 / PI_D / 4.0;  // :223-224
+  const int64_t ldb = RF_COUNT * Nz;
+  for (int r = 0; r < Nr; r++) {
+    double* W = rc.wts + (cache ? r * WS : 0);
+    if (!cache) weights(r, W);
+    RcLParams lp{op.Nl, Nz, rc.layer[r], rc.cell[r], rc.zr[r], mkz(0.0, mw), d_bk};
+    hipLaunchKernelGGL(k_rc_lambda, dim3(NTAPS), dim3(64), 0, op.stream, rc.tabs, d_lms, d_dz, d_z, d_ln, lp);
+    auto Wc = [&](int c) { return W + (int64_t)c * NTAPS * NP; };
+    auto Tf = [&](int f) { return (const zd*)rc.tabs + (int64_t)f * Nz; };
+    const GemmSeg none{nullptr, nullptr};
+    // Calc_Integral_U accumulation (rc_kernel_hankel_module.F90:116-145)
+    const struct { int e, nseg; GemmSeg a, b; } jobs[15] = {
+        {0, 2, {Wc(IE_D0), Tf(RF_G11)}, {Wc(IE_DXX), Tf(RF_G3DZ_G11)}},   // REXX
+        {1, 1, {Wc(IE_DXY), Tf(RF_G3DZ_G11)}, none},                      // REXY
+        {2, 1, {Wc(IE_DX), Tf(RF_G2DZ)}, none},                           // REXZ
+        {3, 2, {Wc(IE_D0), Tf(RF_G11)}, {Wc(IE_DYY), Tf(RF_G3DZ_G11)}},   // REYY
+        {4, 1, {Wc(IE_DY), Tf(RF_G2DZ)}, none},                           // REYZ
+        {5, 1, {Wc(IE_DX), Tf(RF_G2DZ0)}, none},                          // REZX
+        {6, 1, {Wc(IE_DY), Tf(RF_G2DZ0)}, none},                          // REZY
+        {7, 1, {Wc(IE_D0), Tf(RF_G2G2DZZ)}, none},                        // REZZ
+        {8, 1, {Wc(IE_DXY), Tf(RF_G3)}, none},                            // RHXX
+        {9, 2, {Wc(IE_DYY), Tf(RF_G3)}, {Wc(IE_D0), Tf(RF_NG1DZ)}},       // RHXY
+        {10, 1, {Wc(IE_DY), Tf(RF_G2)}, none},                            // RHXZ
+        {11, 2, {Wc(IE_D0), Tf(RF_G1DZ)}, {Wc(IE_DXX), Tf(RF_NG3)}},      // RHYX
+        {12, 1, {Wc(IE_DX), Tf(RF_NG2)}, none},                           // RHYZ
+        {13, 1, {Wc(IE_DY), Tf(RF_NG1)}, none},                           // RHZX
+        {14, 1, {Wc(IE_DX), Tf(RF_G1)}, none},                            // RHZY
+    };
+    for (const auto& j : jobs) {
+      dim3 grid((unsigned)((NP + 63) / 64), (unsigned)((Nz + 31) / 32));
+      hipLaunchKernelGGL(k_wgemm, grid, dim3(256), 0, op.stream, rc.G, NP, BS, (int)((j.e * Nr + r) * Nz),
+                         Nz, ldb, j.a, j.b, j.nseg, scale);
+    }
+  }
+  G2G_HIP(hipGetLastError());
+  G2G_HIP(hipEventRecord(e1, op.stream));
+  G2G_HIP(hipStreamSynchronize(op.stream));
+  float ms = 0;
+  G2G_HIP(hipEventElapsedTime(&ms, e0, e1));
+  rc.t_tensor = ms * 1e-3;
+  rc.have_wts = rc.have_wts || cache;
+  rc.have_tensor = true;
+  rc.spectral = false;
+  rc.tensor_version = op.bkg_version;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (tw) (void)hipFree(tw);
+  if (f1) (void)hipFree(f1);
+  (void)hipFree(d_lms);
+  (void)hipFree(d_bk);
+  (void)hipFree(d_dz);
+  (void)hipFree(d_z);
+  (void)hipFree(d_ln);
+}
+
+void rc_tensor_fft(RcOperator& rc) {
+  Operator& op = *rc.op;
+  G2G_CHECK(rc.have_tensor && !rc.spectral, -4, "receiver tensor not built (or already transformed)");
+  const int Nx2 = 2 * op.Nx, Ny2 = 2 * op.Ny;
+  zd *twx, *twy;
+  make_tw64(Nx2, -1, &twx);
+  make_tw64(Ny2, -1, &twy);
+  hipEvent_t a, b;
+  G2G_HIP(hipEventCreate(&a));
+  G2G_HIP(hipEventCreate(&b));
+  G2G_HIP(hipEventRecord(a, op.stream));
+  pow2_dispatch(Ny2, [&](auto LC) {
+    constexpr int LL = decltype(LC)::value;
+    constexpr int NE = ne_for<LL>();
+    hipLaunchKernelGGL((k_rc_tfft<LL, 0>), dim3(Nx2, (unsigned)((rc.BS + NE - 1) / NE)), dim3(256), 0,
+                       op.stream, rc.G, twy, Ny2, rc.BS);
+  });
+  pow2_dispatch(Nx2, [&](auto LC) {
+    constexpr int LL = decltype(LC)::value;
+    constexpr int NE = ne_for<LL>();
+    hipLaunchKernelGGL((k_rc_tfft<LL, 1>), dim3(Ny2, (unsigned)((rc.BS + NE - 1) / NE)), dim3(256), 0,
+                       op.stream, rc.G, twx, Ny2, rc.BS);
+  });
+  G2G_HIP(hipGetLastError());
+  G2G_HIP(hipEventRecord(b, op.stream));
+  G2G_HIP(hipStreamSynchronize(op.stream));
+  float ms = 0;
+  G2G_HIP(hipEventElapsedTime(&ms, a, b));
+  rc.t_fft = ms * 1e-3;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipFree(twx);
+  (void)hipFree(twy);
+  rc.spectral = true;
 }
 
 }  // namespace g2g

@@ -28,7 +28,11 @@ EXTENDED_SYMBOLS = [
     "giem2g_unet_forward_dev", "giem2g_unet_backward_dev", "giem2g_unet_train_step",
     "giem2g_op_set_learned", "giem2g_op_learned_features", "giem2g_op_learned_apply",
     "giem2g_op_scale", "giem2g_op_set_contraction", "giem2g_solve_pair_dev", "giem2g_unet_adam",
-    "giem2g_op_apply_adjoint_dev",
+    "giem2g_op_apply_adjoint_dev", "giem2g_rc_create", "giem2g_rc_destroy", "giem2g_rc_calc_kernel",
+    "giem2g_rc_calc_fft_of_kernel", "giem2g_rc_get_tensor", "giem2g_rc_put_tensor",
+    "giem2g_rc_receivers",
+    "giem2g_rc_set_anomaly_conductivity", "giem2g_rc_set_anomaly_conductivity_dev",
+    "giem2g_rc_recalculate", "giem2g_rc_recalculate_dev", "giem2g_rc_plane_wave", "giem2g_rc_stats",
 ]
 
 
@@ -133,6 +137,19 @@ def load():
         "giem2g_op_apply_adjoint_dev": (i, [vp, vp, vp]),
         "giem2g_solve_pair_dev": (i, [vp, vp, vp, vp, vp, C.POINTER(SolverParams), vp, vp,
                                       C.POINTER(SolveInfo), C.POINTER(C.c_longlong)]),
+        "giem2g_rc_create": (i, [vp, i, vp, vp, vp, C.POINTER(vp)]),
+        "giem2g_rc_destroy": (i, [vp]),
+        "giem2g_rc_calc_kernel": (i, [vp]),
+        "giem2g_rc_calc_fft_of_kernel": (i, [vp]),
+        "giem2g_rc_get_tensor": (i, [vp, vp]),
+        "giem2g_rc_put_tensor": (i, [vp, vp, i]),
+        "giem2g_rc_receivers": (i, [vp, vp, vp, vp]),
+        "giem2g_rc_set_anomaly_conductivity": (i, [vp, vp]),
+        "giem2g_rc_set_anomaly_conductivity_dev": (i, [vp, vp]),
+        "giem2g_rc_recalculate": (i, [vp, vp, vp, vp]),
+        "giem2g_rc_recalculate_dev": (i, [vp, vp, vp, vp]),
+        "giem2g_rc_plane_wave": (i, [vp, i, vp]),
+        "giem2g_rc_stats": (i, [vp, vp]),
         "giem2g_calc_data_sizes": (None, [Anomaly, C.POINTER(Data)]),
         "giem2g_prepare_ie_kernel": (None, [Anomaly, Data]),
         "giem2g_calc_ie_kernel": (None, [vp, Background, Anomaly, d]),

@@ -207,3 +207,103 @@ def operator_for(model, order=ORDER_XYZ, build=True):
         op.build_tensor()
     op.set_sigma(model.csiga())
     return op
+
+
+class ReceiverOperator:
+    """Continuation of the solution to receivers (the post-solve field
+    recalculation; Continuation_Function_Module.F90).  Follows the
+    reference's life cycle PrepareContinuationOperator ->
+    CalcRecalculationGreenTensor -> CalcFFTofRCTensor -> FinalizeRCOperator
+    -> ReCalculation per frequency, on the background, grid and stream of an
+    IEOperator (which must outlive it).  Receivers sit at every cell centre
+    shifted by (xshift, yshift) metres, at the given depths; the fields are
+    complex (Nr, 3, Nx, Ny) in Fortran order, returned as flat arrays."""
+
+    def __init__(self, ie_op, xshift, yshift, depth):
+        self.lib = L.load()
+        self.ie = ie_op
+        self.xs = np.ascontiguousarray(xshift, np.float64)
+        self.ys = np.ascontiguousarray(yshift, np.float64)
+        self.depth = np.ascontiguousarray(depth, np.float64)
+        self.Nr = len(self.depth)
+        h = C.c_void_p()
+        L.check(self.lib.giem2g_rc_create(ie_op.h, self.Nr, self.xs.ctypes.data, self.ys.ctypes.data,
+                                          self.depth.ctypes.data, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.check(self.lib.giem2g_rc_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def block_size(self):
+        return 15 * self.Nr * self.ie.Nz
+
+    @property
+    def nfield(self):
+        return self.Nr * 3 * self.ie.Nx * self.ie.Ny
+
+    def calc_kernel(self):
+        L.check(self.lib.giem2g_rc_calc_kernel(self.h))
+
+    def calc_fft_of_kernel(self):
+        L.check(self.lib.giem2g_rc_calc_fft_of_kernel(self.h))
+
+    def build(self):
+        """Tensor and tensor spectrum for the IE operator's current frequency."""
+        self.calc_kernel()
+        self.calc_fft_of_kernel()
+
+    def get_tensor(self):
+        G = np.zeros(4 * self.ie.Nx * self.ie.Ny * self.block_size, np.complex128)
+        L.check(self.lib.giem2g_rc_get_tensor(self.h, G.ctypes.data))
+        return G
+
+    def put_tensor(self, G, spectral=True):
+        G = np.ascontiguousarray(G, np.complex128)
+        L.check(self.lib.giem2g_rc_put_tensor(self.h, G.ctypes.data, int(bool(spectral))))
+
+    def receivers(self):
+        zr = np.zeros(self.Nr)
+        layer = np.zeros(self.Nr, np.int32)
+        cell = np.zeros(self.Nr, np.int32)
+        L.check(self.lib.giem2g_rc_receivers(self.h, zr.ctypes.data, layer.ctypes.data, cell.ctypes.data))
+        return zr, layer, cell
+
+    def set_sigma(self, csiga):
+        if hasattr(csiga, "data_ptr"):
+            L.check(self.lib.giem2g_rc_set_anomaly_conductivity_dev(self.h, _ptr(csiga)))
+        else:
+            cs = np.ascontiguousarray(csiga, np.complex128)
+            L.check(self.lib.giem2g_rc_set_anomaly_conductivity(self.h, cs.ctypes.data))
+
+    def recalculate(self, E):
+        """(Ea, Ha) on the host from the total field E (host array)."""
+        E = np.ascontiguousarray(E, np.complex128)
+        Ea = np.zeros(self.nfield, np.complex128)
+        Ha = np.zeros(self.nfield, np.complex128)
+        L.check(self.lib.giem2g_rc_recalculate(self.h, E.ctypes.data, Ea.ctypes.data, Ha.ctypes.data))
+        return Ea, Ha
+
+    def recalculate_dev(self, E, Ea, Ha):
+        """HBM tensors, ordered on the IE operator's stream."""
+        L.check(self.lib.giem2g_rc_recalculate_dev(self.h, _ptr(E), _ptr(Ea), _ptr(Ha)))
+
+    def plane_wave(self, pol):
+        """Normal field at the prepared receiver depths: complex (Nr, 6)."""
+        F = np.zeros(6 * self.Nr, np.complex128)
+        L.check(self.lib.giem2g_rc_plane_wave(self.h, pol, F.ctypes.data))
+        return F.reshape(6, self.Nr).T.copy()
+
+    def stats(self):
+        o = np.zeros(7)
+        L.check(self.lib.giem2g_rc_stats(self.h, o.ctypes.data))
+        keys = ["t_tensor", "t_tensor_fft", "ms_last", "ms_fwd_contract", "ms_inv", "recalcs", "bytes"]
+        return dict(zip(keys, o))

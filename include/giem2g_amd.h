@@ -276,6 +276,59 @@ int giem2g_op_learned_features(void *ie_op, const void *r_dev, const void *e_dev
 /* z = M(v) on complex128 HBM vectors. */
 int giem2g_op_learned_apply(void *ie_op, const void *v_dev, void *z_dev);
 
+/* ---- continuation of the solution to receivers (the post-solve field
+ * recalculation of the reference CLI, Continuation_Function_Module.F90,
+ * calc_rc_tensor_module.F90, rc_kernel_hankel_module.F90).  A receiver
+ * operator shares the grid, background (frequency) and stream of an IE
+ * operator, which must outlive it.  Receivers sit at every cell centre of
+ * the Nx x Ny grid shifted by (xshift, yshift) metres, at the given depth;
+ * Ea, Ha are complex (Nr,3,Nx,Ny) in Fortran order with components in
+ * (x,y,z) order, the anomalous field (SaveOutputOneFile adds the normal
+ * field of giem2g_rc_plane_wave). ---- */
+/* Replaces PrepareContinuationOperator (Continuation_Function_Module.F90:
+ * 82-167) with LoadRecievers' receivers (mpi_saveload_module.F90:174-213);
+ * 1 <= Nr <= 64.  Receivers on a cell corner are rejected (the reference's
+ * filter weights take log(0) there). */
+int giem2g_rc_create(void *ie_op, int Nr, const double *xshift, const double *yshift,
+                     const double *depth, void **rc_op);
+/* Replaces DeleteRC_OP (Continuation_Function_Module.F90:396-405). */
+int giem2g_rc_destroy(void *rc_op);
+/* Replaces CalcRecalculationGreenTensor (calc_rc_tensor_module.F90:37-162)
+ * for the IE operator's current background: the real-space tensor. */
+int giem2g_rc_calc_kernel(void *rc_op);
+/* Replaces CalcFFTofRCTensor (Continuation_Function_Module.F90:349-383). */
+int giem2g_rc_calc_fft_of_kernel(void *rc_op);
+/* Copy the current receiver tensor (real space after calc_kernel, spectrum
+ * after calc_fft_of_kernel): 4*Nx*Ny points p = ix*2Ny + iy, each 15*Nr*Nz
+ * complex with entry e (REXX, REXY, REXZ, REYY, REYZ, REZX, REZY, REZZ,
+ * RHXX, RHXY, RHXZ, RHYX, RHYZ, RHZX, RHZY), receiver r, source layer z at
+ * (e*Nr + r)*Nz + z. */
+int giem2g_rc_get_tensor(void *rc_op, double *G_host);
+/* Load a tensor of that layout (spectral != 0: a spectrum) for the current
+ * frequency. */
+int giem2g_rc_put_tensor(void *rc_op, const double *G_host, int spectral);
+/* PrepareRecvs (data_types_module.F90:124-152) for the current background:
+ * depth after the move to a cell border, background layer, anomaly cell
+ * index (0 above, Nz+1 below).  Any output may be NULL. */
+int giem2g_rc_receivers(void *rc_op, double *zr, int *layer, int *cell);
+/* Replaces FinalizeRCOperator (Continuation_Function_Module.F90:168-196):
+ * complex (Nx,Ny,Nz) anomaly conductivity of the current frequency
+ * (including -i w eps0, like giem2g_set_anomaly_conductivity). */
+int giem2g_rc_set_anomaly_conductivity(void *rc_op, const double *csiga_host);
+int giem2g_rc_set_anomaly_conductivity_dev(void *rc_op, const void *csiga_dev);
+/* Replaces ReCalculation (Continuation_Function_Module.F90:198-240): the
+ * anomalous E and H at the receivers from the total field E_int (complex
+ * (Nx,Ny,Nz,3) in the IE operator's component order) inside the anomaly.
+ * _dev: HBM pointers, ordered on the IE operator's stream. */
+int giem2g_rc_recalculate(void *rc_op, const double *E_int, double *Ea, double *Ha);
+int giem2g_rc_recalculate_dev(void *rc_op, const void *E_int_dev, void *Ea_dev, void *Ha_dev);
+/* Replaces PlaneWave (sources_module.F90:26-66) at the prepared receiver
+ * depths for pol 0 (EX) or 1 (EY): complex (Nr,6) Ex,Ey,Ez,Hx,Hy,Hz. */
+int giem2g_rc_plane_wave(void *rc_op, int pol, double *F0_host);
+/* [tensor build s, tensor FFT s, last recalculation ms, of which forward
+ * FFT + contraction ms, inverse FFT ms, recalculations, HBM bytes]. */
+int giem2g_rc_stats(void *rc_op, double *out7);
+
 #ifdef __cplusplus
 }
 #endif
