@@ -6,9 +6,10 @@ background 0.01/0.1/1 S/m, f = 10 Hz), complex128 currents re/im ~ N(0,1).
 `value` is matvecs per second over K timed applications with inputs
 resident in HBM.  The JSON line also carries the roofline of the dominant
 kernel (the tensor contraction), the oracle's CPU throughput on the same
-workload, and (unless --no-solve) the time to a 1e-8 FP64 true residual of
-the 256x256x32 random-box model for the plain FP64 solve and the mixed
-complex64-inner / FP64-outer solve.
+workload, the continuation of the field to the reference deck's receivers
+on the same grid, and (unless --no-solve) the time to a 1e-8 FP64 true
+residual of the 256x256x32 random-box model for the plain FP64 solve and
+the mixed complex64-inner / FP64-outer solve.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-cpu]
 """
@@ -174,6 +175,72 @@ def c3_bench(torch, gop, models, nfreq=16):
             "per_frequency": rows}
 
 
+# receivers of the reference's own deck (test/giem2g_input.json:32-35)
+REF_DECK_RECEIVERS = [(0.0, -20000.0, 0.0), (0.0, 20000.0, 0.0)]
+
+
+def recalc_bytes(Nx, Ny, Nz, Nr, s=16):
+    """Algorithmic bytes of one continuation to Nr receivers: forward FFTs
+    as in the IE apply (fwd), the contraction reading the receiver tensor
+    (4NxNy points x 15 Nr Nz) and the current spectrum and writing 6 Nr
+    output rows, and the inverse FFTs of those rows (y pass over the whole
+    spectrum, x pass over the kept y rows) with the (Nr,3,Nx,Ny) fields."""
+    nc = Nx * Ny * Nz
+    npt = 4 * Nx * Ny
+    ns = npt * 3 * Nz
+    no = npt * 6 * Nr
+    return {"fwd": s * (3 * nc + nc + ns), "contraction": s * (npt * 15 * Nr * Nz + ns + no),
+            "inv": s * (no + no // 2 + no // 2 + Nx * Ny * 6 * Nr)}
+
+
+def recalc_bench(torch, gop, op, m, E, cpu=True, steps=20):
+    """Continuation of a C1 field to the reference deck's receivers: tensor
+    build + spectrum per frequency, then recalculations from HBM."""
+    xs, ys, zs = zip(*REF_DECK_RECEIVERS)
+    rc = gop.ReceiverOperator(op, xs, ys, zs)
+    rc.build()
+    rc.set_sigma(m.csiga())
+    Ea = torch.empty(rc.nfield, dtype=torch.complex128, device="cuda")
+    Ha = torch.empty_like(Ea)
+    for _ in range(3):
+        rc.recalculate_dev(E, Ea, Ha)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    parts = np.zeros(4)
+    for _ in range(steps):
+        rc.recalculate_dev(E, Ea, Ha)
+        st = rc.stats()
+        parts += [st["ms_last"], st["ms_fwd"], st["ms_contract"], st["ms_inv"]]
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    parts /= steps
+    b = recalc_bytes(m.Nx, m.Ny, m.Nz, rc.Nr)
+    st = rc.stats()
+    row = {"receivers": REF_DECK_RECEIVERS, "tensor_build_s": st["t_tensor"], "tensor_fft_s": st["t_tensor_fft"],
+           "ms_per_recalc": 1e3 * dt, "kernels_ms": {"fwd_fft": parts[1], "contraction": parts[2],
+                                                       "inv_fft": parts[3]},
+           "contraction_gbs": b["contraction"] / (parts[2] * 1e-3) / 1e9,
+           "contraction_frac": b["contraction"] / (parts[2] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+           "recalc_gbs": sum(b.values()) / (parts[0] * 1e-3) / 1e9,
+           "algorithmic_bytes": b, "tensor_bytes": 4 * m.Nx * m.Ny * 15 * rc.Nr * m.Nz * 16}
+    if cpu:
+        from oracle import pyoracle as po
+        om = po.Model(m.csigb(), m.thick, m.omega, m.Nx, m.Ny, m.Nz, m.dx, m.dy, m.z)
+        ro = po.Receivers(om, xs, ys, zs)
+        Gs = rc.get_tensor()
+        En = E.cpu().numpy()
+        nth = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t0 = time.perf_counter()
+        Eo, Ho = ro.apply(Gs, m.csiga(), En, nthreads=nth)
+        tc = time.perf_counter() - t0
+        rel = max(float(np.linalg.norm(Ea.cpu().numpy() - Eo) / np.linalg.norm(Eo)),
+                  float(np.linalg.norm(Ha.cpu().numpy() - Ho) / np.linalg.norm(Ho)))
+        row["cpu_baseline"] = {"recalc_s": tc, "cores": nth, "kind": "port", "gpu_vs_cpu_rel": rel}
+    rc.close()
+    log("recalc", json.dumps(row))
+    return row
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,6 +326,7 @@ def main():
         line["cpu_baseline"] = cb
     else:
         line["cpu_baseline"] = None
+    line["recalc"] = recalc_bench(torch, gop, op, m, v, cpu=not a.no_cpu)
     op.close()
     if not a.no_solve:
         line["solve"] = solve_bench(torch, gop, models)

@@ -1130,6 +1130,7 @@ void rc_apply(RcOperator& rc, const zd* E, zd* Ea, zd* Ha, hipStream_t s) {
     hipLaunchKernelGGL((k_fwd_y<double, LL, true>), dim3(Nx2, (3 * op.Nz + nry - 1) / nry), dim3(NT), 0, s,
                        rc.Xf, rc.S, twy, gf);
   });
+  G2G_HIP(hipEventRecord(op.ev[7], s));
   {
     int P = 1;
     while (P < op.Nz) P <<= 1;

@@ -1008,14 +1008,16 @@ int giem2g_rc_stats(void* p, double* o) {
   return guard([&] {
     RcOperator& rc = RC(p);
     Operator& op = *rc.op;
-    float ms[3] = {0, 0, 0};
+    float ms[4] = {0, 0, 0, 0};
     if (rc.applies > 0) {
       G2G_HIP(hipEventSynchronize(op.ev[6]));
       G2G_HIP(hipEventElapsedTime(&ms[0], op.ev[4], op.ev[6]));
-      G2G_HIP(hipEventElapsedTime(&ms[1], op.ev[4], op.ev[5]));
-      G2G_HIP(hipEventElapsedTime(&ms[2], op.ev[5], op.ev[6]));
+      G2G_HIP(hipEventElapsedTime(&ms[1], op.ev[4], op.ev[7]));
+      G2G_HIP(hipEventElapsedTime(&ms[2], op.ev[7], op.ev[5]));
+      G2G_HIP(hipEventElapsedTime(&ms[3], op.ev[5], op.ev[6]));
     }
-    const double v[7] = {rc.t_tensor, rc.t_fft, ms[0], ms[1], ms[2], (double)rc.applies, (double)rc.bytes};
+    const double v[8] = {rc.t_tensor, rc.t_fft, ms[0], ms[1], ms[2], ms[3], (double)rc.applies,
+                         (double)rc.bytes};
     std::memcpy(o, v, sizeof v);
   });
 }

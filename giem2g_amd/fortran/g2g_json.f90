@@ -11,7 +11,7 @@ module g2g_json
   implicit none
   private
   public :: json_doc, json_load, json_has, json_get_real, json_get_int, json_get_logical, &
-            json_get_string, json_get_real_array, json_array_len
+            json_get_string, json_get_real_array, json_array_len, json_get_elem_real
 
   integer, parameter :: K_OBJ = 1, K_ARR = 2, K_NUM = 3, K_STR = 4, K_BOOL = 5, K_NULL = 6
   integer, parameter :: MAXKEY = 128
@@ -348,5 +348,35 @@ contains
        c = d%next(c)
     end do
   end subroutine json_get_real_array
+
+  double precision function json_get_elem_real(d, path, i, field, default) result(v)
+    ! number `field` of the i-th (1-based) object of the array at path
+    ! ("Recievers", 2, "depth"); default when absent
+    type(json_doc), intent(in) :: d
+    character(len=*), intent(in) :: path, field
+    integer, intent(in) :: i
+    double precision, intent(in) :: default
+    integer :: id, c, k
+    v = default
+    id = find(d, path)
+    if (id == 0) return
+    if (d%kind(id) /= K_ARR) return
+    c = d%first(id)
+    k = 1
+    do while (c /= 0 .and. k < i)
+       c = d%next(c)
+       k = k + 1
+    end do
+    if (c == 0) return
+    if (d%kind(c) /= K_OBJ) return
+    c = d%first(c)
+    do while (c /= 0)
+       if (trim(d%key(c)) == field) then
+          if (d%kind(c) == K_NUM) v = d%num(c)
+          return
+       end if
+       c = d%next(c)
+    end do
+  end function json_get_elem_real
 
 end module g2g_json

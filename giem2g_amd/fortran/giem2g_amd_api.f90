@@ -104,6 +104,55 @@ module giem2g_amd_api
        type(c_ptr), value :: op, net
        real(c_double), value :: accept
      end function giem2g_op_set_learned
+     ! continuation of the solution to receivers
+     integer(c_int) function giem2g_rc_create(ie_op, nr, xshift, yshift, depth, rc) bind(c)
+       import
+       type(c_ptr), value :: ie_op
+       integer(c_int), value :: nr
+       real(c_double) :: xshift(*), yshift(*), depth(*)
+       type(c_ptr) :: rc
+     end function giem2g_rc_create
+
+     integer(c_int) function giem2g_rc_destroy(rc) bind(c)
+       import
+       type(c_ptr), value :: rc
+     end function giem2g_rc_destroy
+
+     integer(c_int) function giem2g_rc_calc_kernel(rc) bind(c)
+       import
+       type(c_ptr), value :: rc
+     end function giem2g_rc_calc_kernel
+
+     integer(c_int) function giem2g_rc_calc_fft_of_kernel(rc) bind(c)
+       import
+       type(c_ptr), value :: rc
+     end function giem2g_rc_calc_fft_of_kernel
+
+     integer(c_int) function giem2g_rc_receivers(rc, zr, layer, cell) bind(c)
+       import
+       type(c_ptr), value :: rc
+       real(c_double) :: zr(*)
+       integer(c_int) :: layer(*), cell(*)
+     end function giem2g_rc_receivers
+
+     integer(c_int) function giem2g_rc_set_anomaly_conductivity(rc, csiga) bind(c)
+       import
+       type(c_ptr), value :: rc
+       complex(c_double_complex) :: csiga(*)
+     end function giem2g_rc_set_anomaly_conductivity
+
+     integer(c_int) function giem2g_rc_recalculate(rc, eint, ea, ha) bind(c)
+       import
+       type(c_ptr), value :: rc
+       complex(c_double_complex) :: eint(*), ea(*), ha(*)
+     end function giem2g_rc_recalculate
+
+     integer(c_int) function giem2g_rc_plane_wave(rc, pol, f0) bind(c)
+       import
+       type(c_ptr), value :: rc
+       integer(c_int), value :: pol
+       complex(c_double_complex) :: f0(*)
+     end function giem2g_rc_plane_wave
   end interface
 
 contains

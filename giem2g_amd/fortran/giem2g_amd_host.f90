@@ -13,11 +13,21 @@
 ! in (z, component, x, y) order (SaveIESolutionOneFileBinary,
 ! mpi_saveload_module.F90:568-600).
 !
+! With "Recalculate Fields" and a "Recievers" list the solution is continued
+! to the receivers after each solve (ReCalculation, giem2g.F90:412-428,
+! 487-503): the anomalous E and H at every cell centre shifted by each
+! receiver's (xshift, yshift) at its depth, plus the normal field, written to
+! P{Y,X}_F<freq>.dat in the 28-column text format of SaveOutputOneFile
+! (mpi_saveload_module.F90:391-483).  With "Solve Equation": false the
+! solutions are read from SOL_P{Y,X}_F<freq>.bin instead of solved, and a
+! frequency whose EY solution is missing is skipped (giem2g.F90:390-411).
+!
 ! Deck keys (as the reference's, giem2g.F90 and mpi_saveload_module.F90):
 !   Background.Conductivity [Nl], Background.Thickness [Nl-1],
 !   Anomaly.{Nx, Ny, Nz, dx, dy, zborders[Nz+1], Conductivity (file name)},
 !   Frequencies [...], FGMRES.{OuterBufferSize, InnerBufferSize, MaxRestart,
-!   Tolerance}, "Save Solution".
+!   Tolerance}, "Save Solution", "Solve Equation", "Recalculate Fields",
+!   Recievers [{xshift, yshift, depth}].
 ! The conductivity file holds Nx*Ny*Nz values in (Nz, Nx, Ny) order
 ! (LoadAnomalySigma, mpi_saveload_module.F90:113-147).
 ! Keys of this build: Solver.{InnerPrecision (0 FP64, 1 complex64),
@@ -36,8 +46,8 @@ program giem2g_amd_host
   type(json_doc) :: deck
   character(len=4096) :: deck_path, sig_path, net_path, fname
   character(len=22) :: fnum
-  logical :: ok, found, save_solution, have_guess
-  integer :: nl, nx, ny, nz, nf, n, i, ix, iy, iz, ic, ifreq, ipol, pol, rc, ncell, u, ios
+  logical :: ok, found, save_solution, have_guess, solve_equation, recalc
+  integer :: nl, nx, ny, nz, nf, n, i, ix, iy, iz, ic, ifreq, ipol, pol, rc, ncell, u, ios, nr, ir
   double precision :: dx, dy, omega, accept
   double precision, allocatable :: sigb(:), thick(:), z(:), freqs(:), siga(:, :, :)
   complex(c_double_complex), allocatable :: csigb(:), csiga(:), ebkg(:), esol(:)
@@ -45,7 +55,10 @@ program giem2g_amd_host
   complex(c_double_complex), allocatable :: e4(:, :, :, :), f4(:, :, :, :)
   real(c_float), allocatable :: wnet(:)
   integer(c_long_long) :: np
-  type(c_ptr) :: op, net
+  type(c_ptr) :: op, net, rcop
+  double precision, allocatable :: rxs(:), rys(:), rdepth(:), rzr(:)
+  integer(c_int), allocatable :: rlayer(:), rcell(:)
+  complex(c_double_complex), allocatable :: ea(:, :, :, :), ha(:, :, :, :), f0(:, :)
   type(giem2g_solver_params) :: prm
   type(giem2g_solve_info) :: info
   logical :: have_prev(0:1)
@@ -97,6 +110,22 @@ program giem2g_amd_host
   prm%reserved = 0
   accept = json_get_real(deck, 'Solver.Accept', 1d0)
   save_solution = json_get_logical(deck, 'Save Solution', .true.)
+  solve_equation = json_get_logical(deck, 'Solve Equation', .true.)
+  ! receivers (LoadRecievers, mpi_saveload_module.F90:174-213)
+  recalc = json_get_logical(deck, 'Recalculate Fields', .true.)
+  nr = max(json_array_len(deck, 'Recievers'), 0)
+  if (recalc .and. nr == 0) then
+     write (*, '(A)') 'no Recievers in the deck: fields are not continued to receivers'
+     recalc = .false.
+  end if
+  if (recalc) then
+     allocate(rxs(nr), rys(nr), rdepth(nr), rzr(nr), rlayer(nr), rcell(nr))
+     do ir = 1, nr
+        rxs(ir) = json_get_elem_real(deck, 'Recievers', ir, 'xshift', 0d0)
+        rys(ir) = json_get_elem_real(deck, 'Recievers', ir, 'yshift', 0d0)
+        rdepth(ir) = json_get_elem_real(deck, 'Recievers', ir, 'depth', 0d0)
+     end do
+  end if
 
   ! anomaly conductivity, file order (Nz, Nx, Ny)
   call json_get_string(deck, 'Anomaly.Conductivity', sig_path, found)
@@ -111,6 +140,11 @@ program giem2g_amd_host
      write (*, '(A,*(ES15.7))') 'Frequencies ', freqs
      write (*, '(A,3(1X,I0),1X,ES15.7)') 'FGMRES', prm%outer_m, prm%inner_m, prm%max_restart, prm%tol
      write (*, '(A,L1)') 'SaveSolution ', save_solution
+     write (*, '(A,L1)') 'SolveEquation ', solve_equation
+     write (*, '(A,L1)') 'RecalculateFields ', recalc
+     do ir = 1, merge(nr, 0, recalc)
+        write (*, '(A,3(1X,ES15.7))') 'Reciever', rxs(ir), rys(ir), rdepth(ir)
+     end do
      write (*, '(A,A)') 'SigmaFile ', trim(sig_path)
      stop
   end if
@@ -128,6 +162,11 @@ program giem2g_amd_host
 
   rc = giem2g_op_create(nx, ny, nz, dx, dy, z, op)
   call check(rc, 'giem2g_op_create')
+  rcop = c_null_ptr
+  if (recalc) then
+     call check(giem2g_rc_create(op, nr, rxs, rys, rdepth, rcop), 'giem2g_rc_create')
+     allocate(ea(nr, 3, nx, ny), ha(nr, 3, nx, ny), f0(nr, 6))
+  end if
   net = c_null_ptr
   if (prm%precond /= 0) then
      call json_get_string(deck, 'Solver.Network', net_path, found)
@@ -152,7 +191,7 @@ program giem2g_amd_host
         csigb(i) = cmplx(sigb(i), -omega * EPS0, c_double_complex)
      end do
      call check(giem2g_op_set_background(op, nl, csigb, thick, omega), 'giem2g_op_set_background')
-     call check(giem2g_op_build_tensor(op), 'giem2g_op_build_tensor')
+     if (solve_equation) call check(giem2g_op_build_tensor(op), 'giem2g_op_build_tensor')
      ! SetAnomalySigma (ie_solver_module.F90:129-131): (Nz,Nx,Ny) -> (Nx,Ny,Nz)
      do iy = 1, ny
         do ix = 1, nx
@@ -162,7 +201,15 @@ program giem2g_amd_host
            end do
         end do
      end do
-     call check(giem2g_op_set_sigma(op, csiga), 'giem2g_op_set_sigma')
+     if (solve_equation) call check(giem2g_op_set_sigma(op, csiga), 'giem2g_op_set_sigma')
+     if (recalc) then
+        ! CalcRecalculationGreenTensor, CalcFFTofRCTensor (giem2g.F90:340-343),
+        ! FinalizeRCOperator
+        call check(giem2g_rc_calc_kernel(rcop), 'giem2g_rc_calc_kernel')
+        call check(giem2g_rc_calc_fft_of_kernel(rcop), 'giem2g_rc_calc_fft_of_kernel')
+        call check(giem2g_rc_set_anomaly_conductivity(rcop, csiga), 'giem2g_rc_set_anomaly_conductivity')
+        call check(giem2g_rc_receivers(rcop, rzr, rlayer, rcell), 'giem2g_rc_receivers')
+     end if
      write (fnum, '(F22.10)') freqs(ifreq)
      do i = 1, 11
         if (fnum(i:i) == ' ') fnum(i:i) = '0'
@@ -170,7 +217,6 @@ program giem2g_amd_host
      do ipol = 1, 2
         pol = GIEM2G_PW_EY
         if (ipol == 2) pol = GIEM2G_PW_EX
-        call check(giem2g_plane_wave(op, pol, ebkg), 'giem2g_plane_wave')
         fname = 'SOL_' // pname(pol) // '_F' // trim(fnum) // '.bin'
         inquire (file=trim(fname), exist=have_guess)
         if (have_guess) then
@@ -186,6 +232,16 @@ program giem2g_amd_host
            have_prev(pol) = .true.
            write (*, '(A)') 'Load IE Solution from ' // trim(fname)
         end if
+        if (.not. solve_equation) then
+           ! recalculation only (giem2g.F90:390-411, 473-486)
+           if (.not. have_guess) then
+              write (*, '(A)') 'Solution not found: ' // trim(fname)
+              exit
+           end if
+           call recalculate(eprev(:, pol))
+           cycle
+        end if
+        call check(giem2g_plane_wave(op, pol, ebkg), 'giem2g_plane_wave')
         if (have_prev(pol)) then
            call check(giem2g_solve(op, ebkg, c_loc(eprev(1, pol)), prm, esol, info), 'giem2g_solve')
         else
@@ -208,8 +264,10 @@ program giem2g_amd_host
            write (u) f4
            close(u)
         end if
+        if (recalc) call recalculate(esol)
      end do
   end do
+  if (c_associated(rcop)) call check(giem2g_rc_destroy(rcop), 'giem2g_rc_destroy')
   if (c_associated(net)) then
      call check(giem2g_op_set_learned(op, c_null_ptr, 1d0), 'giem2g_op_set_learned')
      call check(giem2g_unet_destroy(net), 'giem2g_unet_destroy')
@@ -217,6 +275,39 @@ program giem2g_amd_host
   call check(giem2g_op_destroy(op), 'giem2g_op_destroy')
 
 contains
+
+  ! ReCalculation of the field E of polarization pol at the receivers, the
+  ! total field with PlaneWave (giem2g.F90:412-426), and SaveOutputOneFile
+  ! (mpi_saveload_module.F90:391-483) to P{Y,X}_F<freq>.dat
+  subroutine recalculate(e)
+    complex(c_double_complex), intent(in) :: e(:)
+    character(len=*), parameter :: title_fmt = '(A1, 28A20)', output_fmt = '(A1, 28ES20.10E3)'
+    character(len=4096) :: oname
+    double precision :: x, y
+    integer :: uo, k, jr, jx, jy
+    call check(giem2g_rc_recalculate(rcop, e, ea, ha), 'giem2g_rc_recalculate')
+    call check(giem2g_rc_plane_wave(rcop, pol, f0), 'giem2g_rc_plane_wave')
+    oname = pname(pol) // '_F' // trim(fnum) // '.dat'
+    open(newunit=uo, file=trim(oname), status='replace')
+    write (uo, title_fmt) '%', 'x_recv', 'y_recv', 'z_recv', 'frequency', &
+         'Re Ex^a', 'Im Ex^a', 'Re Ey^a', 'Im Ey^a', 'Re Ez^a', 'Im Ez^a', &
+         'Re Hx^a', 'Im Hx^a', 'Re Hy^a', 'Im Hy^a', 'Re Hz^a', 'Im Hz^a', &
+         'Re Ex^t', 'Im Ex^t', 'Re Ey^t', 'Im Ey^t', 'Re Ez^t', 'Im Ez^t', &
+         'Re Hx^t', 'Im Hx^t', 'Re Hy^t', 'Im Hy^t', 'Re Hz^t', 'Im Hz^t'
+    do jr = 1, nr
+       do jy = 1, ny
+          y = jy * dy - dy / 2d0 + rys(jr)
+          do jx = 1, nx
+             x = jx * dx - dx / 2d0 + rxs(jr)
+             write (uo, output_fmt) ' ', x, y, rzr(jr), freqs(ifreq), &
+                  (ea(jr, k, jx, jy), k = 1, 3), (ha(jr, k, jx, jy), k = 1, 3), &
+                  (ea(jr, k, jx, jy) + f0(jr, k), k = 1, 3), (ha(jr, k, jx, jy) + f0(jr, 3 + k), k = 1, 3)
+          end do
+       end do
+    end do
+    close(uo)
+    write (*, '(A)') 'Fields at receivers written to ' // trim(oname)
+  end subroutine recalculate
 
   subroutine check(code, what)
     integer(c_int), intent(in) :: code

@@ -303,7 +303,8 @@ class ReceiverOperator:
         return F.reshape(6, self.Nr).T.copy()
 
     def stats(self):
-        o = np.zeros(7)
+        o = np.zeros(8)
         L.check(self.lib.giem2g_rc_stats(self.h, o.ctypes.data))
-        keys = ["t_tensor", "t_tensor_fft", "ms_last", "ms_fwd_contract", "ms_inv", "recalcs", "bytes"]
+        keys = ["t_tensor", "t_tensor_fft", "ms_last", "ms_fwd", "ms_contract", "ms_inv", "recalcs",
+                "bytes"]
         return dict(zip(keys, o))

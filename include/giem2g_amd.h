@@ -326,8 +326,8 @@ int giem2g_rc_recalculate_dev(void *rc_op, const void *E_int_dev, void *Ea_dev, 
  * depths for pol 0 (EX) or 1 (EY): complex (Nr,6) Ex,Ey,Ez,Hx,Hy,Hz. */
 int giem2g_rc_plane_wave(void *rc_op, int pol, double *F0_host);
 /* [tensor build s, tensor FFT s, last recalculation ms, of which forward
- * FFT + contraction ms, inverse FFT ms, recalculations, HBM bytes]. */
-int giem2g_rc_stats(void *rc_op, double *out7);
+ * FFT ms, contraction ms, inverse FFT ms, recalculations, HBM bytes]. */
+int giem2g_rc_stats(void *rc_op, double *out8);
 
 #ifdef __cplusplus
 }

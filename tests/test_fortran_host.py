@@ -43,6 +43,9 @@ def test_deck_parse_reference_deck():
     assert np.allclose([float(v) for v in lines["Frequencies"]], [1e-3, 1e-2, 1e-1, 1.0])
     assert lines["FGMRES"][:3] == ["15", "15", "300"] and float(lines["FGMRES"][3]) == 1e-8
     assert lines["SigmaFile"] == ["128_128_501.dat"]
+    assert lines["SolveEquation"] == ["T"] and lines["RecalculateFields"] == ["T"]
+    recv = [[float(v) for v in l.split()[1:]] for l in out.stdout.splitlines() if l.startswith("Reciever")]
+    assert np.allclose(recv, [[0, -20000, 0], [0, 20000, 0]])
 
 
 def test_deck_parse_errors(tmp_path):
@@ -52,11 +55,11 @@ def test_deck_parse_errors(tmp_path):
     assert out.returncode != 0 and "Thickness" in out.stdout
 
 
-def write_case(d, m, freqs, outer=10, inner=10):
+def write_case(d, m, freqs, outer=10, inner=10, receivers=None, solve=True):
     # list-directed text in Fortran (Nz, Nx, Ny) order: z fastest, then x, then y
     sig = np.transpose(m.siga, (1, 2, 0)).reshape(-1)  # siga is (Nz, Ny, Nx), C order
     np.savetxt(os.path.join(d, "sigma.dat"), sig, fmt="%.17e")
-    deck = {"Save Solution": True, "Solve Equation": True, "Recalculate Fields": False,
+    deck = {"Save Solution": True, "Solve Equation": solve, "Recalculate Fields": receivers is not None,
             "Background": {"Conductivity": list(m.sigma_bkg), "Permittivity": [],
                            "Thickness": list(m.thick)},
             "Anomaly": {"Nx": m.Nx, "Ny": m.Ny, "Nz": m.Nz, "dx": m.dx, "dy": m.dy,
@@ -64,6 +67,8 @@ def write_case(d, m, freqs, outer=10, inner=10):
             "Frequencies": list(freqs),
             "FGMRES": {"OuterBufferSize": outer, "InnerBufferSize": inner, "MaxRestart": 50,
                        "Tolerance": 1e-8}}
+    if receivers is not None:
+        deck["Recievers"] = [{"xshift": x, "yshift": y, "depth": z} for x, y, z in receivers]
     with open(os.path.join(d, "deck.json"), "w") as f:
         json.dump(deck, f, indent=1)
 
@@ -123,3 +128,93 @@ def test_fortran_host_multifrequency(tmp_path, oracle):
     its = [int(l.split("outer")[1].split()[0]) for l in r2.stdout.splitlines() if l.startswith("freq")]
     its1 = [int(l.split("outer")[1].split()[0]) for l in runs]
     assert all(a <= 1 for a in its) and sum(its) < sum(its1), (its, its1)
+
+
+def fnum_of(f):
+    """WRITE (fnum, '(F22.10)') with blanks of the first 11 columns -> '0'."""
+    return "".join("0" if (i < 11 and c == " ") else c for i, c in enumerate("%22.10f" % f)).strip()
+
+
+def read_dat(path):
+    """SaveOutputOneFile text (mpi_saveload_module.F90:391-483): title line
+    and rows of x, y, z, f, then re/im of Ea, Ha, Et, Ht (x, y, z each)."""
+    with open(path) as fh:
+        title = fh.readline()
+        rows = np.array([[float(v) for v in l.split()] for l in fh if l.strip()])
+    return title, rows
+
+
+@pytest.mark.gpu
+def test_fortran_host_receivers(tmp_path, oracle):
+    """Solve + continuation to receivers (giem2g.F90:412-428, 487-503): the
+    P{Y,X}_F*.dat columns against the Python API (same solution, receiver
+    operator, normal field) and the oracle's continuation; then a rerun
+    with "Solve Equation": false recalculates from the saved solutions and
+    writes the same files."""
+    from giem2g_amd import models, operator as gop
+    host = _host()
+    m = models.small(16, 16, 4, seed=33, freq=1.0)
+    freqs = [0.7]
+    recv = [(0.0, 0.0, 0.0), (30.0, -40.0, 125.0), (-45.0, 10.0, 400.0)]
+    write_case(str(tmp_path), m, freqs, receivers=recv)
+    print("host run receivers", flush=True)
+    r = subprocess.run([host, "deck.json"], cwd=str(tmp_path), capture_output=True, text=True, timeout=300)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    m.freq = freqs[0]
+    op = gop.operator_for(m)
+    rc = gop.ReceiverOperator(op, [v[0] for v in recv], [v[1] for v in recv], [v[2] for v in recv])
+    rc.build()
+    rc.set_sigma(m.csiga())
+    zr, _, _ = rc.receivers()
+    om = oracle.Model(m.csigb(), m.thick, m.omega, m.Nx, m.Ny, m.Nz, m.dx, m.dy, m.z)
+    ro = oracle.Receivers(om, [v[0] for v in recv], [v[1] for v in recv], [v[2] for v in recv])
+    Gs = ro.tensor_fft(ro.tensor_realspace())
+    Nr = len(recv)
+    texts = {}
+    for p, pol in (("PY", 1), ("PX", 0)):
+        path = tmp_path / ("%s_F%s.dat" % (p, fnum_of(freqs[0])))
+        title, rows = read_dat(str(path))
+        texts[p] = path.read_text()
+        names = ["%s%s^%s" % (f, a, k) for k in "at" for f in "EH" for a in "xyz"]
+        assert title.split() == ["%", "x_recv", "y_recv", "z_recv", "frequency"] + \
+            [w for n in names for w in ("Re", n, "Im", n)]
+        assert rows.shape == (Nr * m.Nx * m.Ny, 28)
+        sol = read_sol(str(tmp_path / ("SOL_%s_F%s.bin" % (p, fnum_of(freqs[0])))), m)
+        Ea, Ha = rc.recalculate(sol)
+        F0 = rc.plane_wave(pol)
+        A = lambda F: F.reshape(m.Ny, m.Nx, 3, Nr)  # Fortran (Nr,3,Nx,Ny) as C (y,x,c,r)  # noqa: E731
+        Ea4, Ha4 = A(Ea), A(Ha)
+        # rows: r outer, then y, then x
+        want = np.zeros((Nr, m.Ny, m.Nx, 28))
+        for ir in range(Nr):
+            x = (np.arange(1, m.Nx + 1) * m.dx - m.dx / 2 + recv[ir][0])[None, :]
+            y = (np.arange(1, m.Ny + 1) * m.dy - m.dy / 2 + recv[ir][1])[:, None]
+            want[ir, :, :, 0] = x
+            want[ir, :, :, 1] = y
+            want[ir, :, :, 2] = zr[ir]
+            want[ir, :, :, 3] = freqs[0]
+            cols = [Ea4[:, :, k, ir] for k in range(3)] + [Ha4[:, :, k, ir] for k in range(3)] + \
+                   [Ea4[:, :, k, ir] + F0[ir, k] for k in range(3)] + [Ha4[:, :, k, ir] + F0[ir, 3 + k] for k in range(3)]
+            for j, c in enumerate(cols):
+                want[ir, :, :, 4 + 2 * j] = c.real
+                want[ir, :, :, 5 + 2 * j] = c.imag
+        want = want.reshape(-1, 28)
+        assert np.allclose(rows[:, :4], want[:, :4], rtol=1e-10, atol=1e-9)
+        for j in range(4, 28, 2):  # printed with 11 significant digits
+            got = rows[:, j] + 1j * rows[:, j + 1]
+            ref = want[:, j] + 1j * want[:, j + 1]
+            assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max(), (p, j)
+        Eo, Ho = ro.apply(Gs, m.csiga(), sol)
+        assert rel(Ea, Eo) < 1e-9 and rel(Ha, Ho) < 1e-9
+    rc.close()
+    op.close()
+    # recalculation only, from the saved solutions
+    write_case(str(tmp_path), m, freqs, receivers=recv, solve=False)
+    for p in ("PY", "PX"):
+        os.remove(str(tmp_path / ("%s_F%s.dat" % (p, fnum_of(freqs[0])))))
+    r2 = subprocess.run([host, "deck.json"], cwd=str(tmp_path), capture_output=True, text=True, timeout=300)
+    assert r2.returncode == 0, r2.stdout + r2.stderr
+    assert not any(l.startswith("freq") for l in r2.stdout.splitlines())
+    for p in ("PY", "PX"):
+        assert (tmp_path / ("%s_F%s.dat" % (p, fnum_of(freqs[0])))).read_text() == texts[p]
