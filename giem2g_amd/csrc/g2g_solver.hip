@@ -29,17 +29,22 @@ namespace g2g {
 using zc = std::complex<double>;
 constexpr int BT = 256;
 constexpr int NBLK = 1024;  // blocks for reductions
-constexpr int KC = 8;       // columns per multi-dot pass
+constexpr int KC = 8;       // columns per multi-dot block row
+constexpr int MAXK = 64;    // columns of one multi-dot call (Krylov basis size bound)
 
 template <typename A> __device__ inline A wave_sum(A v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
 }
 
-// part[blockIdx.x][2*KC]: partial sums of conj(V[:,k]) . w for k < kc
+// Partial sums of conj(V[:,k]) . w for k < K: block row blockIdx.y takes
+// columns KC*y .. KC*y + KC - 1 and writes part[(y*gridDim.x + x)*2KC + j]
 template <typename T2, typename A>
-__global__ __launch_bounds__(BT) void k_mdot(const T2* __restrict__ V, int64_t N, int kc,
+__global__ __launch_bounds__(BT) void k_mdot(const T2* __restrict__ V, int64_t N, int K,
                                              const T2* __restrict__ w, A* __restrict__ part) {
+  const int k0 = blockIdx.y * KC, kc = K - k0 < KC ? K - k0 : KC;
+  V += (int64_t)k0 * N;
+  part += (int64_t)blockIdx.y * gridDim.x * 2 * KC;
   A re[KC], im[KC];
 #pragma unroll
   for (int k = 0; k < KC; k++) re[k] = im[k] = 0;
@@ -71,9 +76,11 @@ __global__ __launch_bounds__(BT) void k_mdot(const T2* __restrict__ V, int64_t N
   }
 }
 
-// out[j] = sum_b part[b][j]; one block, 2*KC outputs (written as double)
+// out[2KC*y + j] = sum_b part[(y*nb + b)*2KC + j]; block y, 2*KC outputs (as double)
 template <typename A>
 __global__ __launch_bounds__(BT) void k_reduce(const A* __restrict__ part, int nb, double* out) {
+  part += (int64_t)blockIdx.x * nb * 2 * KC;
+  out += blockIdx.x * 2 * KC;
   __shared__ double red[BT / 64][2 * KC];
   double acc[2 * KC];
   for (int j = 0; j < 2 * KC; j++) acc[j] = 0;
@@ -178,15 +185,15 @@ static inline dim3 gridN(int64_t N) {
 // ---------------------------------------------------------------- BLAS ---
 struct DevBlas {
   hipStream_t s;
-  void* part = nullptr;  // NBLK * 2KC doubles
-  double* res = nullptr; // 2KC
+  void* part = nullptr;  // (MAXK/KC) * NBLK * 2KC doubles
+  double* res = nullptr; // 2*MAXK dot products (complex pairs) + 2KC for a norm
   zd* hbuf = nullptr; // 64
-  double hres[2 * KC];
+  double hres[2 * MAXK + 2 * KC];
   int64_t* dotcount = nullptr;
   void init(hipStream_t st) {
     s = st;
-    G2G_HIP(hipMalloc(&part, sizeof(double) * NBLK * 2 * KC));
-    G2G_HIP(hipMalloc(&res, sizeof(double) * 2 * KC));
+    G2G_HIP(hipMalloc(&part, sizeof(double) * (MAXK / KC) * NBLK * 2 * KC));
+    G2G_HIP(hipMalloc(&res, sizeof(double) * (2 * MAXK + 2 * KC)));
     G2G_HIP(hipMalloc(&hbuf, sizeof(zd) * 64));
   }
   void fini() {
@@ -195,37 +202,44 @@ struct DevBlas {
     if (hbuf) (void)hipFree(hbuf);
     part = nullptr;
   }
-  // res[k] = V[:,k]^H w for k < K
+  static int nblocks(int64_t N) { return (int)std::min<int64_t>(NBLK, (N + BT - 1) / BT); }
+  // res[0 .. 2K) = V[:,k]^H w for k < K, on the device (no host round trip)
+  template <typename T2, typename A> void mdot_dev(const T2* V, int64_t N, int K, const T2* w) {
+    G2G_CHECK(K >= 1 && K <= MAXK, -3, "multi-dot of more than 64 columns");
+    const int nb = nblocks(N), nrow = (K + KC - 1) / KC;
+    hipLaunchKernelGGL((k_mdot<T2, A>), dim3(nb, nrow), dim3(BT), 0, s, V, N, K, w, (A*)part);
+    hipLaunchKernelGGL((k_reduce<A>), dim3(nrow), dim3(BT), 0, s, (const A*)part, nb, res);
+  }
+  void fetch(int n) {
+    G2G_HIP(hipMemcpyAsync(hres, res, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    G2G_HIP(hipStreamSynchronize(s));
+  }
+  // out[k] = V[:,k]^H w for k < K
   template <typename T2, typename A>
   void mdot(const T2* V, int64_t N, int K, const T2* w, zc* out) {
-    for (int k0 = 0; k0 < K; k0 += KC) {
-      const int kc = K - k0 < KC ? K - k0 : KC;
-      const int nb = (int)std::min<int64_t>(NBLK, (N + BT - 1) / BT);
-      hipLaunchKernelGGL((k_mdot<T2, A>), dim3(nb), dim3(BT), 0, s, V + (int64_t)k0 * N, N, kc, w,
-                         (A*)part);
-      hipLaunchKernelGGL((k_reduce<A>), dim3(1), dim3(BT), 0, s, (const A*)part, nb, res);
-      G2G_HIP(hipMemcpyAsync(hres, res, sizeof(double) * 2 * KC, hipMemcpyDeviceToHost, s));
-      G2G_HIP(hipStreamSynchronize(s));
-      for (int k = 0; k < kc; k++) out[k0 + k] = zc(hres[2 * k], hres[2 * k + 1]);
-    }
+    mdot_dev<T2, A>(V, N, K, w);
+    fetch(2 * K);
+    for (int k = 0; k < K; k++) out[k] = zc(hres[2 * k], hres[2 * k + 1]);
     if (dotcount) *dotcount += K;
+  }
+  // One classical Gram-Schmidt pass: h = V^H w, w -= V h, returns |w|; h goes
+  // from the reduction straight to the update kernel, one host round trip.
+  template <typename T2, typename A>
+  double ortho_pass(T2* w, const T2* V, int64_t N, int K, zc* h) {
+    mdot_dev<T2, A>(V, N, K, w);
+    const int nb = nblocks(N);
+    hipLaunchKernelGGL((k_update_norm<T2, A>), dim3(nb), dim3(BT), 0, s, w, V, N, K, (const zd*)res,
+                       (A*)part);
+    hipLaunchKernelGGL((k_reduce<A>), dim3(1), dim3(BT), 0, s, (const A*)part, nb, res + 2 * MAXK);
+    fetch(2 * MAXK + 2 * KC);
+    for (int k = 0; k < K; k++) h[k] = zc(hres[2 * k], hres[2 * k + 1]);
+    if (dotcount) *dotcount += K + 1;
+    return std::sqrt(hres[2 * MAXK]);
   }
   template <typename T2, typename A> double norm(const T2* v, int64_t N) {
     zc t;
     mdot<T2, A>(v, N, 1, v, &t);
     return std::sqrt(t.real());
-  }
-  // w -= V h, returns |w|
-  template <typename T2, typename A>
-  double update_norm(T2* w, const T2* V, int64_t N, int K, const zc* h) {
-    G2G_HIP(hipMemcpyAsync(hbuf, h, sizeof(zd) * K, hipMemcpyHostToDevice, s));
-    const int nb = (int)std::min<int64_t>(NBLK, (N + BT - 1) / BT);
-    hipLaunchKernelGGL((k_update_norm<T2, A>), dim3(nb), dim3(BT), 0, s, w, V, N, K, hbuf, (A*)part);
-    hipLaunchKernelGGL((k_reduce<A>), dim3(1), dim3(BT), 0, s, (const A*)part, nb, res);
-    G2G_HIP(hipMemcpyAsync(hres, res, sizeof(double) * 2 * KC, hipMemcpyDeviceToHost, s));
-    G2G_HIP(hipStreamSynchronize(s));
-    if (dotcount) *dotcount += 1;
-    return std::sqrt(hres[0]);
   }
   template <typename T2>
   void lincomb(T2* x, const T2* B, int64_t N, int K, const zc* y, bool acc) {
@@ -383,9 +397,8 @@ template <typename R> struct Gmres {
       double dnormw = 0;
       for (;;) {
         northo++;
-        blas->mdot<T2, R>(V, N, jH, w, dp.data());
+        dnormw = blas->ortho_pass<T2, R>(w, V, N, jH, dp.data());
         for (int i = 0; i < jH; i++) HH(i, j) += dp[i];
-        dnormw = blas->update_norm<T2, R>(w, V, N, jH, dp.data());
         double dloo = 0;
         for (int i = 0; i < jH; i++) dloo += std::norm(dp[i]);
         dloo = std::sqrt(dloo);
