@@ -304,7 +304,7 @@ def main():
                    "grid": [m.Nx, m.Ny, m.Nz], "precision": "complex128"},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_contract2<double,4> (tensor contraction, two components in flight)",
+                     "kernel": "k_contract2<double,4,256,32> (tensor contraction, two components in flight)",
                      "algorithmic_bytes_per_launch": bytes64["contraction"],
                      "avg_launch_ms": float(stage_ms[1])},
         "kernels_ms": {"fwd_fft": float(stage_ms[0]), "contraction": float(stage_ms[1]),

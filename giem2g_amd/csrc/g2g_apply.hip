@@ -357,14 +357,24 @@ __global__ __launch_bounds__(NT) void k_contract(cx_t<R>* __restrict__ S,
 
 // Two-deep variant of k_contract: two tensor components in flight in
 // registers while a third is computed from LDS.
-template <typename R, int PF, int NTC>  // NTC threads; Nz*Nz <= PF*NTC
+// NZT > 0: Nz known at compile time (a power of two dividing NTC, with
+// Nz >= NTC/Nz lanes per row), so the
+// per-row loops over source layers unroll and their LDS reads issue
+// together; NZT = 0: any Nz.  The inputs are stored pre-multiplied by the
+// point signs D_p = diag(s, m, 1), so every product accumulates straight
+// into the outputs (out_p = D_p G0 (D_p v_p), the sign pattern of
+// VERTICAL_MULT_GENERAL) and the signs are applied once at the end.
+template <typename R, int PF, int NTC, int NZT>  // NTC threads; Nz*Nz <= PF*NTC
 __global__ __launch_bounds__(NTC) void k_contract2(cx_t<R>* __restrict__ S,
                                                  const cx_t<R>* __restrict__ G, Geo g,
                                                  int64_t BS, int split_log) {
   using T2 = cx_t<R>;
   static_assert(PF == 4 || PF == 8 || PF == 16, "PF must be 4, 8 or 16");
+  // compile-time Nz: a power of two with at least one source layer per lane
+  static_assert(NZT == 0 || (NTC % NZT == 0 && (NZT & (NZT - 1)) == 0 && NZT * NZT >= NTC), "NZT");
   extern __shared__ unsigned char smem_raw[];
-  const int Nz = g.Nz, Nz3 = g.Nz3, NZP = Nz + 1;
+  const int Nz = NZT > 0 ? NZT : g.Nz;
+  const int Nz3 = 3 * Nz, NZP = Nz + 1;
   const int P = Nz * (Nz + 1) / 2;
   const int GB = Nz * NZP;                    // one LDS component buffer
   T2* vin = reinterpret_cast<T2*>(smem_raw);  // [4][3Nz]
@@ -414,17 +424,18 @@ __global__ __launch_bounds__(NTC) void k_contract2(cx_t<R>* __restrict__ S,
     const int p = i / Nz3, e = i - p * Nz3;
     T2 vv = zero;
     if (VAL(p)) vv = S[((int64_t)PX(p) * Ny2 + PY(p)) * Nz3 + e];
+    // D_p: x times s (-1 for -kx), y times m (-1 for -ky)
+    if ((e < Nz && (p & 1)) || (e >= Nz && e < 2 * Nz && (p & 2))) vv = -vv;
     vin[i] = vv;
   }
   G2G_STASH(0, F, gbuf)
   __syncthreads();
-  const int split = 1 << split_log;
+  constexpr int SPLIT = NZT > 0 ? NTC / NZT : 1;
+  const int split = NZT > 0 ? SPLIT : 1 << split_log;
   const int part = threadIdx.x & (split - 1);
-  const int i = threadIdx.x >> split_log;
+  const int i = NZT > 0 ? (int)threadIdx.x / SPLIT : (int)threadIdx.x >> split_log;
   const bool active = i < Nz;
-  // sign of point p: s = -1 for -kx (bit 0), m = -1 for -ky (bit 1)
-#define G2G_SG(p) (((p) & 1) ? R(-1) : R(1))
-#define G2G_MR(p) (((p) & 2) ? R(-1) : R(1))
+  const int nj = NZT > 0 ? NZT / SPLIT : (Nz - part + split - 1) / split;  // source layers per lane
 #define G2G_ACC(p) T2 ax##p = zero, ay##p = zero, az##p = zero;
   G2G_FOR4(G2G_ACC)
 #undef G2G_ACC
@@ -438,56 +449,43 @@ __global__ __launch_bounds__(NTC) void k_contract2(cx_t<R>* __restrict__ S,
       else G2G_PREFETCH(comp + 2, F)
     }
     if (active) {
-#define G2G_T(p) T2 t1##p = zero, t2##p = zero;
-      G2G_FOR4(G2G_T)
-#undef G2G_T
       if (comp < 4) {
         const int ii = i * (i + 1) / 2;
         const int off1 = comp == 0 ? 0 : (comp == 3 ? 2 * Nz : Nz);  // x for xx, z for zz, y otherwise
-        for (int j = part; j < Nz; j += split) {
+#pragma unroll
+        for (int jj = 0; jj < nj; jj++) {
+          const int j = part + jj * split;
           const T2 sv = cur[j <= i ? j + ii : i + j * (j + 1) / 2];
 #define G2G_SYM(p)                                      \
   {                                                     \
     const T2* v = vin + (p) * Nz3;                      \
-    cmac(t1##p, sv, v[off1 + j]);                       \
-    if (comp == 1) cmac(t2##p, sv, v[j]);               \
+    if (comp == 0) cmac(ax##p, sv, v[j]);               \
+    else if (comp == 1) {                               \
+      cmac(ax##p, sv, v[Nz + j]);                       \
+      cmac(ay##p, sv, v[j]);                            \
+    } else if (comp == 2) cmac(ay##p, sv, v[off1 + j]); \
+    else cmac(az##p, sv, v[off1 + j]);                  \
   }
           G2G_FOR4(G2G_SYM)
 #undef G2G_SYM
         }
-#define G2G_FOLD(p)                                                  \
-  if (comp == 0) ax##p = ax##p + t1##p;                              \
-  else if (comp == 1) {                                              \
-    ax##p = ax##p + t1##p * (G2G_SG(p) * G2G_MR(p));                 \
-    ay##p = ay##p + t2##p * (G2G_SG(p) * G2G_MR(p));                 \
-  } else if (comp == 2) ay##p = ay##p + t1##p;                       \
-  else az##p = az##p + t1##p;
-        G2G_FOR4(G2G_FOLD)
-#undef G2G_FOLD
       } else {
         const int offT = comp == 4 ? 0 : Nz;  // A^T acts on x (xz) or y (yz)
-        for (int j = part; j < Nz; j += split) {
-          const T2 a = cur[j * NZP + i];   // A(i,j)
-          const T2 at = cur[i * NZP + j];  // A(j,i)
-#define G2G_ASY(p)                          \
-  {                                         \
-    const T2* v = vin + (p) * Nz3;          \
-    cmac(t1##p, a, v[2 * Nz + j]);          \
-    cmac(t2##p, at, v[offT + j]);           \
+#pragma unroll
+        for (int jj = 0; jj < nj; jj++) {
+          const int j = part + jj * split;
+          const T2 a = cur[j * NZP + i];     // A(i,j)
+          const T2 atn = -cur[i * NZP + j];  // -A(j,i)
+#define G2G_ASY(p)                                  \
+  {                                                 \
+    const T2* v = vin + (p) * Nz3;                  \
+    if (comp == 4) cmac(ax##p, a, v[2 * Nz + j]);   \
+    else cmac(ay##p, a, v[2 * Nz + j]);             \
+    cmac(az##p, atn, v[offT + j]);                  \
   }
           G2G_FOR4(G2G_ASY)
 #undef G2G_ASY
         }
-#define G2G_FOLD(p)                                   \
-  if (comp == 4) {                                    \
-    ax##p = ax##p + t1##p * G2G_SG(p);                \
-    az##p = az##p - t2##p * G2G_SG(p);                \
-  } else {                                            \
-    ay##p = ay##p + t1##p * G2G_MR(p);                \
-    az##p = az##p - t2##p * G2G_MR(p);                \
-  }
-        G2G_FOR4(G2G_FOLD)
-#undef G2G_FOLD
       }
     }
     if (comp < 5) {
@@ -509,15 +507,13 @@ __global__ __launch_bounds__(NTC) void k_contract2(cx_t<R>* __restrict__ S,
   {                                                                 \
     T2* dst = S + ((int64_t)PX(p) * Ny2 + PY(p)) * Nz3;             \
     if (!VAL(p)) ax##p = ay##p = az##p = zero;                      \
-    dst[i] = ax##p;                                                 \
-    dst[Nz + i] = ay##p;                                            \
+    dst[i] = ((p) & 1) ? -ax##p : ax##p;                            \
+    dst[Nz + i] = ((p) & 2) ? -ay##p : ay##p;                       \
     dst[2 * Nz + i] = az##p;                                        \
   }
     G2G_FOR4(G2G_OUT)
 #undef G2G_OUT
   }
-#undef G2G_SG
-#undef G2G_MR
 #undef PX
 #undef PY
 #undef VAL
@@ -861,13 +857,17 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
       if (op.Nz * op.Nz <= 4 * NT) launch(k_contract<R, 4>);
       else launch(k_contract<R, 16>);
     } else if (op.Nz * op.Nz <= 4 * NT) {
-      launch(k_contract2<R, 4, NT>);
+      switch (op.Nz) {  // compile-time Nz for the common layer counts
+        case 16: launch(k_contract2<R, 4, NT, 16>); break;
+        case 32: launch(k_contract2<R, 4, NT, 32>); break;
+        default: launch(k_contract2<R, 4, NT, 0>); break;
+      }
     } else {
       // Nz > 32: 512 threads (8 lanes per receiver row at Nz = 64) keep twice
       // the waves per CU that the LDS budget (one or two blocks) allows
       int sl2 = 0;
       while ((op.Nz << (sl2 + 1)) <= 2 * NT) sl2++;
-      auto kern = k_contract2<R, 8, 2 * NT>;
+      auto kern = op.Nz == 64 ? k_contract2<R, 8, 2 * NT, 64> : k_contract2<R, 8, 2 * NT, 0>;
       if (sm > 65536) {
         G2G_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     160 * 1024));

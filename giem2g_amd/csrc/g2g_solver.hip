@@ -383,8 +383,14 @@ template <typename R> struct Gmres {
     double bea = -1;
     for (;;) {
       const int j = jH - 1;
-      right_precond(Vc(j), w, j);
-      apply(w, r0);
+      // without a preconditioner (plain inner GMRES) the operator takes the
+      // basis vector itself
+      const T2* zj = Vc(j);
+      if (precond || flex) {
+        right_precond(Vc(j), w, j);
+        zj = w;
+      }
+      apply(zj, r0);
       if (accept_step && !accept_step(Vc(j), r0)) {
         // the variable preconditioner's step is rejected: redo it with the fallback
         fallback(Vc(j), w);
