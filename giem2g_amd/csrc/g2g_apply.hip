@@ -857,7 +857,7 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
       if (op.Nz * op.Nz <= 4 * NT) launch(k_contract<R, 4>);
       else launch(k_contract<R, 16>);
     } else if (op.Nz * op.Nz <= 4 * NT) {
-      switch (op.Nz) {  // compile-time Nz for the common layer counts
+      switch (op.Nz) {  // compile-time Nz for the common layer counts (one to four layers per lane)
         case 16: launch(k_contract2<R, 4, NT, 16>); break;
         case 32: launch(k_contract2<R, 4, NT, 32>); break;
         default: launch(k_contract2<R, 4, NT, 0>); break;
@@ -867,7 +867,9 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
       // the waves per CU that the LDS budget (one or two blocks) allows
       int sl2 = 0;
       while ((op.Nz << (sl2 + 1)) <= 2 * NT) sl2++;
-      auto kern = op.Nz == 64 ? k_contract2<R, 8, 2 * NT, 64> : k_contract2<R, 8, 2 * NT, 0>;
+      // (a compile-time Nz = 64 unrolls eight source layers per lane and
+      // spills registers: measured 40% slower in FP64, so Nz stays a runtime value)
+      auto kern = k_contract2<R, 8, 2 * NT, 0>;
       if (sm > 65536) {
         G2G_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     160 * 1024));
@@ -975,35 +977,56 @@ __global__ __launch_bounds__(NT) void k_rc_contract(const zd* __restrict__ S, co
   const bool ok = pair_ok && lane < Nz;
   const int64_t pt = pair_ok ? q / Nr : 0;
   const int r = pair_ok ? (int)(q - pt * Nr) : 0;
-  const zd zero = mkz(0, 0);
   const zd* s = S + pt * 3 * Nz + lane;
-  const zd jx = ok ? s[0] : zero, jy = ok ? s[Nz] : zero, jz = ok ? s[2 * Nz] : zero;
   const zd* g = G + (pt * 15 * Nr + r) * Nz + lane;  // entry e at g[e * Nr * Nz]
   const int64_t es = (int64_t)Nr * Nz;
-#define GE(e) (ok ? g[(e) * es] : zero)
-  const zd rexx = GE(0), rexy = GE(1), rexz = GE(2), reyy = GE(3), reyz = GE(4), rezx = GE(5),
-           rezy = GE(6), rezz = GE(7), rhxx = GE(8), rhxy = GE(9), rhxz = GE(10), rhyx = GE(11),
-           rhyz = GE(12), rhzx = GE(13), rhzy = GE(14);
-#undef GE
-  // APPLY_RC_E_OP (:420-434) and APPLY_RC_H_OP (:461-473); RHYY = -RHXX (:468)
-  zd o[6];
-  o[0] = rexx * jx + rexy * jy + rexz * jz;
-  o[1] = rexy * jx + reyy * jy + reyz * jz;
-  o[2] = rezx * jx + rezy * jy + rezz * jz;
-  o[3] = rhxx * jx + rhxy * jy + rhxz * jz;
-  o[4] = rhyx * jx - rhxx * jy + rhyz * jz;
-  o[5] = rhzx * jx + rhzy * jy;
+  // predicated loads into zero-initialised registers (a select between a
+  // global element and a local constant would put the constant in scratch)
+  zd jx = mkz(0, 0), jy = jx, jz = jx, rexx = jx, rexy = jx, rexz = jx, reyy = jx, reyz = jx,
+     rezx = jx, rezy = jx, rezz = jx, rhxx = jx, rhxy = jx, rhxz = jx, rhyx = jx, rhyz = jx,
+     rhzx = jx, rhzy = jx;
+  if (ok) {
+    jx = s[0];
+    jy = s[Nz];
+    jz = s[2 * Nz];
+    rexx = g[0];
+    rexy = g[es];
+    rexz = g[2 * es];
+    reyy = g[3 * es];
+    reyz = g[4 * es];
+    rezx = g[5 * es];
+    rezy = g[6 * es];
+    rezz = g[7 * es];
+    rhxx = g[8 * es];
+    rhxy = g[9 * es];
+    rhxz = g[10 * es];
+    rhyx = g[11 * es];
+    rhyz = g[12 * es];
+    rhzx = g[13 * es];
+    rhzy = g[14 * es];
+  }
+  // APPLY_RC_E_OP (:420-434) and APPLY_RC_H_OP (:461-473); RHYY = -RHXX (:468).
+  // Named registers, not an array, so that nothing is placed in scratch.
+  zd o0 = rexx * jx + rexy * jy + rexz * jz;
+  zd o1 = rexy * jx + reyy * jy + reyz * jz;
+  zd o2 = rezx * jx + rezy * jy + rezz * jz;
+  zd o3 = rhxx * jx + rhxy * jy + rhxz * jz;
+  zd o4 = rhyx * jx - rhxx * jy + rhyz * jz;
+  zd o5 = rhzx * jx + rhzy * jy;
 #pragma unroll
-  for (int c = 0; c < 6; c++)
-#pragma unroll
-    for (int off = P / 2; off > 0; off >>= 1) {
-      o[c].x += __shfl_xor(o[c].x, off);
-      o[c].y += __shfl_xor(o[c].y, off);
-    }
+  for (int off = P / 2; off > 0; off >>= 1) {
+#define G2G_XR(v) v.x += __shfl_xor(v.x, off); v.y += __shfl_xor(v.y, off);
+    G2G_XR(o0) G2G_XR(o1) G2G_XR(o2) G2G_XR(o3) G2G_XR(o4) G2G_XR(o5)
+#undef G2G_XR
+  }
   if (pair_ok && lane == 0) {
     zd* d = So + pt * 6 * Nr + r;
-#pragma unroll
-    for (int c = 0; c < 6; c++) d[c * Nr] = o[c];
+    d[0] = o0;
+    d[Nr] = o1;
+    d[2 * Nr] = o2;
+    d[3 * Nr] = o3;
+    d[4 * Nr] = o4;
+    d[5 * Nr] = o5;
   }
 }
 

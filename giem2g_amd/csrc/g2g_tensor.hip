@@ -384,8 +384,8 @@ __global__ __launch_bounds__(64) void k_lambda(zd* __restrict__ tabs,
   zd* tb = ta + P;
   zd* tc = tb + P;
   zd* td = tc + P;
+  __shared__ zd el[65];  // thread 0 only (a private array would live in scratch)
   if (threadIdx.x == 0) {
-    zd el[65];
     layer_apq(lm, Nl, sh_cs, sh_k2, sh_td, eta, el, p, q, Arr);
     layer_pqexpz(Nz, Nl, lnumg, zg, dzg, sh_td, eta, p, q, expz, pez, qez);
   }
@@ -918,8 +918,8 @@ __global__ __launch_bounds__(64) void k_rc_lambda(zd* __restrict__ tabs,
   __syncthreads();
   const int K = blockIdx.x;
   const double lm = lms[K];
+  __shared__ zd el[65];  // thread 0 only (a private array would live in scratch)
   if (threadIdx.x == 0) {
-    zd el[65];
     layer_apq(lm, Nl, sh_cs, sh_k2, sh_td, eta, el, p, q, Arr);
     // the receiver row A(rl, l) of the layer-coupling matrix (apq_module.F90:250-264)
     for (int i = 1; i <= Nl - 1; i++) {
