@@ -69,10 +69,17 @@ def cpu_baseline(m, Gspec, v, budget_s=12.0):
         out = om.apply(Gspec, m.csiga(), v, nthreads)
     dt = (time.perf_counter() - t0) / reps
     split = om.apply_timings()
+    t0 = time.perf_counter()
+    om.apply(Gspec, m.csiga(), v, 1)
+    dt1 = time.perf_counter() - t0
+    split1 = om.apply_timings()
     return {"value": 1.0 / dt, "unit": "matvec/s", "cores": nthreads, "kind": "port",
             "sample": "%d FP64 matvecs of the 128x128x32 operator (oracle/giem2g_oracle.c, "
                       "OpenMP, radix-2 FFT), split fwd/contraction/bwd = %.3f/%.3f/%.3f s"
-                      % (reps, split[0], split[1], split[2])}, out
+                      % (reps, split[0], split[1], split[2]),
+            "value_1thread": 1.0 / dt1,
+            "sample_1thread": "1 matvec on one thread, split fwd/contraction/bwd = %.3f/%.3f/%.3f s"
+                              % (split1[0], split1[1], split1[2])}, out
 
 
 # learned preconditioner in the benchmark: mode 2 (network step, fallback to
