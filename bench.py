@@ -262,8 +262,8 @@ def main():
                     help="also run a plain 1 GiB device copy (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the operator would shard)", file=sys.stderr)
+        print("bench.py runs on one GPU only: there is no multi-GPU path (the operator "
+              "would shard by kx slabs with one all-to-all per FFT direction)", file=sys.stderr)
         return 2
     import torch
     from giem2g_amd import models, operator as gop
