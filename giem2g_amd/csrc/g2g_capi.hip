@@ -41,6 +41,7 @@ template <typename F> int guard(F&& f) {
 
 struct Handle {
   Operator op;
+  int receivers = 0;  // receiver operators created on this operator (they hold a pointer to it)
   std::vector<double> hist;
   PrecondHook learned;
   UNet* net = nullptr;  // attached learned preconditioner (not owned)
@@ -69,6 +70,7 @@ Handle* H(void* p) {
 
 struct RcHandle {
   RcOperator rc;
+  Handle* owner = nullptr;
 };
 
 RcOperator& RC(void* p) {
@@ -432,6 +434,7 @@ int giem2g_op_create(int Nx, int Ny, int Nz, double dx, double dy, const double*
 int giem2g_op_destroy(void* p) {
   return guard([&] {
     Handle* h = H(p);
+    G2G_CHECK(h->receivers == 0, -4, "receiver operators still use this operator (giem2g_rc_destroy them first)");
     learned_free(h->lctx);
     op_fini(h->op);
     delete h;
@@ -907,12 +910,15 @@ int giem2g_rc_create(void* ie_op, int Nr, const double* xshift, const double* ys
   return guard([&] {
     G2G_CHECK(out != nullptr && xshift && yshift && depth, -2, "null argument");
     auto h = std::make_unique<RcHandle>();
+    Handle* owner = H(ie_op);
     try {
-      rc_init(h->rc, H(ie_op)->op, Nr, xshift, yshift, depth);
+      rc_init(h->rc, owner->op, Nr, xshift, yshift, depth);
     } catch (...) {
       rc_fini(h->rc);
       throw;
     }
+    h->owner = owner;
+    owner->receivers++;
     *out = h.release();
   });
 }
@@ -922,6 +928,7 @@ int giem2g_rc_destroy(void* p) {
     if (!p) return;
     RcHandle* h = static_cast<RcHandle*>(p);
     rc_fini(h->rc);
+    if (h->owner) h->owner->receivers--;
     delete h;
   });
 }

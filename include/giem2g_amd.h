@@ -119,7 +119,7 @@ int giem2g_set_component_order(void *ie_op, int order);
 /* Status-returning creation without the reference structs. */
 int giem2g_op_create(int Nx, int Ny, int Nz, double dx, double dy,
                      const double *z /* Nz+1 */, void **ie_op);
-int giem2g_op_destroy(void *ie_op);
+int giem2g_op_destroy(void *ie_op); /* fails while receiver operators use it */
 /* csigb: 2*(Nl+1) doubles (complex, [0] = air); thick: Nl-1 doubles. */
 int giem2g_op_set_background(void *ie_op, int Nl, const double *csigb,
                              const double *thick, double omega);

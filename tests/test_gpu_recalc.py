@@ -187,5 +187,7 @@ def test_receiver_errors():
     op.set_background(m.csigb(2.0), m.thick, 2 * np.pi * 2.0)  # a new frequency
     with pytest.raises(L.G2GError, match="current frequency"):
         rc.recalculate(np.zeros(3 * m.ncell, np.complex128))
+    with pytest.raises(L.G2GError, match="receiver operators"):
+        op.close()  # the receiver operator still holds it
     rc.close()
     op.close()
