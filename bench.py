@@ -311,7 +311,8 @@ def main():
                    "grid": [m.Nx, m.Ny, m.Nz], "precision": "complex128"},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_contract2<double,4,256,32> (tensor contraction, two components in flight)",
+                     "kernel": "k_contract3<double,32,1,1,4> (tensor contraction on the FP64 matrix cores, "
+                               "source layers split over four waves)",
                      "algorithmic_bytes_per_launch": bytes64["contraction"],
                      "avg_launch_ms": float(stage_ms[1])},
         "kernels_ms": {"fwd_fft": float(stage_ms[0]), "contraction": float(stage_ms[1]),

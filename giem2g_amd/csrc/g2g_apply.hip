@@ -19,6 +19,7 @@
 #include "g2g_op.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 namespace g2g {
 
@@ -529,21 +530,18 @@ __global__ __launch_bounds__(NTC) void k_contract2(cx_t<R>* __restrict__ S,
 }
 
 // ------------------------------------------------------ K3 on MFMA ----
-// Same product as k_contract, as matrix-core tiles.  With the point signs
-// s (for -kx) and m (for -ky), every point uses the same 3Nz x 3Nz block
-// G0 = [[Sxx, Sxy, Axz], [Sxy, Syy, Ayz], [-Axz^T, -Ayz^T, Szz]]:
+// The same product as k_contract2, as matrix-core tiles.  With the point
+// signs s (for -kx) and m (for -ky), every point uses the same 3Nz x 3Nz
+// block G0 = [[Sxx, Sxy, Axz], [Sxy, Syy, Ayz], [-Axz^T, -Ayz^T, Szz]]:
 //   out_p = D_p G0 (D_p v_p),  D_p = diag(s, m, 1) per component
 // (the sign pattern of VERTICAL_MULT_GENERAL, apply_ie_operator_module.F90:
 // 276-292).  The four points form the columns of a real 16-column operand
 // [Re v | Im v] (Gr pass) and [-Im v | Re v] (Gi pass) so that columns 0-3
 // accumulate Re(G0 v_p) and columns 4-7 Im(G0 v_p) in one
 // v_mfma_{f64,f32}_16x16x4 tile; columns 8-15 carry a second right-hand
-// side (NRHS = 2: both polarizations read the tensor once) or repeat the
-// first.  Lane l supplies A[row l%16][k l/16] and B[k l/16][col l%16] and
-// receives four rows of column l%16 (mma_row).  Wave w owns TPW consecutive 16-row
-// tiles of the 3Nz output rows.  The six tensor components stream through
-// two LDS buffers (symmetric ones unpacked to full Nz x Nz), with two
-// components in flight in registers while the matrix cores work on a third.
+// side (NRHS = 2: two vectors read the tensor once) or repeat the first.
+// Lane l supplies A[row l%16][k l/16] and B[k l/16][col l%16] and receives
+// four rows of column l%16 (mma_row).
 template <typename R> struct v4t;
 template <> struct v4t<double> { typedef double t __attribute__((ext_vector_type(4))); };
 template <> struct v4t<float> { typedef float t __attribute__((ext_vector_type(4))); };
@@ -560,193 +558,259 @@ template <typename R> __device__ inline int mma_row(int kq, int q);
 template <> __device__ inline int mma_row<float>(int kq, int q) { return 4 * kq + q; }
 template <> __device__ inline int mma_row<double>(int kq, int q) { return kq + 4 * q; }
 
-template <typename R, int NW, int PF, int NRHS>
-__global__ __launch_bounds__(64 * NW) void k_contract_mma(cx_t<R>* __restrict__ S0, cx_t<R>* __restrict__ S1,
-                                                         const cx_t<R>* __restrict__ G, Geo g,
-                                                         int64_t BS) {
+// The work of every tensor component is spread over all NW waves: wave w
+// takes the source layers [w Nz/NW, (w+1) Nz/NW) of every output tile, so
+// each component keeps every wave's matrix cores busy.  The point operand is
+// loaded once into registers (three input components x Nz/(4 NW) steps per lane);
+// the tensor operand is read from LDS in its stored layout: packed symmetric
+// components addressed directly (S(i,j) = S(j,i)), full components with column
+// stride Nz+1.  Each wave accumulates all 3Nz/16 output tiles over its layers;
+// the partial tiles are summed through LDS by a fixed halving tree (NW = 4:
+// (w0 + w2) + (w1 + w3)), so results do not depend on scheduling.  DEEP = 2
+// keeps two components in flight in registers, DEEP = 1 one (FP64: fewer
+// registers, more waves per SIMD).
+template <typename R, int NZ, int NRHS, int NW>
+struct C3Cfg {
+  static constexpr int NTH = 64 * NW, NZ3 = 3 * NZ, NT16 = NZ / 16, KW = NZ / NW, KS = KW / 4;
+  static constexpr int P = NZ * (NZ + 1) / 2, FULL = NZ * NZ, SF = NZ + 1, GBS = NZ * SF;
+  static constexpr int PF = (FULL + NTH - 1) / NTH;
+  static constexpr int SL = 3 * NT16 * 256;  // reals per partial-tile slot
+  static constexpr size_t vin_bytes = sizeof(cx_t<R>) * NRHS * 4 * NZ3;
+  static constexpr size_t stage_bytes = sizeof(cx_t<R>) * 2 * GBS;
+  static constexpr size_t red_bytes = sizeof(R) * (NW / 2) * SL;
+  static constexpr size_t smem = vin_bytes + (stage_bytes > red_bytes ? stage_bytes : red_bytes);
+};
+
+template <typename R, int NZ, int NRHS, int DEEP, int NW>
+__global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S0, cx_t<R>* __restrict__ S1,
+                                                   const cx_t<R>* __restrict__ G, Geo g, int64_t BS) {
   using T2 = cx_t<R>;
   using V4 = typename v4t<R>::t;
-  constexpr int NTH = 64 * NW;
-  static_assert(PF <= 16, "PF must be <= 16");
+  using C = C3Cfg<R, NZ, NRHS, NW>;
+  constexpr int NTH = C::NTH, NZ3 = C::NZ3, NT16 = C::NT16, KW = C::KW, KS = C::KS;
+  constexpr int P = C::P, FULL = C::FULL, SF = C::SF, GBS = C::GBS, PF = C::PF, SL = C::SL;
+  static_assert(NZ % 16 == 0 && PF <= 16, "Nz must be a multiple of 16, at most 64");
+  static_assert((NW == 4 || NW == 8) && KW % 4 == 0, "four or eight waves, whole MFMA steps each");
   static_assert(NRHS == 1 || NRHS == 2, "one or two right-hand sides");
+  static_assert(DEEP == 1 || DEEP == 2, "one or two components in flight");
   extern __shared__ unsigned char smem_raw[];
-  const int Nz = g.Nz, Nz3 = g.Nz3, NZP = Nz + 1;
-  const int P = Nz * (Nz + 1) / 2;
-  const int GB = Nz * NZP;
-  T2* vin = reinterpret_cast<T2*>(smem_raw);  // [NRHS][4][3Nz]
-  T2* gbuf = vin + NRHS * 4 * Nz3;            // [2][Nz x NZP], element (i,j) at j*NZP + i
+  T2* vin = reinterpret_cast<T2*>(smem_raw);  // [NRHS][4][3Nz], point-signed inputs
+  T2* gbuf = vin + NRHS * 4 * NZ3;            // [2][GBS]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int b = blockIdx.x, kx = b % g.Nx, ky = b / g.Nx;
-  const int Nx2 = 2 * g.Nx, Ny2 = 2 * g.Ny;
-  const int gNx = g.Nx, gNy = g.Ny;
+  const int Nx2 = 2 * g.Nx, Ny2 = 2 * g.Ny, gNx = g.Nx, gNy = g.Ny;
 #define PX(p) (((p) & 1) ? (kx == 0 ? gNx : Nx2 - kx) : kx)
 #define PY(p) (((p) & 2) ? (ky == 0 ? gNy : Ny2 - ky) : ky)
 #define VAL(p) ((!((p) & 1) || kx > 0) && (!((p) & 2) || ky > 0))
   const T2* blk = G + (int64_t)b * BS;
   const T2 zero = mk<T2>(0, 0);
-#define G2G_RDECL(q) T2 ra##q = zero, rb##q = zero;
-  G2G_FOR16(G2G_RDECL)
-#undef G2G_RDECL
-#define G2G_CLEN(c) ((c) < 4 ? P : Nz * Nz)
-#define G2G_CSRC(c) ((c) < 4 ? blk + (c) * P : blk + 4 * P + ((c) - 4) * Nz * Nz)
-  // unconditional loads (index clamped into the component): exact load counts
-#define G2G_LDA(q) if ((q) < PF) ra##q = src_[min(tid + (q) * NTH, n_ - 1)];
-#define G2G_LDB(q) if ((q) < PF) rb##q = src_[min(tid + (q) * NTH, n_ - 1)];
-#define G2G_PREF(c, SET)            \
-  {                                 \
-    const T2* src_ = G2G_CSRC(c);   \
-    const int n_ = G2G_CLEN(c);     \
-    G2G_FOR16(G2G_LD##SET)          \
+  // two register sets F and G (named registers: arrays of FP64 pairs were
+  // placed in scratch memory by the compiler)
+#define C3_DECL(q) T2 rf##q = zero, rg##q = zero;
+  G2G_FOR16(C3_DECL)
+#undef C3_DECL
+  // loads are unconditional (index clamped into the component) so that the
+  // compiler counts outstanding loads exactly and keeps later sets in flight
+#define C3_LDF(q) if ((q) < PF) rf##q = src_[min(tid + (q) * NTH, n_ - 1)];
+#define C3_LDG(q) if ((q) < PF) rg##q = src_[min(tid + (q) * NTH, n_ - 1)];
+#define C3_PREFETCH(c, SET)                                                           \
+  {                                                                                   \
+    const T2* src_ = (c) < 4 ? blk + (c) * P : blk + 4 * P + ((c) - 4) * FULL;        \
+    const int n_ = (c) < 4 ? P : FULL;                                                \
+    G2G_FOR16(C3_LD##SET)                                                             \
   }
-  // stash with the packed-symmetric components unpacked to full matrices
-#define G2G_STX(q, r)                                                   \
-  if ((q) < PF && tid + (q) * NTH < n_) {                               \
-    const int e = tid + (q) * NTH;                                      \
-    if (c_ < 4) {                                                       \
-      int j = (int)((sqrtf(8.0f * e + 1.0f) - 1.0f) * 0.5f);            \
-      if ((j + 1) * (j + 2) / 2 <= e) j++;                              \
-      if (j * (j + 1) / 2 > e) j--;                                     \
-      const int i = e - j * (j + 1) / 2;                                \
-      dst_[j * NZP + i] = r##q;                                         \
-      dst_[i * NZP + j] = r##q;                                         \
-    } else {                                                            \
-      dst_[(e / Nz) * NZP + e % Nz] = r##q;                             \
-    }                                                                   \
+  // packed symmetric components as stored, full ones with column stride SF
+#define C3_STX(q, r)                                                 \
+  if ((q) < PF && tid + (q) * NTH < n_) {                            \
+    const int e = tid + (q) * NTH;                                   \
+    dst_[c_ < 4 ? e : (e / NZ) * SF + e % NZ] = r##q;                \
   }
-#define G2G_STA(q) G2G_STX(q, ra)
-#define G2G_STB(q) G2G_STX(q, rb)
-#define G2G_STASH(c, SET, dst) \
-  {                            \
-    const int c_ = (c);        \
-    T2* dst_ = (dst);          \
-    const int n_ = G2G_CLEN(c_); \
-    G2G_FOR16(G2G_ST##SET)     \
+#define C3_STF(q) C3_STX(q, rf)
+#define C3_STG(q) C3_STX(q, rg)
+#define C3_STASH(c, SET, dst)          \
+  {                                    \
+    const int c_ = (c);                \
+    T2* dst_ = (dst);                  \
+    const int n_ = c_ < 4 ? P : FULL;  \
+    G2G_FOR16(C3_ST##SET)              \
   }
-  G2G_PREF(0, A)
-  G2G_PREF(1, B)
-  for (int i = tid; i < NRHS * 4 * Nz3; i += NTH) {
-    const int rp = i / Nz3, e = i - rp * Nz3, p = rp & 3;
+  C3_PREFETCH(0, F)
+  if (DEEP == 2) C3_PREFETCH(1, G)
+  for (int i = tid; i < NRHS * 4 * NZ3; i += NTH) {
+    const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3;
     const T2* Sr = rp < 4 ? S0 : S1;
     T2 vv = zero;
-    if (VAL(p)) vv = Sr[((int64_t)PX(p) * Ny2 + PY(p)) * Nz3 + e];
+    if (VAL(p)) vv = Sr[((int64_t)PX(p) * Ny2 + PY(p)) * NZ3 + e];
+    // D_p: x times s (-1 for -kx), y times m (-1 for -ky)
+    if ((e < NZ && (p & 1)) || (e >= NZ && e < 2 * NZ && (p & 2))) vv = -vv;
     vin[i] = vv;
   }
-  G2G_STASH(0, A, gbuf)
+  C3_STASH(0, F, gbuf)
   __syncthreads();
-  // per-lane operand roles
-  const int col = lane & 15, kq = lane >> 4, rr = lane & 15;
-  // column = (rhs, re/im, point): cols 4*part + pnt, part = 2*rhs + (0 re | 1 im);
-  // with one right-hand side columns 8-15 repeat columns 0-7 and are not stored
-  const int pnt = col & 3, part = col >> 2, rhs = (part >> 1) & (NRHS - 1);
-  const R sgx = (pnt & 1) ? R(-1) : R(1), sgy = (pnt & 2) ? R(-1) : R(1);
-  const int ntile = Nz3 / 16;
-  const int tpw = (ntile + NW - 1) / NW;
-  V4 acc0 = {0, 0, 0, 0}, acc1 = acc0, acc2 = acc0;
-  // one component's MFMA work for the wave's tiles
-#define G2G_TILE(k)                                                                    \
-  if ((k) < tpw) {                                                                     \
-    const int t = wave * tpw + (k);                                                    \
-    if (t < ntile) {                                                                   \
-      const int co = (16 * t) / Nz, i0 = (16 * t) % Nz;                                \
-      int ci = -1;                                                                     \
-      bool tr = false;                                                                 \
-      if (comp == 0 && co == 0) ci = 0;                                                \
-      if (comp == 1 && co == 0) ci = 1;                                                \
-      if (comp == 1 && co == 1) ci = 0;                                                \
-      if (comp == 2 && co == 1) ci = 1;                                                \
-      if (comp == 3 && co == 2) ci = 2;                                                \
-      if (comp == 4 && co == 0) ci = 2;                                                \
-      if (comp == 4 && co == 2) { ci = 0; tr = true; }                                 \
-      if (comp == 5 && co == 1) ci = 2;                                                \
-      if (comp == 5 && co == 2) { ci = 1; tr = true; }                                 \
-      if (ci >= 0) {                                                                   \
-        const R sv = ci == 0 ? sgx : (ci == 1 ? sgy : R(1));                           \
-        const R sa = tr ? R(-1) : R(1);                                                \
-        const T2* vp = vin + (rhs * 4 + pnt) * Nz3 + ci * Nz;                         \
-        const int i = i0 + rr;                                                         \
-        for (int j0 = 0; j0 < Nz; j0 += 4) {                                           \
-          const int j = j0 + kq;                                                       \
-          const T2 a = tr ? cur[i * NZP + j] : cur[j * NZP + i];                       \
-          const T2 v = vp[j];                                                          \
-          const R vr = v.x * sv, vi = v.y * sv;                                        \
-          R br = vr, bi = -vi;                                                         \
-          if (part & 1) { br = vi; bi = vr; }                                          \
-          acc##k = mma4(a.x * sa, br, acc##k);                                         \
-          acc##k = mma4(a.y * sa, bi, acc##k);                                         \
-        }                                                                              \
-      }                                                                                \
-    }                                                                                  \
+  // operand roles: lane l supplies A[row l%16][k l/16] and B[k l/16][col l%16];
+  // column = 4*part + point, part = 2*rhs + (0 re | 1 im)
+  const int rr = lane & 15, kq = lane >> 4;
+  const int pnt = rr & 3, part = rr >> 2, rhs = (part >> 1) & (NRHS - 1), reim = part & 1;
+  const int jw = wave * KW;
+  R bR[3][KS], bI[3][KS];
+#pragma unroll
+  for (int ci = 0; ci < 3; ci++)
+#pragma unroll
+    for (int s = 0; s < KS; s++) {
+      const T2 v = vin[(rhs * 4 + pnt) * NZ3 + ci * NZ + jw + 4 * s + kq];
+      bR[ci][s] = reim ? v.y : v.x;   // Gr pass: [Re v | Im v]
+      bI[ci][s] = reim ? v.x : -v.y;  // Gi pass: [-Im v | Re v]
+    }
+  V4 acc[3][NT16];
+#pragma unroll
+  for (int co = 0; co < 3; co++)
+#pragma unroll
+    for (int t = 0; t < NT16; t++) acc[co][t] = V4{0, 0, 0, 0};
+  // one tensor component per step; the component is a compile-time value so
+  // that accumulator and register-set indices stay static
+  auto step = [&](auto CC) {
+    constexpr int c = decltype(CC)::value;
+    if constexpr (DEEP == 2) {
+      if constexpr (c + 2 < 6) {
+        if constexpr ((c & 1) != 0) C3_PREFETCH(c + 2, G)
+        else C3_PREFETCH(c + 2, F)
+      }
+    } else if constexpr (c + 1 < 6) {
+      C3_PREFETCH(c + 1, F)
+    }
+    const T2* cur = gbuf + (c & 1) * GBS;
+#pragma unroll
+    for (int s = 0; s < KS; s++) {
+      const int j = jw + 4 * s + kq;
+#pragma unroll
+      for (int t = 0; t < NT16; t++) {
+        const int r = 16 * t + rr;
+        if (c < 4) {
+          // symmetric, packed by rows of the lower triangle
+          const T2 a = cur[j <= r ? j + r * (r + 1) / 2 : r + j * (j + 1) / 2];
+          // xx: x <- x; xy: x <- y (and y <- x below); yy: y <- y; zz: z <- z
+          const int co = c <= 1 ? 0 : (c == 2 ? 1 : 2);
+          const int ci = c == 0 ? 0 : (c == 3 ? 2 : 1);
+          acc[co][t] = mma4(a.x, bR[ci][s], acc[co][t]);
+          acc[co][t] = mma4(a.y, bI[ci][s], acc[co][t]);
+          if (c == 1) {  // S_xy also maps x to y
+            acc[1][t] = mma4(a.x, bR[0][s], acc[1][t]);
+            acc[1][t] = mma4(a.y, bI[0][s], acc[1][t]);
+          }
+        } else {
+          // A (xz or yz) at column j, row r; its use in z is -A^T
+          const int co = c - 4;
+          const T2 a = cur[j * SF + r];
+          const T2 at = cur[r * SF + j];
+          acc[co][t] = mma4(a.x, bR[2][s], acc[co][t]);
+          acc[co][t] = mma4(a.y, bI[2][s], acc[co][t]);
+          acc[2][t] = mma4(-at.x, bR[co][s], acc[2][t]);
+          acc[2][t] = mma4(-at.y, bI[co][s], acc[2][t]);
+        }
+      }
+    }
+    if constexpr (c < 5) {
+      if constexpr (DEEP == 2 && (c & 1) == 0) C3_STASH(c + 1, G, gbuf + ((c + 1) & 1) * GBS)
+      else C3_STASH(c + 1, F, gbuf + ((c + 1) & 1) * GBS)
+    }
+    __syncthreads();
+  };
+  step(std::integral_constant<int, 0>{});
+  step(std::integral_constant<int, 1>{});
+  step(std::integral_constant<int, 2>{});
+  step(std::integral_constant<int, 3>{});
+  step(std::integral_constant<int, 4>{});
+  step(std::integral_constant<int, 5>{});
+  // partial tiles: slot [tile][q][lane]; halving tree, wave w adds wave w + h
+  R* red = reinterpret_cast<R*>(gbuf);
+  auto put = [&](int slot) {
+#pragma unroll
+    for (int co = 0; co < 3; co++)
+#pragma unroll
+      for (int t = 0; t < NT16; t++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) red[slot * SL + ((co * NT16 + t) * 4 + q) * 64 + lane] = acc[co][t][q];
+  };
+  auto add = [&](int slot) {
+#pragma unroll
+    for (int co = 0; co < 3; co++)
+#pragma unroll
+      for (int t = 0; t < NT16; t++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[co][t][q] += red[slot * SL + ((co * NT16 + t) * 4 + q) * 64 + lane];
+  };
+#pragma unroll
+  for (int h = NW / 2; h >= 1; h /= 2) {
+    if (wave >= h && wave < 2 * h) put(wave - h);
+    __syncthreads();
+    if (wave < h) add(wave);
+    if (h > 1) __syncthreads();
   }
-#define G2G_COMPUTE(c, buf)   \
-  {                           \
-    const int comp = (c);     \
-    const T2* cur = (buf);    \
-    G2G_TILE(0) G2G_TILE(1) G2G_TILE(2) \
+  if (wave == 0) {
+    // results with the output signs -> LDS (over vin), then coalesced stores
+    R* vo = reinterpret_cast<R*>(vin);
+    const R sgx = (pnt & 1) ? R(-1) : R(1), sgy = (pnt & 2) ? R(-1) : R(1);
+    if ((part >> 1) < NRHS) {
+#pragma unroll
+      for (int co = 0; co < 3; co++) {
+        const R so = co == 0 ? sgx : (co == 1 ? sgy : R(1));
+#pragma unroll
+        for (int t = 0; t < NT16; t++)
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            const int row = co * NZ + 16 * t + mma_row<R>(kq, q);
+            vo[2 * ((rhs * 4 + pnt) * NZ3 + row) + reim] = acc[co][t][q] * so;
+          }
+      }
+    }
   }
-  T2* buf0 = gbuf;
-  T2* buf1 = gbuf + GB;
-  G2G_PREF(2, A)
-  G2G_COMPUTE(0, buf0)
-  G2G_STASH(1, B, buf1)
   __syncthreads();
-  G2G_PREF(3, B)
-  G2G_COMPUTE(1, buf1)
-  G2G_STASH(2, A, buf0)
-  __syncthreads();
-  G2G_PREF(4, A)
-  G2G_COMPUTE(2, buf0)
-  G2G_STASH(3, B, buf1)
-  __syncthreads();
-  G2G_PREF(5, B)
-  G2G_COMPUTE(3, buf1)
-  G2G_STASH(4, A, buf0)
-  __syncthreads();
-  G2G_COMPUTE(4, buf0)
-  G2G_STASH(5, B, buf1)
-  __syncthreads();
-  G2G_COMPUTE(5, buf1)
-  __syncthreads();
-  // results -> LDS (reusing vin) with the output signs, then coalesced stores
-  R* vo = reinterpret_cast<R*>(vin);
-#define G2G_OUTT(k)                                                         \
-  if ((k) < tpw && (part >> 1) < NRHS) {                                    \
-    const int t = wave * tpw + (k);                                         \
-    if (t < ntile) {                                                        \
-      const int co = (16 * t) / Nz, i0 = (16 * t) % Nz;                     \
-      const R so = co == 0 ? sgx : (co == 1 ? sgy : R(1));                  \
-      for (int q = 0; q < 4; q++) {                                         \
-        const int row = co * Nz + i0 + mma_row<R>(kq, q);                   \
-        vo[2 * (((part >> 1) * 4 + pnt) * Nz3 + row) + (part & 1)] = acc##k[q] * so; \
-      }                                                                     \
-    }                                                                       \
-  }
-  G2G_OUTT(0) G2G_OUTT(1) G2G_OUTT(2)
-  __syncthreads();
-  for (int i = tid; i < NRHS * 4 * Nz3; i += NTH) {
-    const int rp = i / Nz3, e = i - rp * Nz3, p = rp & 3;
+  for (int i = tid; i < NRHS * 4 * NZ3; i += NTH) {
+    const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3;
     T2* Sr = rp < 4 ? S0 : S1;
     T2 o = vin[i];
     if (!VAL(p)) o = zero;
-    Sr[((int64_t)PX(p) * Ny2 + PY(p)) * Nz3 + e] = o;
+    Sr[((int64_t)PX(p) * Ny2 + PY(p)) * NZ3 + e] = o;
   }
-#undef G2G_OUTT
-#undef G2G_TILE
-#undef G2G_COMPUTE
-#undef G2G_LDA
-#undef G2G_LDB
-#undef G2G_PREF
-#undef G2G_STX
-#undef G2G_STA
-#undef G2G_STB
-#undef G2G_STASH
-#undef G2G_CLEN
-#undef G2G_CSRC
+#undef C3_LDF
+#undef C3_LDG
+#undef C3_PREFETCH
+#undef C3_STX
+#undef C3_STF
+#undef C3_STG
+#undef C3_STASH
 #undef PX
 #undef PY
 #undef VAL
 }
 
 // ------------------------------------------------------------ dispatch ---
+// The layer-split matrix-core contraction (Nz = 16, 32, 48 or 64) of one or
+// two spectra (S1 is ignored for one); false for other Nz.
+template <typename R, int NRHS>
+static bool launch_contract3(const Operator& op, const cx_t<R>* G, cx_t<R>* S0, cx_t<R>* S1, const Geo& g,
+                             hipStream_t s) {
+  auto go = [&](auto kern, size_t sm, int nw) {
+    G2G_CHECK(sm <= 160 * 1024, -3, "matrix-core contraction does not fit in LDS for this Nz");
+    if (sm > 65536)
+      G2G_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    hipLaunchKernelGGL(kern, dim3(op.Nx * op.Ny), dim3(64 * nw), sm, s, S0, S1, G, g, op.BS);
+  };
+  // measured choices: FP64 keeps one component in flight (Nz = 32: four
+  // waves per SIMD instead of three, 1-3% faster) and at Nz = 64 uses eight
+  // waves (9-11% faster); complex64 at Nz = 64 stays at four waves (eight
+  // were 12-18% slower)
+  constexpr bool F64 = sizeof(R) == 8;
+  constexpr int DP = F64 ? 1 : 2, W64 = F64 ? 8 : 4;
+  switch (op.Nz) {
+    case 16: go(k_contract3<R, 16, NRHS, 2, 4>, C3Cfg<R, 16, NRHS, 4>::smem, 4); break;
+    case 32: go(k_contract3<R, 32, NRHS, DP, 4>, C3Cfg<R, 32, NRHS, 4>::smem, 4); break;
+    case 48: go(k_contract3<R, 48, NRHS, 2, 4>, C3Cfg<R, 48, NRHS, 4>::smem, 4); break;
+    case 64: go(k_contract3<R, 64, NRHS, DP, W64>, C3Cfg<R, 64, NRHS, W64>::smem, W64); break;
+    default: return false;
+  }
+  return true;
+}
 template <typename R, template <typename, int> class KF> struct Dummy {};
 
 static Geo make_geo(const Operator& op, int nrx) {
@@ -827,7 +891,7 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
   });
   if (timed) G2G_HIP(hipEventRecord(op.ev[1], s));
   {
-    // rows = Nz receiver layers, split lanes per row (power of two)
+    // vector ALU: rows = Nz layers, split lanes per row (power of two)
     int split_log = 0;
     while ((op.Nz << (split_log + 1)) <= NT) split_log++;
     const size_t sm = sizeof(cx_t<R>) * (4 * (size_t)Nz3 + 2 * (size_t)op.Nz * (op.Nz + 1));
@@ -838,21 +902,9 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
       }
       hipLaunchKernelGGL(kern, dim3(op.Nx * op.Ny), dim3(NT), sm, s, b.S, b.G, g, op.BS, split_log);
     };
-    // single right-hand side: the vector-ALU kernel unless the matrix cores
-    // are requested (measured: equal or slower for one vector, as the
-    // tensor stream, not the arithmetic, sets the pace)
-    const bool mma = op.contraction == 2 && op.Nz % 16 == 0;
-    if (mma) {
-      auto launch_mma = [&](auto kern, int nth) {
-        if (sm > 65536) {
-          G2G_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      160 * 1024));
-        }
-        hipLaunchKernelGGL(kern, dim3(op.Nx * op.Ny), dim3(nth), sm, s, b.S, b.S, b.G, g, op.BS);
-      };
-      if (op.Nz <= 32) launch_mma(k_contract_mma<R, 3, 6, 1>, 192);
-      else if (op.Nz == 48) launch_mma(k_contract_mma<R, 3, 16, 1>, 192);
-      else launch_mma(k_contract_mma<R, 6, 11, 1>, 384);
+    // matrix cores for Nz = 16, 32, 48 and 64 unless a vector-ALU kernel is
+    // requested (FP64 at 128x128x32: 0.28-0.30 ms against 0.33 ms)
+    if ((op.contraction == 0 || op.contraction == 2) && launch_contract3<R, 1>(op, b.G, b.S, b.S, g, s)) {
     } else if (op.contraction == 3) {
       if (op.Nz * op.Nz <= 4 * NT) launch(k_contract<R, 4>);
       else launch(k_contract<R, 16>);
@@ -895,12 +947,16 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
 // Two right-hand sides through one tensor read: both vectors are
 // transformed into their own spectrum buffers, one two-column-set MFMA
 // contraction serves both, then both are transformed back.  Needs
-// Nz % 16 == 0 (matrix-core contraction).
+// Nz = 16, 32, 48 or 64 (matrix-core contraction).
+bool op_pair_supported(const Operator& op) {
+  return op.Nz == 16 || op.Nz == 32 || op.Nz == 48 || op.Nz == 64;
+}
+
 template <typename R>
 void op_apply2(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin0, const cx_t<R>* vin1, cx_t<R>* vout0,
                cx_t<R>* vout1, hipStream_t s) {
   G2G_CHECK(op.have_tensor && op.have_sigma, -4, "operator not ready: tensor or conductivity missing");
-  G2G_CHECK(op.Nz % 16 == 0, -3, "two-vector apply needs Nz % 16 == 0");
+  G2G_CHECK(op_pair_supported(op), -3, "two-vector apply needs Nz = 16, 32, 48 or 64");
   const int Nx2 = 2 * op.Nx, Ny2 = 2 * op.Ny, Nz3 = 3 * op.Nz;
   if (!b.S2) {
     G2G_HIP(hipMalloc((void**)&b.S2, sizeof(cx_t<R>) * (size_t)Nx2 * Ny2 * Nz3));
@@ -922,20 +978,7 @@ void op_apply2(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin0, const cx_t<R>
       hipLaunchKernelGGL((k_fwd_y<R, LL>), dim3(Nx2, chy), dim3(NT), 0, s, b.X, Ss[r], b.twy, g);
     });
   }
-  {
-    const size_t sm = sizeof(cx_t<R>) * (8 * (size_t)Nz3 + 2 * (size_t)op.Nz * (op.Nz + 1));
-    auto launch_mma = [&](auto kern, int nth) {
-      if (sm > 65536) {
-        G2G_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    160 * 1024));
-      }
-      hipLaunchKernelGGL(kern, dim3(op.Nx * op.Ny), dim3(nth), sm, s, b.S, b.S2, b.G, g, op.BS);
-    };
-    G2G_CHECK(sm <= 160 * 1024, -3, "two-vector contraction does not fit in LDS for this Nz");
-    if (op.Nz <= 32) launch_mma(k_contract_mma<R, 3, 6, 2>, 192);
-    else if (op.Nz == 48) launch_mma(k_contract_mma<R, 3, 16, 2>, 192);
-    else launch_mma(k_contract_mma<R, 6, 11, 2>, 384);  // 12 tiles, two per wave
-  }
+  launch_contract3<R, 2>(op, b.G, b.S, b.S2, g, s);
   for (int r = 0; r < 2; r++) {
     pow2_dispatch(Ny2, [&](auto LC) {
       constexpr int LL = decltype(LC)::value;

@@ -510,6 +510,24 @@ int giem2g_op_apply_dev_f32(void* p, const void* in, void* out) {
   });
 }
 
+int giem2g_op_apply_pair_dev(void* p, int precision, const void* in0, const void* in1, void* out0,
+                             void* out1) {
+  return guard([&] {
+    Operator& op = H(p)->op;
+    G2G_CHECK(precision == 0 || precision == 1, -3, "precision must be 0 or 1");
+    if (precision == 0) {
+      op_apply2<double>(op, op.d64, static_cast<const zd*>(in0), static_cast<const zd*>(in1),
+                        static_cast<zd*>(out0), static_cast<zd*>(out1), op.stream);
+      op.cnt.matvecs += 2;
+    } else {
+      if (!op.have_g32) op_make_g32(op);
+      op_apply2<float>(op, op.d32, static_cast<const zf*>(in0), static_cast<const zf*>(in1),
+                       static_cast<zf*>(out0), static_cast<zf*>(out1), op.stream);
+      op.cnt.matvecs32 += 2;
+    }
+  });
+}
+
 int giem2g_op_apply_host(void* p, const double* in, double* out) {
   return guard([&] { apply_host(H(p)->op, in, out); });
 }
@@ -671,7 +689,8 @@ int giem2g_op_set_contraction(void* p, int mode) {
   return guard([&] {
     Handle* h = H(p);
     G2G_CHECK(mode >= 0 && mode <= 3, -3, "contraction mode must be 0, 1, 2 or 3");
-    G2G_CHECK(mode != 2 || h->op.Nz % 16 == 0, -3, "the matrix-core contraction needs Nz % 16 == 0");
+    G2G_CHECK(mode != 2 || op_pair_supported(h->op), -3,
+              "the matrix-core contraction needs Nz = 16, 32, 48 or 64");
     h->op.contraction = mode;
   });
 }

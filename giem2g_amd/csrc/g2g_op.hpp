@@ -59,7 +59,7 @@ struct Operator {
   OpBuffers<double> d64;
   OpBuffers<float> d32;
   hipStream_t stream = nullptr;
-  int contraction = 0;  // 0 automatic (= 1), 1 vector ALU two-deep, 2 matrix cores, 3 vector ALU one-deep
+  int contraction = 0;  // 0 automatic (2 where it exists, else 1), 1 vector ALU two-deep, 2 matrix cores, 3 vector ALU one-deep
   Counters cnt;
   // tensor-build scratch (frequency independent weights are cached)
   double* wts = nullptr;  // 6 * 501 * Nx*Ny masked, lambda-scaled weights
@@ -120,9 +120,12 @@ template <typename R> void op_free_buffers(OpBuffers<R>& b);
 template <typename R>
 void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, hipStream_t s,
               bool timed);
+// two vectors through one tensor read; needs op_pair_supported(op)
 template <typename R>
 void op_apply2(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin0, const cx_t<R>* vin1, cx_t<R>* vout0,
                cx_t<R>* vout1, hipStream_t s);
+// Nz for which the matrix-core contraction exists (16, 32, 48, 64)
+bool op_pair_supported(const Operator& op);
 void op_set_sigma(Operator& op, const zd* d_csiga);
 // out = A^H in (FP64), from the reciprocity of the contraction kernel
 void op_apply_adjoint(Operator& op, const zd* vin, zd* vout, hipStream_t s);

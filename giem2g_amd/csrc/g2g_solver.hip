@@ -681,7 +681,7 @@ struct PairBatcher {
         return;
       }
       if (r[ot].pending || !active[ot]) {
-        if (r[ot].pending && r[ot].prec == prec && op->Nz % 16 == 0) {
+        if (r[ot].pending && r[ot].prec == prec && op_pair_supported(*op)) {
           const Req& a = r[0];
           const Req& b = r[1];
           if (prec == 0)

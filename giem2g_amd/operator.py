@@ -114,6 +114,14 @@ class IEOperator:
     def apply_dev_f32(self, vin, vout):
         L.check(self.lib.giem2g_op_apply_dev_f32(self.h, _ptr(vin), _ptr(vout)))
 
+    def apply_pair_dev(self, in0, in1, out0, out1):
+        """out0 = A in0, out1 = A in1 with one tensor read (torch tensors on
+        the GPU, all complex128 or all complex64; Nz = 16, 32, 48 or 64)."""
+        import torch
+        prec = 1 if in0.dtype == torch.complex64 else 0
+        L.check(self.lib.giem2g_op_apply_pair_dev(self.h, prec, _ptr(in0), _ptr(in1), _ptr(out0),
+                                                   _ptr(out1)))
+
     def sync(self):
         L.check(self.lib.giem2g_op_sync(self.h))
 

@@ -138,6 +138,12 @@ int giem2g_op_set_sigma_dev(void *ie_op, const void *csiga_dev);
 /* out = A*in on HBM pointers, complex128 (the FP64 drop-in) or complex64. */
 int giem2g_op_apply_dev(void *ie_op, const void *in_dev, void *out_dev);
 int giem2g_op_apply_dev_f32(void *ie_op, const void *in_dev, void *out_dev);
+/* out0 = A*in0 and out1 = A*in1 with one read of the tensor (two-column
+ * matrix-core contraction; needs Nz = 16, 32, 48 or 64); precision 0 complex128,
+ * 1 complex64.  No reference counterpart: the reference applies the operator
+ * to one vector at a time (apply_ie_operator_module.F90:39-96). */
+int giem2g_op_apply_pair_dev(void *ie_op, int precision, const void *in0_dev, const void *in1_dev,
+                             void *out0_dev, void *out1_dev);
 /* Same on host pointers (PCIe copies included). */
 int giem2g_op_apply_host(void *ie_op, const double *in, double *out);
 /* Stream used by the operator (hipStream_t) and device synchronisation. */
@@ -205,10 +211,11 @@ int giem2g_solve_dev(void *ie_op, const void *E_bkg_dev, const void *E0_dev,
                      const giem2g_solver_params *p, void *E_sol_dev,
                      giem2g_solve_info *info);
 /* Kernel of the per-(kx,ky) tensor contraction of one vector: 0 automatic
- * (= 1), 1 vector ALU with two tensor components in flight, 2 matrix cores
- * (MFMA; requires Nz % 16 == 0), 3 vector ALU with one component in flight.
- * All compute the same product.  Paired two-vector products
- * (giem2g_solve_pair_dev) use the matrix cores. */
+ * (2 for Nz = 16, 32, 48 or 64, otherwise 1), 1 vector ALU with two tensor
+ * components in flight, 2 matrix cores (MFMA; Nz = 16, 32, 48 or 64), 3
+ * vector ALU with one component in flight.  All compute the same product.
+ * Two-vector products (giem2g_op_apply_pair_dev, giem2g_solve_pair_dev) use
+ * the matrix cores. */
 int giem2g_op_set_contraction(void *ie_op, int mode);
 
 /* Scalings between the field and the scaled system (ie_solver_module.F90:
@@ -220,7 +227,7 @@ int giem2g_op_scale(void *ie_op, int mode, const void *in_dev, void *out_dev);
 /* Two independent solves with the same operator (e.g. the EY and EX
  * polarizations of one frequency) run together: their operator products
  * are paired so that one two-vector apply reads the tensor once for both
- * (needs Nz % 16 == 0, otherwise the products run one by one).  HBM
+ * (needs Nz = 16, 32, 48 or 64, otherwise the products run one by one).  HBM
  * pointers; E0x may be NULL; info2 receives two records (seconds = wall
  * time of the pair); paired (nullable) receives [two-vector applies,
  * one-vector applies].  precond must be 0.  No reference counterpart: the

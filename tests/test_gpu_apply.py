@@ -51,8 +51,9 @@ def test_apply_matches_oracle_on_oracle_tensor(oracle, Nx, Ny, Nz):
 
 @pytest.mark.parametrize("Nx,Ny,Nz", [(16, 16, 16), (8, 16, 32), (8, 8, 48), (8, 8, 64)])
 def test_matrix_core_contraction_matches_oracle(oracle, Nx, Ny, Nz):
-    """Nz % 16 == 0: the MFMA contraction (mode 2) and the vector-ALU one
-    (mode 1) against the oracle, FP64 at 1e-12; complex64 within 2e-6."""
+    """Nz = 16, 32, 48, 64: the MFMA contraction (mode 2, the default) and
+    the vector-ALU one (mode 1) against the oracle, FP64 at 1e-12; complex64
+    within 2e-6."""
     torch = _torch()
     from giem2g_amd import models, operator as gop
     print("case", Nx, Ny, Nz, flush=True)
@@ -73,6 +74,42 @@ def test_matrix_core_contraction_matches_oracle(oracle, Nx, Ny, Nz):
         op.apply_dev_f32(vt, o32)
         torch.cuda.synchronize()
         assert rel(o32.to(torch.complex128), ref) < 2e-6, mode
+    op.close()
+
+
+@pytest.mark.parametrize("Nx,Ny,Nz", [(16, 8, 16), (8, 16, 32), (8, 8, 48), (8, 8, 64)])
+def test_paired_apply_matches_oracle(oracle, Nx, Ny, Nz):
+    """Two vectors through one tensor read (giem2g_op_apply_pair_dev): each
+    result against the oracle, FP64 at 1e-12 and complex64 within 2e-6, and
+    the pair against two one-vector matrix-core applies, FP64 at 1e-14."""
+    torch = _torch()
+    from giem2g_amd import models, operator as gop
+    print("case", Nx, Ny, Nz, flush=True)
+    m = models.small(Nx, Ny, Nz, seed=2 * Nx + Nz, freq=0.8, dz=30.0)
+    om = oracle_model(oracle, m)
+    Gs = om.tensor_fft(om.tensor_realspace())
+    v0, v1 = rand_vec(3 * m.ncell, 21), rand_vec(3 * m.ncell, 22)
+    r0, r1 = om.apply(Gs, m.csiga(), v0), om.apply(Gs, m.csiga(), v1)
+    op = gop.IEOperator.from_model(m)
+    op.set_background(m.csigb(), m.thick, m.omega)
+    op.put_tensor(Gs, spectral=True)
+    op.set_sigma(m.csiga())
+    for dt, tol in ((torch.complex128, 1e-12), (torch.complex64, 2e-6)):
+        a0, a1 = torch.from_numpy(v0).cuda().to(dt), torch.from_numpy(v1).cuda().to(dt)
+        o0, o1 = torch.empty_like(a0), torch.empty_like(a1)
+        op.apply_pair_dev(a0, a1, o0, o1)
+        op.sync()
+        assert rel(o0.cpu().numpy().astype(np.complex128), r0) < tol, dt
+        assert rel(o1.cpu().numpy().astype(np.complex128), r1) < tol, dt
+        if dt == torch.complex128:
+            op.set_contraction(2)
+            s0, s1 = torch.empty_like(a0), torch.empty_like(a1)
+            op.apply_dev(a0, s0)
+            op.apply_dev(a1, s1)
+            op.sync()
+            assert rel(o0.cpu().numpy(), s0.cpu().numpy()) < 1e-14
+            assert rel(o1.cpu().numpy(), s1.cpu().numpy()) < 1e-14
+            op.set_contraction(0)
     op.close()
 
 
