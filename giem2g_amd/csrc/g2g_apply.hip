@@ -597,7 +597,9 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S0,
   T2* vin = reinterpret_cast<T2*>(smem_raw);  // [NRHS][4][3Nz], point-signed inputs
   T2* gbuf = vin + NRHS * 4 * NZ3;            // [2][GBS]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int b = blockIdx.x, kx = b % g.Nx, ky = b / g.Nx;
+  // blocks walk ky fastest: concurrent blocks touch neighbouring spectrum
+  // rows (measured 2-3% faster at 256x256 than kx fastest)
+  const int kx = blockIdx.x / g.Ny, ky = blockIdx.x % g.Ny, b = kx + g.Nx * ky;
   const int Nx2 = 2 * g.Nx, Ny2 = 2 * g.Ny, gNx = g.Nx, gNy = g.Ny;
 #define PX(p) (((p) & 1) ? (kx == 0 ? gNx : Nx2 - kx) : kx)
 #define PY(p) (((p) & 2) ? (ky == 0 ? gNy : Ny2 - ky) : ky)
@@ -634,16 +636,43 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S0,
     const int n_ = c_ < 4 ? P : FULL;  \
     G2G_FOR16(C3_ST##SET)              \
   }
+  // the point rows: complex64 issues them first as unrolled clamped loads
+  // (paired Nz = 64: 6% faster); FP64 keeps the plain loop (the early
+  // loads were 20% slower there)
+  constexpr int NV = NRHS * 4 * NZ3, NVQ = (NV + NTH - 1) / NTH;
+  constexpr bool EARLY = sizeof(R) == 4;
+  T2 vq[EARLY ? NVQ : 1];
+  if constexpr (EARLY) {
+#pragma unroll
+    for (int k = 0; k < NVQ; k++) {
+      const int i = min(tid + k * NTH, NV - 1);
+      const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3;
+      vq[k] = (rp < 4 ? S0 : S1)[((int64_t)PX(p) * Ny2 + PY(p)) * NZ3 + e];
+    }
+  }
   C3_PREFETCH(0, F)
   if (DEEP == 2) C3_PREFETCH(1, G)
-  for (int i = tid; i < NRHS * 4 * NZ3; i += NTH) {
-    const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3;
-    const T2* Sr = rp < 4 ? S0 : S1;
-    T2 vv = zero;
-    if (VAL(p)) vv = Sr[((int64_t)PX(p) * Ny2 + PY(p)) * NZ3 + e];
-    // D_p: x times s (-1 for -kx), y times m (-1 for -ky)
-    if ((e < NZ && (p & 1)) || (e >= NZ && e < 2 * NZ && (p & 2))) vv = -vv;
-    vin[i] = vv;
+  if constexpr (EARLY) {
+#pragma unroll
+    for (int k = 0; k < NVQ; k++) {
+      const int i = tid + k * NTH;
+      if (i < NV) {
+        const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3;
+        T2 vv = VAL(p) ? vq[k] : zero;
+        // D_p: x times s (-1 for -kx), y times m (-1 for -ky)
+        if ((e < NZ && (p & 1)) || (e >= NZ && e < 2 * NZ && (p & 2))) vv = -vv;
+        vin[i] = vv;
+      }
+    }
+  } else {
+    for (int i = tid; i < NV; i += NTH) {
+      const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3;
+      const T2* Sr = rp < 4 ? S0 : S1;
+      T2 vv = zero;
+      if (VAL(p)) vv = Sr[((int64_t)PX(p) * Ny2 + PY(p)) * NZ3 + e];
+      if ((e < NZ && (p & 1)) || (e >= NZ && e < 2 * NZ && (p & 2))) vv = -vv;
+      vin[i] = vv;
+    }
   }
   C3_STASH(0, F, gbuf)
   __syncthreads();
@@ -652,12 +681,14 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S0,
   const int rr = lane & 15, kq = lane >> 4;
   const int pnt = rr & 3, part = rr >> 2, rhs = (part >> 1) & (NRHS - 1), reim = part & 1;
   const int jw = wave * KW;
+  const T2* vpt = vin + (rhs * 4 + pnt) * NZ3 + jw + kq;
+  // the point operand of all three input components, loaded once
   R bR[3][KS], bI[3][KS];
 #pragma unroll
   for (int ci = 0; ci < 3; ci++)
 #pragma unroll
     for (int s = 0; s < KS; s++) {
-      const T2 v = vin[(rhs * 4 + pnt) * NZ3 + ci * NZ + jw + 4 * s + kq];
+      const T2 v = vpt[ci * NZ + 4 * s];
       bR[ci][s] = reim ? v.y : v.x;   // Gr pass: [Re v | Im v]
       bI[ci][s] = reim ? v.x : -v.y;  // Gi pass: [-Im v | Re v]
     }
@@ -679,6 +710,15 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S0,
       C3_PREFETCH(c + 1, F)
     }
     const T2* cur = gbuf + (c & 1) * GBS;
+    // input components of the step's first and second use
+    constexpr int ciA = c == 0 ? 0 : (c == 3 || c >= 4 ? 2 : 1);
+    constexpr int ciB = c == 1 ? 0 : (c >= 4 ? c - 4 : 0);
+    R pR[2][KS], pI[2][KS];
+#pragma unroll
+    for (int s = 0; s < KS; s++) {
+      pR[0][s] = bR[ciA][s], pI[0][s] = bI[ciA][s];
+      pR[1][s] = bR[ciB][s], pI[1][s] = bI[ciB][s];
+    }
 #pragma unroll
     for (int s = 0; s < KS; s++) {
       const int j = jw + 4 * s + kq;
@@ -689,23 +729,22 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S0,
           // symmetric, packed by rows of the lower triangle
           const T2 a = cur[j <= r ? j + r * (r + 1) / 2 : r + j * (j + 1) / 2];
           // xx: x <- x; xy: x <- y (and y <- x below); yy: y <- y; zz: z <- z
-          const int co = c <= 1 ? 0 : (c == 2 ? 1 : 2);
-          const int ci = c == 0 ? 0 : (c == 3 ? 2 : 1);
-          acc[co][t] = mma4(a.x, bR[ci][s], acc[co][t]);
-          acc[co][t] = mma4(a.y, bI[ci][s], acc[co][t]);
-          if (c == 1) {  // S_xy also maps x to y
-            acc[1][t] = mma4(a.x, bR[0][s], acc[1][t]);
-            acc[1][t] = mma4(a.y, bI[0][s], acc[1][t]);
+          constexpr int co = c <= 1 ? 0 : (c == 2 ? 1 : 2);
+          acc[co][t] = mma4(a.x, pR[0][s], acc[co][t]);
+          acc[co][t] = mma4(a.y, pI[0][s], acc[co][t]);
+          if constexpr (c == 1) {  // S_xy also maps x to y
+            acc[1][t] = mma4(a.x, pR[1][s], acc[1][t]);
+            acc[1][t] = mma4(a.y, pI[1][s], acc[1][t]);
           }
         } else {
           // A (xz or yz) at column j, row r; its use in z is -A^T
-          const int co = c - 4;
+          constexpr int co = c - 4;
           const T2 a = cur[j * SF + r];
           const T2 at = cur[r * SF + j];
-          acc[co][t] = mma4(a.x, bR[2][s], acc[co][t]);
-          acc[co][t] = mma4(a.y, bI[2][s], acc[co][t]);
-          acc[2][t] = mma4(-at.x, bR[co][s], acc[2][t]);
-          acc[2][t] = mma4(-at.y, bI[co][s], acc[2][t]);
+          acc[co][t] = mma4(a.x, pR[0][s], acc[co][t]);
+          acc[co][t] = mma4(a.y, pI[0][s], acc[co][t]);
+          acc[2][t] = mma4(-at.x, pR[1][s], acc[2][t]);
+          acc[2][t] = mma4(-at.y, pI[1][s], acc[2][t]);
         }
       }
     }
@@ -800,6 +839,8 @@ static bool launch_contract3(const Operator& op, const cx_t<R>* G, cx_t<R>* S0, 
   // waves per SIMD instead of three, 1-3% faster) and at Nz = 64 uses eight
   // waves (9-11% faster); complex64 at Nz = 64 stays at four waves (eight
   // were 12-18% slower)
+  // (the point operand read from LDS per step instead of registers, and two
+  // FP64 components in flight at four waves per SIMD, measured within 1%)
   constexpr bool F64 = sizeof(R) == 8;
   constexpr int DP = F64 ? 1 : 2, W64 = F64 ? 8 : 4;
   switch (op.Nz) {
