@@ -47,13 +47,8 @@ __device__ inline int64_t xidx(const Geo& g, int cz, int y, int kx) {
   return ((((int64_t)(cz >> g.nrx_log) * g.Ny + y) * (2 * g.Nx) + kx) << g.nrx_log) + (cz & (nrx - 1));
 }
 
-// Load of a tensor element, which each apply reads exactly once: the
-// non-temporal hint keeps the tensor stream from evicting the spectrum that
-// the next FFT pass reads from L2 (FP64, 128x128x32: contraction 6%, inverse
-// y pass 17% faster)
-template <typename T2> __device__ inline T2 ld_stream(const T2* p) {
-  return mk<T2>(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
-}
+// Tensor elements are loaded with ld_stream (each apply reads them once):
+// FP64, 128x128x32, contraction 6% and inverse y pass 17% faster.
 
 template <typename T2, int L>
 __device__ inline void load_tw(T2* tw, const T2* g) {

@@ -51,6 +51,13 @@ template <typename T2> __host__ __device__ inline T2 mk(decltype(T2::x) a, declt
   return r;
 }
 __host__ __device__ inline zd mkz(double a, double b) { return zd{a, b}; }
+// Load of an element that a kernel streams through once (a tensor, a Krylov
+// basis vector): the non-temporal hint keeps the stream from evicting data
+// that is read again soon from the caches (the spectrum the next FFT pass
+// reads; the vector that a multi-dot reads for every block row)
+template <typename T2> __device__ inline T2 ld_stream(const T2* p) {
+  return mk<T2>(__builtin_nontemporal_load(&p->x), __builtin_nontemporal_load(&p->y));
+}
 #define G2G_CPLX_OPS(T, R)                                                                  \
   __host__ __device__ inline T operator+(T a, T b) { return T{a.x + b.x, a.y + b.y}; }     \
   __host__ __device__ inline T operator-(T a, T b) { return T{a.x - b.x, a.y - b.y}; }     \

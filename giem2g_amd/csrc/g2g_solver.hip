@@ -53,7 +53,7 @@ __global__ __launch_bounds__(BT) void k_mdot(const T2* __restrict__ V, int64_t N
 #pragma unroll
     for (int k = 0; k < KC; k++)
       if (k < kc) {
-        const T2 v = V[(int64_t)k * N + i];
+        const T2 v = ld_stream(V + (int64_t)k * N + i);
         re[k] += (A)v.x * (A)wv.x + (A)v.y * (A)wv.y;
         im[k] += (A)v.x * (A)wv.y - (A)v.y * (A)wv.x;
       }
@@ -111,7 +111,7 @@ __global__ __launch_bounds__(BT) void k_update_norm(T2* __restrict__ w, const T2
   for (int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x; i < N; i += (int64_t)gridDim.x * BT) {
     T2 acc = w[i];
     for (int k = 0; k < K; k++) {
-      const T2 v = V[(int64_t)k * N + i], hk = hs[k];
+      const T2 v = ld_stream(V + (int64_t)k * N + i), hk = hs[k];
       acc.x -= v.x * hk.x - v.y * hk.y;
       acc.y -= v.x * hk.y + v.y * hk.x;
     }
