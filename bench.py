@@ -7,7 +7,8 @@ background 0.01/0.1/1 S/m, f = 10 Hz), complex128 currents re/im ~ N(0,1).
 resident in HBM.  The JSON line also carries the roofline of the dominant
 kernel (the tensor contraction), the oracle's CPU throughput on the same
 workload, the continuation of the field to the reference deck's receivers
-on the same grid, and (unless --no-solve) the time to a 1e-8 FP64 true
+on the same grid, eight right-hand sides through one tensor read per
+contraction (batched apply), and (unless --no-solve) the time to a 1e-8 FP64 true
 residual of the 256x256x32 random-box model for the plain FP64 solve and
 the mixed complex64-inner / FP64-outer solve.
 
@@ -186,6 +187,37 @@ def c3_bench(torch, gop, models, nfreq=16):
 REF_DECK_RECEIVERS = [(0.0, -20000.0, 0.0), (0.0, 20000.0, 0.0)]
 
 
+def batch_bench(torch, op, m, stage64, stage32, nrhs=8, reps=20):
+    """Eight right-hand sides through giem2g_op_apply_batch_dev on the C1
+    grid (complex64: one tensor read per contraction for all eight; FP64: two
+    reads of four): time per batch and matvecs/s against one-vector applies.
+    The contraction kernel's own time, matrix-core rate and HBM rate come
+    from the rocprofv3 trace of this run (kernel k_contract3<..., 32, 8 or 4,
+    ...>); mfma_flop_per_batch is the matrix-core work of one batch."""
+    out = {"nrhs": nrhs}
+    g = torch.Generator(device="cuda").manual_seed(5)
+    for name, dt, stage, reads in (("complex64", torch.complex64, stage32, 1), ("fp64", torch.complex128, stage64, 2)):
+        vin = torch.randn(nrhs, 3 * m.ncell, dtype=dt, device="cuda", generator=g)
+        vout = torch.empty_like(vin)
+        for _ in range(3):
+            op.apply_batch_dev(vin, vout)
+        op.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            op.apply_batch_dev(vin, vout)
+        op.sync()
+        ms = 1e3 * (time.perf_counter() - t0) / reps
+        # 9 sub-block uses x Nz/16 row tiles x Nz/4 layer steps x 2 MFMA x 2048
+        # flop per (kx,ky) block, one 16-column operand per two right-hand sides
+        flop = 9 * (m.Nz // 16) * (m.Nz // 4) * 2 * 2048 * (nrhs // 2) * m.Nx * m.Ny
+        rec = {"ms_per_batch": ms, "matvec_s": nrhs / (ms * 1e-3),
+               "single_matvec_s": 1e3 / float(stage[0] + stage[1] + stage[2]),
+               "tensor_reads_per_batch": reads, "mfma_flop_per_batch": flop}
+        log("batch", name, rec)
+        out[name] = rec
+    return out
+
+
 def recalc_bytes(Nx, Ny, Nz, Nr, s=16):
     """Algorithmic bytes of one continuation to Nr receivers: forward FFTs
     as in the IE apply (fwd), the contraction reading the receiver tensor
@@ -335,6 +367,7 @@ def main():
     else:
         line["cpu_baseline"] = None
     line["recalc"] = recalc_bench(torch, gop, op, m, v, cpu=not a.no_cpu)
+    line["batched"] = batch_bench(torch, op, m, stage_ms, stage32)
     op.close()
     if not a.no_solve:
         line["solve"] = solve_bench(torch, gop, models)

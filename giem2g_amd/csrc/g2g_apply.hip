@@ -575,6 +575,7 @@ template <> __device__ inline int mma_row<double>(int kq, int q) { return kq + 4
 template <typename R, int NZ, int NRHS, int NW>
 struct C3Cfg {
   static constexpr int NTH = 64 * NW, NZ3 = 3 * NZ, NT16 = NZ / 16, KW = NZ / NW, KS = KW / 4;
+  static constexpr int NG = NRHS == 1 ? 1 : NRHS / 2;  // 16-column operand groups
   static constexpr int P = NZ * (NZ + 1) / 2, FULL = NZ * NZ, SF = NZ + 1, GBS = NZ * SF;
   static constexpr int PF = (FULL + NTH - 1) / NTH;
   static constexpr int SL = 3 * NT16 * 256;  // reals per partial-tile slot
@@ -585,19 +586,19 @@ struct C3Cfg {
 };
 
 template <typename R, int NZ, int NRHS, int DEEP, int NW>
-__global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S0, cx_t<R>* __restrict__ S1,
+__global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S, int64_t sst,
                                                    const cx_t<R>* __restrict__ G, Geo g, int64_t BS) {
   using T2 = cx_t<R>;
   using V4 = typename v4t<R>::t;
   using C = C3Cfg<R, NZ, NRHS, NW>;
   constexpr int NTH = C::NTH, NZ3 = C::NZ3, NT16 = C::NT16, KW = C::KW, KS = C::KS;
-  constexpr int P = C::P, FULL = C::FULL, SF = C::SF, GBS = C::GBS, PF = C::PF, SL = C::SL;
+  constexpr int P = C::P, FULL = C::FULL, SF = C::SF, GBS = C::GBS, PF = C::PF, SL = C::SL, NG = C::NG;
   static_assert(NZ % 16 == 0 && PF <= 16, "Nz must be a multiple of 16, at most 64");
   static_assert((NW == 4 || NW == 8) && KW % 4 == 0, "four or eight waves, whole MFMA steps each");
-  static_assert(NRHS == 1 || NRHS == 2, "one or two right-hand sides");
+  static_assert(NRHS == 1 || NRHS == 2 || NRHS == 4 || NRHS == 8, "1, 2, 4 or 8 right-hand sides");
   static_assert(DEEP == 1 || DEEP == 2, "one or two components in flight");
   extern __shared__ unsigned char smem_raw[];
-  T2* vin = reinterpret_cast<T2*>(smem_raw);  // [NRHS][4][3Nz], point-signed inputs
+  T2* vin = reinterpret_cast<T2*>(smem_raw);  // [NRHS][4][3Nz], point-signed inputs; rhs r at S + r*sst
   T2* gbuf = vin + NRHS * 4 * NZ3;            // [2][GBS]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   // blocks walk ky fastest: concurrent blocks touch neighbouring spectrum
@@ -644,33 +645,37 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S0,
   // loads were 20% slower there)
   constexpr int NV = NRHS * 4 * NZ3, NVQ = (NV + NTH - 1) / NTH;
   constexpr bool EARLY = sizeof(R) == 4;
-  T2 vq[EARLY ? NVQ : 1];
+  static_assert(!EARLY || NVQ <= 16, "point rows: at most sixteen registers per lane");
+  // named registers (a private array was placed in scratch memory for some sizes)
+#define C3_VQD(k) T2 vq##k = zero;
+  G2G_FOR16(C3_VQD)
+#undef C3_VQD
+#define C3_VQL(k)                                                                \
+  if ((k) < NVQ) {                                                               \
+    const int i = min(tid + (k) * NTH, NV - 1);                                  \
+    const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3;                        \
+    vq##k = S[(rp >> 2) * sst + ((int64_t)PX(p) * Ny2 + PY(p)) * NZ3 + e];       \
+  }
+#define C3_VQP(k)                                                                \
+  if ((k) < NVQ && tid + (k) * NTH < NV) {                                       \
+    const int i = tid + (k) * NTH;                                               \
+    const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3;                        \
+    T2 vv = VAL(p) ? vq##k : zero;                                               \
+    /* D_p: x times s (-1 for -kx), y times m (-1 for -ky) */                    \
+    if ((e < NZ && (p & 1)) || (e >= NZ && e < 2 * NZ && (p & 2))) vv = -vv;     \
+    vin[i] = vv;                                                                 \
+  }
   if constexpr (EARLY) {
-#pragma unroll
-    for (int k = 0; k < NVQ; k++) {
-      const int i = min(tid + k * NTH, NV - 1);
-      const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3;
-      vq[k] = (rp < 4 ? S0 : S1)[((int64_t)PX(p) * Ny2 + PY(p)) * NZ3 + e];
-    }
+    G2G_FOR16(C3_VQL)
   }
   C3_PREFETCH(0, F)
   if (DEEP == 2) C3_PREFETCH(1, G)
   if constexpr (EARLY) {
-#pragma unroll
-    for (int k = 0; k < NVQ; k++) {
-      const int i = tid + k * NTH;
-      if (i < NV) {
-        const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3;
-        T2 vv = VAL(p) ? vq[k] : zero;
-        // D_p: x times s (-1 for -kx), y times m (-1 for -ky)
-        if ((e < NZ && (p & 1)) || (e >= NZ && e < 2 * NZ && (p & 2))) vv = -vv;
-        vin[i] = vv;
-      }
-    }
+    G2G_FOR16(C3_VQP)
   } else {
     for (int i = tid; i < NV; i += NTH) {
       const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3;
-      const T2* Sr = rp < 4 ? S0 : S1;
+      const T2* Sr = S + (rp >> 2) * sst;
       T2 vv = zero;
       if (VAL(p)) vv = Sr[((int64_t)PX(p) * Ny2 + PY(p)) * NZ3 + e];
       if ((e < NZ && (p & 1)) || (e >= NZ && e < 2 * NZ && (p & 2))) vv = -vv;
@@ -680,26 +685,30 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S0,
   C3_STASH(0, F, gbuf)
   __syncthreads();
   // operand roles: lane l supplies A[row l%16][k l/16] and B[k l/16][col l%16];
-  // column = 4*part + point, part = 2*rhs + (0 re | 1 im)
+  // column = 4*part + point, part = 2*(rhs within the group) + (0 re | 1 im);
+  // operand group q serves right-hand sides 2q and 2q+1
   const int rr = lane & 15, kq = lane >> 4;
-  const int pnt = rr & 3, part = rr >> 2, rhs = (part >> 1) & (NRHS - 1), reim = part & 1;
+  const int pnt = rr & 3, part = rr >> 2, rl = NRHS == 1 ? 0 : (part >> 1), reim = part & 1;
   const int jw = wave * KW;
-  const T2* vpt = vin + (rhs * 4 + pnt) * NZ3 + jw + kq;
   // the point operand of all three input components, loaded once
-  R bR[3][KS], bI[3][KS];
+  R bR[NG][3][KS], bI[NG][3][KS];
 #pragma unroll
-  for (int ci = 0; ci < 3; ci++)
+  for (int q = 0; q < NG; q++)
 #pragma unroll
-    for (int s = 0; s < KS; s++) {
-      const T2 v = vpt[ci * NZ + 4 * s];
-      bR[ci][s] = reim ? v.y : v.x;   // Gr pass: [Re v | Im v]
-      bI[ci][s] = reim ? v.x : -v.y;  // Gi pass: [-Im v | Re v]
-    }
-  V4 acc[3][NT16];
+    for (int ci = 0; ci < 3; ci++)
 #pragma unroll
-  for (int co = 0; co < 3; co++)
+      for (int s = 0; s < KS; s++) {
+        const T2 v = vin[((2 * q + rl) * 4 + pnt) * NZ3 + ci * NZ + jw + 4 * s + kq];
+        bR[q][ci][s] = reim ? v.y : v.x;   // Gr pass: [Re v | Im v]
+        bI[q][ci][s] = reim ? v.x : -v.y;  // Gi pass: [-Im v | Re v]
+      }
+  V4 acc[NG][3][NT16];
 #pragma unroll
-    for (int t = 0; t < NT16; t++) acc[co][t] = V4{0, 0, 0, 0};
+  for (int q = 0; q < NG; q++)
+#pragma unroll
+    for (int co = 0; co < 3; co++)
+#pragma unroll
+      for (int t = 0; t < NT16; t++) acc[q][co][t] = V4{0, 0, 0, 0};
   // one tensor component per step; the component is a compile-time value so
   // that accumulator and register-set indices stay static
   auto step = [&](auto CC) {
@@ -716,12 +725,6 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S0,
     // input components of the step's first and second use
     constexpr int ciA = c == 0 ? 0 : (c == 3 || c >= 4 ? 2 : 1);
     constexpr int ciB = c == 1 ? 0 : (c >= 4 ? c - 4 : 0);
-    R pR[2][KS], pI[2][KS];
-#pragma unroll
-    for (int s = 0; s < KS; s++) {
-      pR[0][s] = bR[ciA][s], pI[0][s] = bI[ciA][s];
-      pR[1][s] = bR[ciB][s], pI[1][s] = bI[ciB][s];
-    }
 #pragma unroll
     for (int s = 0; s < KS; s++) {
       const int j = jw + 4 * s + kq;
@@ -733,21 +736,27 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S0,
           const T2 a = cur[j <= r ? j + r * (r + 1) / 2 : r + j * (j + 1) / 2];
           // xx: x <- x; xy: x <- y (and y <- x below); yy: y <- y; zz: z <- z
           constexpr int co = c <= 1 ? 0 : (c == 2 ? 1 : 2);
-          acc[co][t] = mma4(a.x, pR[0][s], acc[co][t]);
-          acc[co][t] = mma4(a.y, pI[0][s], acc[co][t]);
-          if constexpr (c == 1) {  // S_xy also maps x to y
-            acc[1][t] = mma4(a.x, pR[1][s], acc[1][t]);
-            acc[1][t] = mma4(a.y, pI[1][s], acc[1][t]);
+#pragma unroll
+          for (int q = 0; q < NG; q++) {
+            acc[q][co][t] = mma4(a.x, bR[q][ciA][s], acc[q][co][t]);
+            acc[q][co][t] = mma4(a.y, bI[q][ciA][s], acc[q][co][t]);
+            if constexpr (c == 1) {  // S_xy also maps x to y
+              acc[q][1][t] = mma4(a.x, bR[q][ciB][s], acc[q][1][t]);
+              acc[q][1][t] = mma4(a.y, bI[q][ciB][s], acc[q][1][t]);
+            }
           }
         } else {
           // A (xz or yz) at column j, row r; its use in z is -A^T
           constexpr int co = c - 4;
           const T2 a = cur[j * SF + r];
           const T2 at = cur[r * SF + j];
-          acc[co][t] = mma4(a.x, pR[0][s], acc[co][t]);
-          acc[co][t] = mma4(a.y, pI[0][s], acc[co][t]);
-          acc[2][t] = mma4(-at.x, pR[1][s], acc[2][t]);
-          acc[2][t] = mma4(-at.y, pI[1][s], acc[2][t]);
+#pragma unroll
+          for (int q = 0; q < NG; q++) {
+            acc[q][co][t] = mma4(a.x, bR[q][ciA][s], acc[q][co][t]);
+            acc[q][co][t] = mma4(a.y, bI[q][ciA][s], acc[q][co][t]);
+            acc[q][2][t] = mma4(-at.x, bR[q][ciB][s], acc[q][2][t]);
+            acc[q][2][t] = mma4(-at.y, bI[q][ciB][s], acc[q][2][t]);
+          }
         }
       }
     }
@@ -764,35 +773,38 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S0,
   step(std::integral_constant<int, 4>{});
   step(std::integral_constant<int, 5>{});
   // partial tiles: slot [tile][q][lane]; halving tree, wave w adds wave w + h
+  // one operand group at a time (the slots are reused; wave 0 finishes
+  // reading a group before the next group is written)
   R* red = reinterpret_cast<R*>(gbuf);
-  auto put = [&](int slot) {
+  R* vo = reinterpret_cast<R*>(vin);  // results with the output signs, over vin
+  const R sgx = (pnt & 1) ? R(-1) : R(1), sgy = (pnt & 2) ? R(-1) : R(1);
 #pragma unroll
-    for (int co = 0; co < 3; co++)
+  for (int gq = 0; gq < NG; gq++) {
+    auto put = [&](int slot) {
 #pragma unroll
-      for (int t = 0; t < NT16; t++)
+      for (int co = 0; co < 3; co++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) red[slot * SL + ((co * NT16 + t) * 4 + q) * 64 + lane] = acc[co][t][q];
-  };
-  auto add = [&](int slot) {
+        for (int t = 0; t < NT16; t++)
 #pragma unroll
-    for (int co = 0; co < 3; co++)
+          for (int q = 0; q < 4; q++) red[slot * SL + ((co * NT16 + t) * 4 + q) * 64 + lane] = acc[gq][co][t][q];
+    };
+    auto add = [&](int slot) {
 #pragma unroll
-      for (int t = 0; t < NT16; t++)
+      for (int co = 0; co < 3; co++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) acc[co][t][q] += red[slot * SL + ((co * NT16 + t) * 4 + q) * 64 + lane];
-  };
+        for (int t = 0; t < NT16; t++)
 #pragma unroll
-  for (int h = NW / 2; h >= 1; h /= 2) {
-    if (wave >= h && wave < 2 * h) put(wave - h);
-    __syncthreads();
-    if (wave < h) add(wave);
-    if (h > 1) __syncthreads();
-  }
-  if (wave == 0) {
-    // results with the output signs -> LDS (over vin), then coalesced stores
-    R* vo = reinterpret_cast<R*>(vin);
-    const R sgx = (pnt & 1) ? R(-1) : R(1), sgy = (pnt & 2) ? R(-1) : R(1);
-    if ((part >> 1) < NRHS) {
+          for (int q = 0; q < 4; q++) acc[gq][co][t][q] += red[slot * SL + ((co * NT16 + t) * 4 + q) * 64 + lane];
+    };
+    if (gq > 0) __syncthreads();
+#pragma unroll
+    for (int h = NW / 2; h >= 1; h /= 2) {
+      if (wave >= h && wave < 2 * h) put(wave - h);
+      __syncthreads();
+      if (wave < h) add(wave);
+      if (h > 1) __syncthreads();
+    }
+    if (wave == 0 && (NRHS > 1 || (part >> 1) == 0)) {
 #pragma unroll
       for (int co = 0; co < 3; co++) {
         const R so = co == 0 ? sgx : (co == 1 ? sgy : R(1));
@@ -801,7 +813,7 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S0,
 #pragma unroll
           for (int q = 0; q < 4; q++) {
             const int row = co * NZ + 16 * t + mma_row<R>(kq, q);
-            vo[2 * ((rhs * 4 + pnt) * NZ3 + row) + reim] = acc[co][t][q] * so;
+            vo[2 * (((2 * gq + rl) * 4 + pnt) * NZ3 + row) + reim] = acc[gq][co][t][q] * so;
           }
       }
     }
@@ -809,11 +821,13 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S0,
   __syncthreads();
   for (int i = tid; i < NRHS * 4 * NZ3; i += NTH) {
     const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3;
-    T2* Sr = rp < 4 ? S0 : S1;
+    T2* Sr = S + (rp >> 2) * sst;
     T2 o = vin[i];
     if (!VAL(p)) o = zero;
     Sr[((int64_t)PX(p) * Ny2 + PY(p)) * NZ3 + e] = o;
   }
+#undef C3_VQL
+#undef C3_VQP
 #undef C3_LDF
 #undef C3_LDG
 #undef C3_PREFETCH
@@ -827,16 +841,17 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S0,
 }
 
 // ------------------------------------------------------------ dispatch ---
-// The layer-split matrix-core contraction (Nz = 16, 32, 48 or 64) of one or
-// two spectra (S1 is ignored for one); false for other Nz.
+// The layer-split matrix-core contraction of NRHS spectra (rhs r at S + r*sst)
+// through one tensor read: Nz = 16, 32, 48 or 64 for one or two, Nz = 16 or 32
+// for four or eight (register budget); false for other cases.
 template <typename R, int NRHS>
-static bool launch_contract3(const Operator& op, const cx_t<R>* G, cx_t<R>* S0, cx_t<R>* S1, const Geo& g,
+static bool launch_contract3(const Operator& op, const cx_t<R>* G, cx_t<R>* S, int64_t sst, const Geo& g,
                              hipStream_t s) {
   auto go = [&](auto kern, size_t sm, int nw) {
     G2G_CHECK(sm <= 160 * 1024, -3, "matrix-core contraction does not fit in LDS for this Nz");
     if (sm > 65536)
       G2G_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    hipLaunchKernelGGL(kern, dim3(op.Nx * op.Ny), dim3(64 * nw), sm, s, S0, S1, G, g, op.BS);
+    hipLaunchKernelGGL(kern, dim3(op.Nx * op.Ny), dim3(64 * nw), sm, s, S, sst, G, g, op.BS);
   };
   // measured choices: FP64 keeps one component in flight (Nz = 32: four
   // waves per SIMD instead of three, 1-3% faster) and at Nz = 64 uses eight
@@ -846,14 +861,21 @@ static bool launch_contract3(const Operator& op, const cx_t<R>* G, cx_t<R>* S0, 
   // FP64 components in flight at four waves per SIMD, measured within 1%)
   constexpr bool F64 = sizeof(R) == 8;
   constexpr int DP = F64 ? 1 : 2, W64 = F64 ? 8 : 4;
-  switch (op.Nz) {
-    case 16: go(k_contract3<R, 16, NRHS, 2, 4>, C3Cfg<R, 16, NRHS, 4>::smem, 4); break;
-    case 32: go(k_contract3<R, 32, NRHS, DP, 4>, C3Cfg<R, 32, NRHS, 4>::smem, 4); break;
-    case 48: go(k_contract3<R, 48, NRHS, 2, 4>, C3Cfg<R, 48, NRHS, 4>::smem, 4); break;
-    case 64: go(k_contract3<R, 64, NRHS, DP, W64>, C3Cfg<R, 64, NRHS, W64>::smem, W64); break;
-    default: return false;
+  if constexpr (NRHS <= 2) {
+    switch (op.Nz) {
+      case 16: go(k_contract3<R, 16, NRHS, 2, 4>, C3Cfg<R, 16, NRHS, 4>::smem, 4); return true;
+      case 32: go(k_contract3<R, 32, NRHS, DP, 4>, C3Cfg<R, 32, NRHS, 4>::smem, 4); return true;
+      case 48: go(k_contract3<R, 48, NRHS, 2, 4>, C3Cfg<R, 48, NRHS, 4>::smem, 4); return true;
+      case 64: go(k_contract3<R, 64, NRHS, DP, W64>, C3Cfg<R, 64, NRHS, W64>::smem, W64); return true;
+      default: return false;
+    }
+  } else {
+    switch (op.Nz) {
+      case 16: go(k_contract3<R, 16, NRHS, 2, 4>, C3Cfg<R, 16, NRHS, 4>::smem, 4); return true;
+      case 32: go(k_contract3<R, 32, NRHS, DP, 4>, C3Cfg<R, 32, NRHS, 4>::smem, 4); return true;
+      default: return false;
+    }
   }
-  return true;
 }
 template <typename R, template <typename, int> class KF> struct Dummy {};
 
@@ -911,7 +933,7 @@ template <typename R> void op_free_buffers(OpBuffers<R>& b) {
     if (p) (void)hipFree((void*)p);
     p = nullptr;
   };
-  F(b.G); F(b.X); F(b.S); F(b.S2); F(b.gsig); F(b.asiga); F(b.sq); F(b.dzn); F(b.twx); F(b.twy);
+  F(b.G); F(b.X); F(b.S); F(b.SB); F(b.gsig); F(b.asiga); F(b.sq); F(b.dzn); F(b.twx); F(b.twy);
 }
 
 template <typename R>
@@ -948,7 +970,7 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
     };
     // matrix cores for Nz = 16, 32, 48 and 64 unless a vector-ALU kernel is
     // requested (FP64 at 128x128x32: 0.28-0.30 ms against 0.33 ms)
-    if ((op.contraction == 0 || op.contraction == 2) && launch_contract3<R, 1>(op, b.G, b.S, b.S, g, s)) {
+    if ((op.contraction == 0 || op.contraction == 2) && launch_contract3<R, 1>(op, b.G, b.S, 0, g, s)) {
     } else if (op.contraction == 3) {
       if (op.Nz * op.Nz <= 4 * NT) launch(k_contract<R, 4>);
       else launch(k_contract<R, 16>);
@@ -988,45 +1010,70 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
   G2G_HIP(hipGetLastError());
 }
 
-// Two right-hand sides through one tensor read: both vectors are
-// transformed into their own spectrum buffers, one two-column-set MFMA
-// contraction serves both, then both are transformed back.  Needs
-// Nz = 16, 32, 48 or 64 (matrix-core contraction).
 bool op_pair_supported(const Operator& op) {
   return op.Nz == 16 || op.Nz == 32 || op.Nz == 48 || op.Nz == 64;
 }
+int op_batch_max(const Operator& op) {
+  return (op.Nz == 16 || op.Nz == 32) ? 8 : (op_pair_supported(op) ? 2 : 1);
+}
 
+template <typename T2> __global__ void k_zero(T2* __restrict__ a, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * NT + threadIdx.x; i < n; i += (int64_t)gridDim.x * NT) a[i] = mk<T2>(0, 0);
+}
+
+// k right-hand sides (1 <= k <= 8, k <= op_batch_max) through one tensor read:
+// each vector is transformed into its own slot of the batch spectrum
+// buffer, one contraction with k operand columns serves all of them, then
+// each is transformed back.  k = 3, 5, 6, 7 run the next larger contraction
+// on zero spectra in the unused slots.
 template <typename R>
-void op_apply2(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin0, const cx_t<R>* vin1, cx_t<R>* vout0,
-               cx_t<R>* vout1, hipStream_t s) {
-  G2G_CHECK(op.have_tensor && op.have_sigma, -4, "operator not ready: tensor or conductivity missing");
-  G2G_CHECK(op_pair_supported(op), -3, "two-vector apply needs Nz = 16, 32, 48 or 64");
+static void op_apply_k(Operator& op, OpBuffers<R>& b, int k, const cx_t<R>* const* vins,
+                       cx_t<R>* const* vouts, hipStream_t s) {
+  using T2 = cx_t<R>;
   const int Nx2 = 2 * op.Nx, Ny2 = 2 * op.Ny, Nz3 = 3 * op.Nz;
-  if (!b.S2) {
-    G2G_HIP(hipMalloc((void**)&b.S2, sizeof(cx_t<R>) * (size_t)Nx2 * Ny2 * Nz3));
-    b.bytes += sizeof(cx_t<R>) * (size_t)Nx2 * Ny2 * Nz3;
+  const int kk = k <= 1 ? 1 : (k <= 2 ? 2 : (k <= 4 ? 4 : 8));
+  const int64_t nS = (int64_t)Nx2 * Ny2 * Nz3;
+  if (b.sb_cap < kk) {
+    if (b.SB) {
+      (void)hipFree(b.SB);
+      b.bytes -= sizeof(T2) * (size_t)nS * b.sb_cap;
+    }
+    G2G_HIP(hipMalloc((void**)&b.SB, sizeof(T2) * (size_t)nS * kk));
+    b.bytes += sizeof(T2) * (size_t)nS * kk;
+    b.sb_cap = kk;
+  }
+  // unused slots hold zero spectra (a fill kernel, not a memset node, so
+  // that a caller may capture the apply in a graph)
+  if (kk > k) {
+    const int64_t nz = nS * (kk - k);
+    hipLaunchKernelGGL((k_zero<T2>), dim3((unsigned)std::min<int64_t>(4096, (nz + NT - 1) / NT)), dim3(NT), 0, s,
+                       b.SB + nS * k, nz);
   }
   Geo g = make_geo(op, b.NR);
   const int nry = nr_rt<R>(Ny2);
   const int chy = (Nz3 + nry - 1) / nry;
-  const cx_t<R>* vins[2] = {vin0, vin1};
-  cx_t<R>* vouts[2] = {vout0, vout1};
-  cx_t<R>* Ss[2] = {b.S, b.S2};
-  for (int r = 0; r < 2; r++) {
+  for (int r = 0; r < k; r++) {
     pow2_dispatch(Nx2, [&](auto LC) {
       constexpr int LL = decltype(LC)::value;
       hipLaunchKernelGGL((k_fwd_x<R, LL>), dim3(op.Ny, b.nchunk), dim3(NT), 0, s, vins[r], b.gsig, b.X, b.twx, g);
     });
     pow2_dispatch(Ny2, [&](auto LC) {
       constexpr int LL = decltype(LC)::value;
-      hipLaunchKernelGGL((k_fwd_y<R, LL>), dim3(Nx2, chy), dim3(NT), 0, s, b.X, Ss[r], b.twy, g);
+      hipLaunchKernelGGL((k_fwd_y<R, LL>), dim3(Nx2, chy), dim3(NT), 0, s, b.X, b.SB + nS * r, b.twy, g);
     });
   }
-  launch_contract3<R, 2>(op, b.G, b.S, b.S2, g, s);
-  for (int r = 0; r < 2; r++) {
+  bool ok = false;
+  switch (kk) {
+    case 1: ok = launch_contract3<R, 1>(op, b.G, b.SB, nS, g, s); break;
+    case 2: ok = launch_contract3<R, 2>(op, b.G, b.SB, nS, g, s); break;
+    case 4: ok = launch_contract3<R, 4>(op, b.G, b.SB, nS, g, s); break;
+    default: ok = launch_contract3<R, 8>(op, b.G, b.SB, nS, g, s); break;
+  }
+  G2G_CHECK(ok, -3, "batched contraction not available for this Nz");
+  for (int r = 0; r < k; r++) {
     pow2_dispatch(Ny2, [&](auto LC) {
       constexpr int LL = decltype(LC)::value;
-      hipLaunchKernelGGL((k_inv_y<R, LL>), dim3(Nx2, chy), dim3(NT), 0, s, Ss[r], b.X, b.twy, g);
+      hipLaunchKernelGGL((k_inv_y<R, LL>), dim3(Nx2, chy), dim3(NT), 0, s, b.SB + nS * r, b.X, b.twy, g);
     });
     pow2_dispatch(Nx2, [&](auto LC) {
       constexpr int LL = decltype(LC)::value;
@@ -1036,8 +1083,50 @@ void op_apply2(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin0, const cx_t<R>
   }
   G2G_HIP(hipGetLastError());
 }
+
+// Two right-hand sides through one tensor read (op_pair_supported).
+template <typename R>
+void op_apply2(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin0, const cx_t<R>* vin1, cx_t<R>* vout0,
+               cx_t<R>* vout1, hipStream_t s) {
+  G2G_CHECK(op.have_tensor && op.have_sigma, -4, "operator not ready: tensor or conductivity missing");
+  G2G_CHECK(op_pair_supported(op), -3, "two-vector apply needs Nz = 16, 32, 48 or 64");
+  const cx_t<R>* vins[2] = {vin0, vin1};
+  cx_t<R>* vouts[2] = {vout0, vout1};
+  op_apply_k<R>(op, b, 2, vins, vouts, s);
+}
 template void op_apply2<double>(Operator&, OpBuffers<double>&, const zd*, const zd*, zd*, zd*, hipStream_t);
 template void op_apply2<float>(Operator&, OpBuffers<float>&, const zf*, const zf*, zf*, zf*, hipStream_t);
+
+// nrhs vectors stored one after another (3 Nx Ny Nz complex each), in
+// batches of at most op_batch_max(op) per tensor read.
+template <typename R>
+void op_apply_batch(Operator& op, OpBuffers<R>& b, int nrhs, const cx_t<R>* vin, cx_t<R>* vout,
+                    hipStream_t s) {
+  G2G_CHECK(op.have_tensor && op.have_sigma, -4, "operator not ready: tensor or conductivity missing");
+  G2G_CHECK(nrhs >= 1, -3, "batched apply needs at least one vector");
+  const int64_t nv = 3 * (int64_t)op.ncell();
+  // FP64 takes at most four per tensor read: the eight-vector FP64 kernel
+  // holds 360 registers per lane (one wave per SIMD) and measured 22% slower
+  // per vector than four at 128x128x32
+  const int kmax = std::min(op_batch_max(op), sizeof(R) == 8 ? 4 : 8);
+  for (int r0 = 0; r0 < nrhs;) {
+    const int k = std::min(kmax, nrhs - r0);
+    if (k == 1) {
+      op_apply<R>(op, b, vin + nv * r0, vout + nv * r0, s, false);
+    } else {
+      const cx_t<R>* vins[8];
+      cx_t<R>* vouts[8];
+      for (int r = 0; r < k; r++) {
+        vins[r] = vin + nv * (r0 + r);
+        vouts[r] = vout + nv * (r0 + r);
+      }
+      op_apply_k<R>(op, b, k, vins, vouts, s);
+    }
+    r0 += k;
+  }
+}
+template void op_apply_batch<double>(Operator&, OpBuffers<double>&, int, const zd*, zd*, hipStream_t);
+template void op_apply_batch<float>(Operator&, OpBuffers<float>&, int, const zf*, zf*, hipStream_t);
 
 // ============================================================ receivers ==
 // Continuation of the solution to receivers (ReCalculation, APPLY_RC_E_OP,

@@ -528,6 +528,22 @@ int giem2g_op_apply_pair_dev(void* p, int precision, const void* in0, const void
   });
 }
 
+int giem2g_op_apply_batch_dev(void* p, int precision, int nrhs, const void* in, void* out) {
+  return guard([&] {
+    Operator& op = H(p)->op;
+    G2G_CHECK(precision == 0 || precision == 1, -3, "precision must be 0 or 1");
+    G2G_CHECK(nrhs >= 1, -3, "nrhs must be at least 1");
+    if (precision == 0) {
+      op_apply_batch<double>(op, op.d64, nrhs, static_cast<const zd*>(in), static_cast<zd*>(out), op.stream);
+      op.cnt.matvecs += nrhs;
+    } else {
+      if (!op.have_g32) op_make_g32(op);
+      op_apply_batch<float>(op, op.d32, nrhs, static_cast<const zf*>(in), static_cast<zf*>(out), op.stream);
+      op.cnt.matvecs32 += nrhs;
+    }
+  });
+}
+
 int giem2g_op_apply_host(void* p, const double* in, double* out) {
   return guard([&] { apply_host(H(p)->op, in, out); });
 }

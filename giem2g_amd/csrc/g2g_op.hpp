@@ -29,7 +29,8 @@ template <typename R> struct OpBuffers {
   T2* G = nullptr;      // spectral tensor, Nx*Ny*BS
   T2* X = nullptr;      // nchunk*Ny*2Nx*NR
   T2* S = nullptr;      // 2Nx*2Ny*3Nz
-  T2* S2 = nullptr;     // second spectrum for two-vector applies (allocated on first use)
+  T2* SB = nullptr;     // spectra of batched applies, sb_cap slots (allocated on first use)
+  int sb_cap = 0;
   T2* gsig = nullptr;   // Nx*Ny*Nz
   T2* asiga = nullptr;  // Nx*Ny*Nz
   R* sq = nullptr;      // Nz: sqrt(Re sigma_b)
@@ -126,6 +127,13 @@ void op_apply2(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin0, const cx_t<R>
                cx_t<R>* vout1, hipStream_t s);
 // Nz for which the matrix-core contraction exists (16, 32, 48, 64)
 bool op_pair_supported(const Operator& op);
+// right-hand sides one tensor read serves: 8 (Nz = 16, 32), 2 (Nz = 48, 64), else 1
+int op_batch_max(const Operator& op);
+// nrhs vectors stored one after another, batches of op_batch_max (FP64: at
+// most 4) per tensor read
+template <typename R>
+void op_apply_batch(Operator& op, OpBuffers<R>& b, int nrhs, const cx_t<R>* vin, cx_t<R>* vout,
+                    hipStream_t s);
 void op_set_sigma(Operator& op, const zd* d_csiga);
 // out = A^H in (FP64), from the reciprocity of the contraction kernel
 void op_apply_adjoint(Operator& op, const zd* vin, zd* vout, hipStream_t s);

@@ -114,6 +114,15 @@ class IEOperator:
     def apply_dev_f32(self, vin, vout):
         L.check(self.lib.giem2g_op_apply_dev_f32(self.h, _ptr(vin), _ptr(vout)))
 
+    def apply_batch_dev(self, vin, vout):
+        """vout[r] = A vin[r] for the rows of two (nrhs, 3 Nx Ny Nz) torch
+        tensors on the GPU (complex128 or complex64); one tensor read serves up
+        to 8 complex64 or 4 complex128 rows (Nz = 16, 32) or 2 (Nz = 48, 64)."""
+        import torch
+        assert vin.dim() == 2 and vin.shape == vout.shape and vin.is_contiguous() and vout.is_contiguous()
+        prec = 1 if vin.dtype == torch.complex64 else 0
+        L.check(self.lib.giem2g_op_apply_batch_dev(self.h, prec, int(vin.shape[0]), _ptr(vin), _ptr(vout)))
+
     def apply_pair_dev(self, in0, in1, out0, out1):
         """out0 = A in0, out1 = A in1 with one tensor read (torch tensors on
         the GPU, all complex128 or all complex64; Nz = 16, 32, 48 or 64)."""

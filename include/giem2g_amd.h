@@ -144,6 +144,14 @@ int giem2g_op_apply_dev_f32(void *ie_op, const void *in_dev, void *out_dev);
  * to one vector at a time (apply_ie_operator_module.F90:39-96). */
 int giem2g_op_apply_pair_dev(void *ie_op, int precision, const void *in0_dev, const void *in1_dev,
                              void *out0_dev, void *out1_dev);
+/* out[r] = A*in[r] for nrhs vectors stored one after another (3*Nx*Ny*Nz
+ * complex each, complex128 for precision 0, complex64 for 1).  One tensor
+ * read serves up to 8 complex64 or 4 complex128 vectors at Nz = 16 or 32 and
+ * 2 at Nz = 48 or 64 (the matrix-core contraction with one 16-column operand
+ * per two vectors);
+ * other Nz apply the vectors one by one.  No reference counterpart (the
+ * reference applies the operator to one vector at a time). */
+int giem2g_op_apply_batch_dev(void *ie_op, int precision, int nrhs, const void *in_dev, void *out_dev);
 /* Same on host pointers (PCIe copies included). */
 int giem2g_op_apply_host(void *ie_op, const double *in, double *out);
 /* Stream used by the operator (hipStream_t) and device synchronisation. */
