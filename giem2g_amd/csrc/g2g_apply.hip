@@ -25,13 +25,15 @@ namespace g2g {
 
 constexpr int NT = 256;  // threads per block for every apply kernel
 
-// Rows per FFT block: about 32 KB of LDS per block (four blocks per CU),
-// between 1 and 64 rows.  Measured at 512-point rows: 64 KB blocks (two per
-// CU) were 17-30% slower.
+// Rows per FFT block: at most 32 KB of LDS per block (four blocks per CU)
+// and at most 8 rows.  Measured: 64 KB blocks (two per CU) were 17-30%
+// slower; at 256-point complex64 rows 8 rows per block were 9-26% faster
+// than 16; 4 rows (32 B contiguous per transposed row segment in complex64,
+// 64 B in FP64) were 9-29% slower than 8.
 template <typename R> constexpr int nr_for(int L) {
   constexpr int kBytes = 32 * 1024;
   const int n = kBytes / (L * 2 * (int)sizeof(R));
-  return n < 1 ? 1 : (n > 64 ? 64 : n);
+  return n < 1 ? 1 : (n > 8 ? 8 : n);
 }
 template <typename R> int nr_rt(int L) { return nr_for<R>(L); }
 
