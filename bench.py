@@ -93,7 +93,8 @@ def solve_bench(torch, gop, models):
     """Time to a 1e-8 FP64 true residual on the 256x256x32 random-box model
     (0.1 Hz, EY polarization): plain FP64 nested FGMRES, complex64 inner, and
     complex64 inner with the learned preconditioner when its trained weights
-    (giem2g_amd/data/unet_c4.f32) are present."""
+    (giem2g_amd/data/unet_c4.f32) are present; then both polarizations, one
+    after the other and paired, for the plain FP64 and complex64-inner solves."""
     from giem2g_amd import learned
     m = models.c2()
     t0 = time.perf_counter()
@@ -137,6 +138,28 @@ def solve_bench(torch, gop, models):
                           "accept": LEARNED_ACCEPT}
         op.set_learned(None)
         net.close()
+    # both polarizations of the model: the EX solve alone (with the EY solve
+    # above: one after the other), then EY and EX together with paired
+    # operator products (one tensor read per pair of applies)
+    Ex = torch.from_numpy(op.plane_wave(0)).cuda()
+    both = {}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, ix = op.solve(Ex, outer_m=30, inner_m=30, max_restart=100, tol=1e-8, inner_precision=prec)
+        torch.cuda.synchronize()
+        t_x = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        _, _, jy, jx, cnt = op.solve_pair(E, Ex, outer_m=30, inner_m=30, max_restart=100, tol=1e-8,
+                                          inner_precision=prec)
+        torch.cuda.synchronize()
+        t_p = time.perf_counter() - t0
+        both[name] = {"one_after_other_s": res[name]["time_to_1e-8_s"] + t_x, "paired_s": t_p,
+                      "paired_applies": int(cnt[0]), "single_applies": int(cnt[1]),
+                      "true_residual": max(jy["true_residual"], jx["true_residual"], ix["true_residual"])}
+        log("solve both polarizations", name, both[name])
+    both["mixed"]["speedup_vs_fp64_paired"] = both["fp64"]["paired_s"] / both["mixed"]["paired_s"]
+    res["both_polarizations"] = both
     op.close()
     return res
 
