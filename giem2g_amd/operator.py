@@ -177,8 +177,9 @@ class IEOperator:
         L.check(self.lib.giem2g_op_scale(self.h, mode, _ptr(vin), _ptr(vout)))
 
     def set_contraction(self, mode):
-        """0 automatic, 1 vector ALU (two components in flight), 2 matrix cores
-        (MFMA), 3 vector ALU (one component in flight)."""
+        """0 automatic (2 for Nz = 16, 32, 48, 64, otherwise 1), 1 vector ALU
+        (two components in flight), 2 matrix cores (MFMA), 3 vector ALU (one
+        component in flight)."""
         L.check(self.lib.giem2g_op_set_contraction(self.h, mode))
 
     def set_learned(self, net, accept=0.9):

@@ -315,6 +315,27 @@ def test_error_paths():
     op = gop.IEOperator(8, 8, 2, 100.0, 100.0, [0, 10, 20])
     with pytest.raises(L.G2GError, match="not ready"):
         op.apply_host(np.zeros(3 * 128, np.complex128))
+    # matrix-core contraction and two-vector applies need Nz = 16, 32, 48 or 64
+    with pytest.raises(L.G2GError, match="Nz = 16, 32, 48 or 64"):
+        op.set_contraction(2)
+    with pytest.raises(L.G2GError, match="contraction mode"):
+        op.set_contraction(5)
+    op.close()
+    import torch
+    from giem2g_amd import models
+    m = models.small(8, 8, 12, seed=2)
+    op = gop.operator_for(m)
+    v = torch.zeros(2, 3 * m.ncell, dtype=torch.complex128, device="cuda")
+    o = torch.empty_like(v)
+    with pytest.raises(L.G2GError, match="two-vector apply needs"):
+        op.apply_pair_dev(v[0], v[1], o[0], o[1])
+    with pytest.raises(L.G2GError, match="nrhs"):
+        L.check(op.lib.giem2g_op_apply_batch_dev(op.h, 0, 0, v.data_ptr(), o.data_ptr()))
+    with pytest.raises(L.G2GError, match="precision"):
+        L.check(op.lib.giem2g_op_apply_batch_dev(op.h, 2, 2, v.data_ptr(), o.data_ptr()))
+    op.apply_batch_dev(v, o)  # Nz = 12: one vector at a time
+    op.sync()
+    assert torch.count_nonzero(o) == 0
     op.close()
 
 
