@@ -1069,7 +1069,10 @@ static void op_apply_k(Operator& op, OpBuffers<R>& b, int k, const cx_t<R>* cons
     case 1: ok = launch_contract3<R, 1>(op, b.G, b.SB, nS, g, s); break;
     case 2: ok = launch_contract3<R, 2>(op, b.G, b.SB, nS, g, s); break;
     case 4: ok = launch_contract3<R, 4>(op, b.G, b.SB, nS, g, s); break;
-    default: ok = launch_contract3<R, 8>(op, b.G, b.SB, nS, g, s); break;
+    default:
+      // eight per read in complex64 only (op_apply_batch caps FP64 at four)
+      if constexpr (sizeof(R) == 4) ok = launch_contract3<R, 8>(op, b.G, b.SB, nS, g, s);
+      break;
   }
   G2G_CHECK(ok, -3, "batched contraction not available for this Nz");
   for (int r = 0; r < k; r++) {
@@ -1107,8 +1110,8 @@ void op_apply_batch(Operator& op, OpBuffers<R>& b, int nrhs, const cx_t<R>* vin,
   G2G_CHECK(op.have_tensor && op.have_sigma, -4, "operator not ready: tensor or conductivity missing");
   G2G_CHECK(nrhs >= 1, -3, "batched apply needs at least one vector");
   const int64_t nv = 3 * (int64_t)op.ncell();
-  // FP64 takes at most four per tensor read: the eight-vector FP64 kernel
-  // holds 360 registers per lane (one wave per SIMD) and measured 22% slower
+  // FP64 takes at most four per tensor read: an eight-vector FP64 kernel
+  // held 360 registers per lane (one wave per SIMD) and measured 22% slower
   // per vector than four at 128x128x32
   const int kmax = std::min(op_batch_max(op), sizeof(R) == 8 ? 4 : 8);
   for (int r0 = 0; r0 < nrhs;) {
