@@ -38,6 +38,7 @@ template <typename R> struct OpBuffers {
   T2* twx = nullptr;    // 2Nx twiddles
   T2* twy = nullptr;    // 2Ny twiddles
   int NR = 0, nchunk = 0;
+  bool half = false;    // half-length FFT passes
   size_t bytes = 0;
 };
 
