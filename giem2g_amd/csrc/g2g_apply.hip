@@ -37,6 +37,11 @@ template <typename R> constexpr int nr_for(int L) {
 }
 template <typename R> int nr_rt(int L) { return nr_for<R>(L); }
 template <typename R> int nr_half_rt(int L) { return nr_for<R>(L / 2); }
+// Nz for which the matrix-core contraction of nrhs right-hand sides exists:
+// up to 64 layers for one or two, up to 32 for four or eight; at least C3_MIN_NZ
+// (a layer count that is not a multiple of 16 is padded to the next one)
+constexpr int C3_MIN_NZ = 8;
+inline bool c3_supported(int nz, int nrhs) { return nz >= C3_MIN_NZ && nz <= (nrhs <= 2 ? 64 : 32); }
 // Half-length passes (two N-point transforms per zero-padded 2N-point one)
 // for rows of 32 points or more.  Measured against the 2N-point passes: at
 // 256x256 (512-point rows) the FP64 and complex64 passes together 15% faster,
@@ -817,9 +822,9 @@ struct C3Cfg {
   static constexpr size_t smem = vin_bytes + (stage_bytes > red_bytes ? stage_bytes : red_bytes);
 };
 
-template <typename R, int NZ, int NRHS, int DEEP, int NW>
-__global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S, int64_t sst,
-                                                   const cx_t<R>* __restrict__ G, Geo g, int64_t BS) {
+template <typename R, int NZ, int NRHS, int DEEP, int NW, bool PAD>
+__device__ __forceinline__ void contract3_body(cx_t<R>* __restrict__ S, int64_t sst,
+                                               const cx_t<R>* __restrict__ G, const Geo& g, int64_t BS) {
   using T2 = cx_t<R>;
   using V4 = typename v4t<R>::t;
   using C = C3Cfg<R, NZ, NRHS, NW>;
@@ -830,8 +835,18 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S, 
   static_assert(NRHS == 1 || NRHS == 2 || NRHS == 4 || NRHS == 8, "1, 2, 4 or 8 right-hand sides");
   static_assert(DEEP == 1 || DEEP == 2, "one or two components in flight");
   extern __shared__ unsigned char smem_raw[];
-  T2* vin = reinterpret_cast<T2*>(smem_raw);  // [NRHS][4][3Nz], point-signed inputs; rhs r at S + r*sst
+  T2* vin = reinterpret_cast<T2*>(smem_raw);  // [NRHS][4][3NZ], point-signed inputs; rhs r at S + r*sst
   T2* gbuf = vin + NRHS * 4 * NZ3;            // [2][GBS]
+  // PAD: the operator has nz < NZ layers; rows and columns nz..NZ-1 of every
+  // tile are zero (zero point operand, zeroed or masked tensor entries, rows
+  // not stored)
+  const int nz = PAD ? g.Nz : NZ, nz3 = 3 * nz;
+  const int Pn = PAD ? nz * (nz + 1) / 2 : P, Fn = PAD ? nz * nz : FULL;
+  // x / nz and x / 3nz as a multiply and a shift: exact for x * divisor < 2^20
+  // (here x < 3072 and divisor <= 192)
+  const uint32_t mnz = PAD ? ((1u << 20) + nz - 1) / nz : 0, mnz3 = PAD ? ((1u << 20) + nz3 - 1) / nz3 : 0;
+#define C3_DIVNZ(x) (PAD ? (int)(((uint32_t)(x) * mnz) >> 20) : (x) / NZ)
+#define C3_DIVNZ3(x) (PAD ? (int)(((uint32_t)(x) * mnz3) >> 20) : (x) / NZ3)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   // blocks walk ky fastest: concurrent blocks touch neighbouring spectrum
   // rows (measured 2-3% faster at 256x256 than kx fastest)
@@ -853,15 +868,15 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S, 
 #define C3_LDG(q) if ((q) < PF) rg##q = ld_stream(src_ + min(tid + (q) * NTH, n_ - 1));
 #define C3_PREFETCH(c, SET)                                                           \
   {                                                                                   \
-    const T2* src_ = (c) < 4 ? blk + (c) * P : blk + 4 * P + ((c) - 4) * FULL;        \
-    const int n_ = (c) < 4 ? P : FULL;                                                \
+    const T2* src_ = (c) < 4 ? blk + (c) * Pn : blk + 4 * Pn + ((c) - 4) * Fn;        \
+    const int n_ = (c) < 4 ? Pn : Fn;                                                 \
     G2G_FOR16(C3_LD##SET)                                                             \
   }
   // packed symmetric components as stored, full ones with column stride SF
 #define C3_STX(q, r)                                                 \
   if ((q) < PF && tid + (q) * NTH < n_) {                            \
-    const int e = tid + (q) * NTH;                                   \
-    dst_[c_ < 4 ? e : (e / NZ) * SF + e % NZ] = r##q;                \
+    const int e = tid + (q) * NTH, col = C3_DIVNZ(e);                \
+    dst_[c_ < 4 ? e : col * SF + e - col * nz] = r##q;               \
   }
 #define C3_STF(q) C3_STX(q, rf)
 #define C3_STG(q) C3_STX(q, rg)
@@ -869,7 +884,7 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S, 
   {                                    \
     const int c_ = (c);                \
     T2* dst_ = (dst);                  \
-    const int n_ = c_ < 4 ? P : FULL;  \
+    const int n_ = c_ < 4 ? Pn : Fn;   \
     G2G_FOR16(C3_ST##SET)              \
   }
   // the point rows: complex64 issues them first as unrolled clamped loads
@@ -882,17 +897,18 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S, 
 #define C3_VQD(k) T2 vq##k = zero;
   G2G_FOR16(C3_VQD)
 #undef C3_VQD
-#define C3_VQL(k)                                                                \
-  if ((k) < NVQ) {                                                               \
-    const int i = min(tid + (k) * NTH, NV - 1);                                  \
-    const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3;                        \
-    vq##k = S[(rp >> 2) * sst + ((int64_t)PX(p) * Ny2 + PY(p)) * NZ3 + e];       \
+  // padded slot i of a point row: component e / NZ, layer e % NZ
+#define C3_VQL(k)                                                                          \
+  if ((k) < NVQ) {                                                                         \
+    const int i = min(tid + (k) * NTH, NV - 1);                                            \
+    const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3, ci = e / NZ, z = min(e % NZ, nz - 1); \
+    vq##k = S[(rp >> 2) * sst + ((int64_t)PX(p) * Ny2 + PY(p)) * nz3 + ci * nz + z];       \
   }
-#define C3_VQP(k)                                                                \
+#define C3_VQP(k)                                                                  \
   if ((k) < NVQ && tid + (k) * NTH < NV) {                                       \
     const int i = tid + (k) * NTH;                                               \
     const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3;                        \
-    T2 vv = VAL(p) ? vq##k : zero;                                               \
+    T2 vv = (VAL(p) && e % NZ < nz) ? vq##k : zero;                              \
     /* D_p: x times s (-1 for -kx), y times m (-1 for -ky) */                    \
     if ((e < NZ && (p & 1)) || (e >= NZ && e < 2 * NZ && (p & 2))) vv = -vv;     \
     vin[i] = vv;                                                                 \
@@ -906,13 +922,18 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S, 
     G2G_FOR16(C3_VQP)
   } else {
     for (int i = tid; i < NV; i += NTH) {
-      const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3;
+      const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3, ci = e / NZ, z = e % NZ;
       const T2* Sr = S + (rp >> 2) * sst;
       T2 vv = zero;
-      if (VAL(p)) vv = Sr[((int64_t)PX(p) * Ny2 + PY(p)) * NZ3 + e];
+      if (VAL(p) && z < nz) vv = Sr[((int64_t)PX(p) * Ny2 + PY(p)) * nz3 + ci * nz + z];
       if ((e < NZ && (p & 1)) || (e >= NZ && e < 2 * NZ && (p & 2))) vv = -vv;
       vin[i] = vv;
     }
+  }
+  if constexpr (PAD) {
+    // packed entries Pn..P-1 of both buffers (rows or columns past nz of the
+    // symmetric components) stay zero: their stashes write 0..Pn-1 only
+    for (int i = Pn + tid; i < P; i += NTH) gbuf[i] = gbuf[GBS + i] = zero;
   }
   C3_STASH(0, F, gbuf)
   __syncthreads();
@@ -964,7 +985,8 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S, 
       for (int t = 0; t < NT16; t++) {
         const int r = 16 * t + rr;
         if (c < 4) {
-          // symmetric, packed by rows of the lower triangle
+          // symmetric, packed by rows of the lower triangle (PAD: columns
+          // j >= nz read the zeroed entries, rows r >= nz are not stored)
           const T2 a = cur[j <= r ? j + r * (r + 1) / 2 : r + j * (j + 1) / 2];
           // xx: x <- x; xy: x <- y (and y <- x below); yy: y <- y; zz: z <- z
           constexpr int co = c <= 1 ? 0 : (c == 2 ? 1 : 2);
@@ -980,8 +1002,9 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S, 
         } else {
           // A (xz or yz) at column j, row r; its use in z is -A^T
           constexpr int co = c - 4;
-          const T2 a = cur[j * SF + r];
-          const T2 at = cur[r * SF + j];
+          T2 a = cur[j * SF + r];
+          T2 at = cur[r * SF + j];
+          if (PAD && j >= nz) a = at = zero;
 #pragma unroll
           for (int q = 0; q < NG; q++) {
             acc[q][co][t] = mma4(a.x, bR[q][ciA][s], acc[q][co][t]);
@@ -1051,15 +1074,17 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S, 
     }
   }
   __syncthreads();
-  for (int i = tid; i < NRHS * 4 * NZ3; i += NTH) {
-    const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3;
+  for (int i = tid; i < NRHS * 4 * nz3; i += NTH) {
+    const int rp = C3_DIVNZ3(i), e = i - rp * nz3, p = rp & 3, ci = C3_DIVNZ(e), z = e - ci * nz;
     T2* Sr = S + (rp >> 2) * sst;
-    T2 o = vin[i];
+    T2 o = vin[rp * NZ3 + ci * NZ + z];
     if (!VAL(p)) o = zero;
-    Sr[((int64_t)PX(p) * Ny2 + PY(p)) * NZ3 + e] = o;
+    Sr[((int64_t)PX(p) * Ny2 + PY(p)) * nz3 + e] = o;
   }
 #undef C3_VQL
 #undef C3_VQP
+#undef C3_DIVNZ
+#undef C3_DIVNZ3
 #undef C3_LDF
 #undef C3_LDG
 #undef C3_PREFETCH
@@ -1071,11 +1096,24 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S, 
 #undef PY
 #undef VAL
 }
+// Nz a multiple of 16 (NZ = Nz)
+template <typename R, int NZ, int NRHS, int DEEP, int NW>
+__global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S, int64_t sst,
+                                                   const cx_t<R>* __restrict__ G, Geo g, int64_t BS) {
+  contract3_body<R, NZ, NRHS, DEEP, NW, false>(S, sst, G, g, BS);
+}
+// other Nz from 8 to 64: the tiles of the next multiple of 16, zero-padded
+template <typename R, int NZ, int NRHS, int DEEP, int NW>
+__global__ __launch_bounds__(64 * NW) void k_contract3_pad(cx_t<R>* __restrict__ S, int64_t sst,
+                                                       const cx_t<R>* __restrict__ G, Geo g, int64_t BS) {
+  contract3_body<R, NZ, NRHS, DEEP, NW, true>(S, sst, G, g, BS);
+}
 
 // ------------------------------------------------------------ dispatch ---
 // The layer-split matrix-core contraction of NRHS spectra (rhs r at S + r*sst)
-// through one tensor read: Nz = 16, 32, 48 or 64 for one or two, Nz = 16 or 32
-// for four or eight (register budget); false for other cases.
+// through one tensor read: one or two for Nz <= 64, four or eight for
+// Nz <= 32 (register budget).  Nz that is not a multiple of 16 runs the next
+// multiple's kernel with zero-padded tiles.  False for other cases.
 template <typename R, int NRHS>
 static bool launch_contract3(const Operator& op, const cx_t<R>* G, cx_t<R>* S, int64_t sst, const Geo& g,
                              hipStream_t s) {
@@ -1093,21 +1131,36 @@ static bool launch_contract3(const Operator& op, const cx_t<R>* G, cx_t<R>* S, i
   // FP64 components in flight at four waves per SIMD, measured within 1%)
   constexpr bool F64 = sizeof(R) == 8;
   constexpr int DP = F64 ? 1 : 2, W64 = F64 ? 8 : 4;
-  if constexpr (NRHS <= 2) {
-    switch (op.Nz) {
-      case 16: go(k_contract3<R, 16, NRHS, 2, 4>, C3Cfg<R, 16, NRHS, 4>::smem, 4); return true;
-      case 32: go(k_contract3<R, 32, NRHS, DP, 4>, C3Cfg<R, 32, NRHS, 4>::smem, 4); return true;
-      case 48: go(k_contract3<R, 48, NRHS, 2, 4>, C3Cfg<R, 48, NRHS, 4>::smem, 4); return true;
-      case 64: go(k_contract3<R, 64, NRHS, DP, W64>, C3Cfg<R, 64, NRHS, W64>::smem, W64); return true;
-      default: return false;
-    }
-  } else {
-    switch (op.Nz) {
-      case 16: go(k_contract3<R, 16, NRHS, 2, 4>, C3Cfg<R, 16, NRHS, 4>::smem, 4); return true;
-      case 32: go(k_contract3<R, 32, NRHS, DP, 4>, C3Cfg<R, 32, NRHS, 4>::smem, 4); return true;
-      default: return false;
-    }
+  if (!c3_supported(op.Nz, NRHS)) return false;
+  const int nzp = (op.Nz + 15) / 16 * 16;
+  const bool pad = nzp != op.Nz;
+  auto run = [&](auto ZC, auto DC, auto WC) {
+    constexpr int Z = decltype(ZC)::value, D = decltype(DC)::value, W = decltype(WC)::value;
+    if (pad) go(k_contract3_pad<R, Z, NRHS, D, W>, C3Cfg<R, Z, NRHS, W>::smem, W);
+    else go(k_contract3<R, Z, NRHS, D, W>, C3Cfg<R, Z, NRHS, W>::smem, W);
+  };
+  using std::integral_constant;
+  switch (nzp) {
+    case 16: run(integral_constant<int, 16>{}, integral_constant<int, 2>{}, integral_constant<int, 4>{}); break;
+    case 32: run(integral_constant<int, 32>{}, integral_constant<int, DP>{}, integral_constant<int, 4>{}); break;
+    default:
+      if constexpr (NRHS <= 2) {
+        if (nzp == 48) run(integral_constant<int, 48>{}, integral_constant<int, 2>{}, integral_constant<int, 4>{});
+        else run(integral_constant<int, 64>{}, integral_constant<int, DP>{}, integral_constant<int, W64>{});
+      }
+      break;
   }
+  return true;
+}
+// Automatic kernel for one vector: the matrix cores, except complex64 at
+// padded layer counts where the vector-ALU kernel measured faster (128x128,
+// contraction alone: Nz = 20, 24, 50, 56 by 12-19%; the padded tiles won at
+// Nz = 8, 12, 28, 40 and in FP64 at every Nz measured), i.e. where padding
+// adds more than half the tile work or pads to 64 layers.
+template <typename R> bool c3_auto(int nz) {
+  const int nzp = (nz + 15) / 16 * 16;
+  if (sizeof(R) == 8 || nzp == nz || nzp == 16) return true;
+  return nzp < 64 && 2 * nzp * nzp <= 3 * nz * nz;
 }
 template <typename R, template <typename, int> class KF> struct Dummy {};
 
@@ -1239,9 +1292,11 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
       }
       hipLaunchKernelGGL(kern, dim3(op.Nx * op.Ny), dim3(NT), sm, s, b.S, b.G, g, op.BS, split_log);
     };
-    // matrix cores for Nz = 16, 32, 48 and 64 unless a vector-ALU kernel is
-    // requested (FP64 at 128x128x32: 0.28-0.30 ms against 0.33 ms)
-    if ((op.contraction == 0 || op.contraction == 2) && launch_contract3<R, 1>(op, b.G, b.S, 0, g, s)) {
+    // matrix cores for Nz from 8 to 64 (complex64: where c3_auto says so)
+    // unless a vector-ALU kernel is requested (FP64 at 128x128x32:
+    // 0.28-0.30 ms against 0.33 ms)
+    if ((op.contraction == 2 || (op.contraction == 0 && c3_auto<R>(op.Nz))) &&
+        launch_contract3<R, 1>(op, b.G, b.S, 0, g, s)) {
     } else if (op.contraction == 3) {
       if (op.Nz * op.Nz <= 4 * NT) launch(k_contract<R, 4>);
       else launch(k_contract<R, 16>);
@@ -1272,11 +1327,9 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
   G2G_HIP(hipGetLastError());
 }
 
-bool op_pair_supported(const Operator& op) {
-  return op.Nz == 16 || op.Nz == 32 || op.Nz == 48 || op.Nz == 64;
-}
+bool op_pair_supported(const Operator& op) { return c3_supported(op.Nz, 2); }
 int op_batch_max(const Operator& op) {
-  return (op.Nz == 16 || op.Nz == 32) ? 8 : (op_pair_supported(op) ? 2 : 1);
+  return c3_supported(op.Nz, 8) ? 8 : (c3_supported(op.Nz, 2) ? 2 : 1);
 }
 
 template <typename T2> __global__ void k_zero(T2* __restrict__ a, int64_t n) {
@@ -1333,7 +1386,7 @@ template <typename R>
 void op_apply2(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin0, const cx_t<R>* vin1, cx_t<R>* vout0,
                cx_t<R>* vout1, hipStream_t s) {
   G2G_CHECK(op.have_tensor && op.have_sigma, -4, "operator not ready: tensor or conductivity missing");
-  G2G_CHECK(op_pair_supported(op), -3, "two-vector apply needs Nz = 16, 32, 48 or 64");
+  G2G_CHECK(op_pair_supported(op), -3, "two-vector apply needs Nz from 8 to 64");
   const cx_t<R>* vins[2] = {vin0, vin1};
   cx_t<R>* vouts[2] = {vout0, vout1};
   op_apply_k<R>(op, b, 2, vins, vouts, s);

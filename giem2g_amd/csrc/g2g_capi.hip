@@ -706,7 +706,7 @@ int giem2g_op_set_contraction(void* p, int mode) {
     Handle* h = H(p);
     G2G_CHECK(mode >= 0 && mode <= 3, -3, "contraction mode must be 0, 1, 2 or 3");
     G2G_CHECK(mode != 2 || op_pair_supported(h->op), -3,
-              "the matrix-core contraction needs Nz = 16, 32, 48 or 64");
+              "the matrix-core contraction needs Nz from 8 to 64");
     h->op.contraction = mode;
   });
 }

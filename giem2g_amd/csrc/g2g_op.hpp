@@ -126,9 +126,9 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
 template <typename R>
 void op_apply2(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin0, const cx_t<R>* vin1, cx_t<R>* vout0,
                cx_t<R>* vout1, hipStream_t s);
-// Nz for which the matrix-core contraction exists (16, 32, 48, 64)
+// Nz for which the matrix-core contraction exists (8 to 64; padded to a multiple of 16)
 bool op_pair_supported(const Operator& op);
-// right-hand sides one tensor read serves: 8 (Nz = 16, 32), 2 (Nz = 48, 64), else 1
+// right-hand sides one tensor read serves: 8 (Nz 8 to 32), 2 (Nz 33 to 64), else 1
 int op_batch_max(const Operator& op);
 // nrhs vectors stored one after another, batches of op_batch_max (FP64: at
 // most 4) per tensor read

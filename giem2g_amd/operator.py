@@ -117,7 +117,7 @@ class IEOperator:
     def apply_batch_dev(self, vin, vout):
         """vout[r] = A vin[r] for the rows of two (nrhs, 3 Nx Ny Nz) torch
         tensors on the GPU (complex128 or complex64); one tensor read serves up
-        to 8 complex64 or 4 complex128 rows (Nz = 16, 32) or 2 (Nz = 48, 64)."""
+        to 8 complex64 or 4 complex128 rows (Nz 8 to 32) or 2 (Nz 33 to 64)."""
         import torch
         assert vin.dim() == 2 and vin.shape == vout.shape and vin.is_contiguous() and vout.is_contiguous()
         prec = 1 if vin.dtype == torch.complex64 else 0
@@ -125,7 +125,7 @@ class IEOperator:
 
     def apply_pair_dev(self, in0, in1, out0, out1):
         """out0 = A in0, out1 = A in1 with one tensor read (torch tensors on
-        the GPU, all complex128 or all complex64; Nz = 16, 32, 48 or 64)."""
+        the GPU, all complex128 or all complex64; Nz from 8 to 64)."""
         import torch
         prec = 1 if in0.dtype == torch.complex64 else 0
         L.check(self.lib.giem2g_op_apply_pair_dev(self.h, prec, _ptr(in0), _ptr(in1), _ptr(out0),
@@ -177,7 +177,8 @@ class IEOperator:
         L.check(self.lib.giem2g_op_scale(self.h, mode, _ptr(vin), _ptr(vout)))
 
     def set_contraction(self, mode):
-        """0 automatic (2 for Nz = 16, 32, 48, 64, otherwise 1), 1 vector ALU
+        """0 automatic (2 for Nz from 8 to 64, otherwise 1; complex64 at
+        Nz = 17-26, 33-39, 49-63 uses 1), 1 vector ALU
         (two components in flight), 2 matrix cores (MFMA), 3 vector ALU (one
         component in flight)."""
         L.check(self.lib.giem2g_op_set_contraction(self.h, mode))

@@ -139,17 +139,17 @@ int giem2g_op_set_sigma_dev(void *ie_op, const void *csiga_dev);
 int giem2g_op_apply_dev(void *ie_op, const void *in_dev, void *out_dev);
 int giem2g_op_apply_dev_f32(void *ie_op, const void *in_dev, void *out_dev);
 /* out0 = A*in0 and out1 = A*in1 with one read of the tensor (two-column
- * matrix-core contraction; needs Nz = 16, 32, 48 or 64); precision 0 complex128,
+ * matrix-core contraction; needs Nz from 8 to 64); precision 0 complex128,
  * 1 complex64.  No reference counterpart: the reference applies the operator
  * to one vector at a time (apply_ie_operator_module.F90:39-96). */
 int giem2g_op_apply_pair_dev(void *ie_op, int precision, const void *in0_dev, const void *in1_dev,
                              void *out0_dev, void *out1_dev);
 /* out[r] = A*in[r] for nrhs vectors stored one after another (3*Nx*Ny*Nz
  * complex each, complex128 for precision 0, complex64 for 1).  One tensor
- * read serves up to 8 complex64 or 4 complex128 vectors at Nz = 16 or 32 and
- * 2 at Nz = 48 or 64 (the matrix-core contraction with one 16-column operand
- * per two vectors);
- * other Nz apply the vectors one by one.  No reference counterpart (the
+ * read serves up to 8 complex64 or 4 complex128 vectors at Nz 8 to 32 and
+ * 2 at Nz 33 to 64 (the matrix-core contraction with one 16-column operand
+ * per two vectors; a layer count that is not a multiple of 16 is padded with
+ * zero layers inside the kernel); Nz below 8 applies the vectors one by one.  No reference counterpart (the
  * reference applies the operator to one vector at a time). */
 int giem2g_op_apply_batch_dev(void *ie_op, int precision, int nrhs, const void *in_dev, void *out_dev);
 /* Same on host pointers (PCIe copies included). */
@@ -219,8 +219,10 @@ int giem2g_solve_dev(void *ie_op, const void *E_bkg_dev, const void *E0_dev,
                      const giem2g_solver_params *p, void *E_sol_dev,
                      giem2g_solve_info *info);
 /* Kernel of the per-(kx,ky) tensor contraction of one vector: 0 automatic
- * (2 for Nz = 16, 32, 48 or 64, otherwise 1), 1 vector ALU with two tensor
- * components in flight, 2 matrix cores (MFMA; Nz = 16, 32, 48 or 64), 3
+ * (2 for Nz from 8 to 64, otherwise 1; complex64 products at Nz = 17-26,
+ * 33-39 and 49-63 use 1, measured faster there than the zero-padded
+ * matrix-core tiles), 1 vector ALU with two tensor components in flight,
+ * 2 matrix cores (MFMA; Nz from 8 to 64, padded to a multiple of 16), 3
  * vector ALU with one component in flight.  All compute the same product.
  * Two-vector products (giem2g_op_apply_pair_dev, giem2g_solve_pair_dev) use
  * the matrix cores. */
@@ -235,7 +237,7 @@ int giem2g_op_scale(void *ie_op, int mode, const void *in_dev, void *out_dev);
 /* Two independent solves with the same operator (e.g. the EY and EX
  * polarizations of one frequency) run together: their operator products
  * are paired so that one two-vector apply reads the tensor once for both
- * (needs Nz = 16, 32, 48 or 64, otherwise the products run one by one).  HBM
+ * (needs Nz from 8 to 64, otherwise the products run one by one).  HBM
  * pointers; E0x may be NULL; info2 receives two records (seconds = wall
  * time of the pair); paired (nullable) receives [two-vector applies,
  * one-vector applies].  precond must be 0.  No reference counterpart: the

@@ -49,11 +49,13 @@ def test_apply_matches_oracle_on_oracle_tensor(oracle, Nx, Ny, Nz):
     op.close()
 
 
-@pytest.mark.parametrize("Nx,Ny,Nz", [(16, 16, 16), (8, 16, 32), (8, 8, 48), (8, 8, 64)])
+@pytest.mark.parametrize("Nx,Ny,Nz", [(16, 16, 16), (8, 16, 32), (8, 8, 48), (8, 8, 64),
+                                      (8, 8, 8), (8, 8, 12), (16, 8, 24), (8, 8, 50)])
 def test_matrix_core_contraction_matches_oracle(oracle, Nx, Ny, Nz):
-    """Nz = 16, 32, 48, 64: the MFMA contraction (mode 2, the default) and
-    the vector-ALU one (mode 1) against the oracle, FP64 at 1e-12; complex64
-    within 2e-6."""
+    """Nz = 16, 32, 48, 64 and zero-padded layer counts (8, 12, 24, 50): the
+    MFMA contraction (mode 2), the vector-ALU one (mode 1)
+    and the automatic choice (mode 0) against the oracle, FP64 at 1e-12;
+    complex64 within 2e-6."""
     torch = _torch()
     from giem2g_amd import models, operator as gop
     print("case", Nx, Ny, Nz, flush=True)
@@ -66,7 +68,7 @@ def test_matrix_core_contraction_matches_oracle(oracle, Nx, Ny, Nz):
     op.set_background(m.csigb(), m.thick, m.omega)
     op.put_tensor(Gs, spectral=True)
     op.set_sigma(m.csiga())
-    for mode in (2, 1):
+    for mode in (2, 1, 0):
         op.set_contraction(mode)
         assert rel(op.apply_host(v), ref) < 1e-12, mode
         vt = torch.from_numpy(v).cuda().to(torch.complex64)
@@ -77,7 +79,8 @@ def test_matrix_core_contraction_matches_oracle(oracle, Nx, Ny, Nz):
     op.close()
 
 
-@pytest.mark.parametrize("Nx,Ny,Nz", [(16, 8, 16), (8, 16, 32), (8, 8, 48), (8, 8, 64)])
+@pytest.mark.parametrize("Nx,Ny,Nz", [(16, 8, 16), (8, 16, 32), (8, 8, 48), (8, 8, 64),
+                                      (8, 8, 12), (8, 16, 50)])
 def test_paired_apply_matches_oracle(oracle, Nx, Ny, Nz):
     """Two vectors through one tensor read (giem2g_op_apply_pair_dev): each
     result against the oracle, FP64 at 1e-12 and complex64 within 2e-6, and
@@ -114,10 +117,11 @@ def test_paired_apply_matches_oracle(oracle, Nx, Ny, Nz):
 
 
 @pytest.mark.parametrize("Nx,Ny,Nz,nrhs", [(8, 8, 16, 8), (16, 8, 32, 8), (8, 16, 32, 5), (8, 8, 32, 3),
-                                            (8, 8, 64, 3), (8, 8, 12, 2)])
+                                            (8, 8, 64, 3), (8, 8, 12, 2), (8, 8, 24, 8), (8, 8, 50, 3),
+                                            (8, 8, 4, 3)])
 def test_batched_apply_matches_oracle(oracle, Nx, Ny, Nz, nrhs):
     """nrhs vectors through giem2g_op_apply_batch_dev (one tensor read for up
-    to 8 at Nz = 16, 32; pairs at Nz = 64; one by one at other Nz): every
+    to 8 at Nz 8 to 32; pairs at Nz 33 to 64; one by one below 8): every
     row against the oracle, FP64 at 1e-12 and complex64 within 2e-6, and the
     FP64 rows against one-vector applies at 1e-14."""
     torch = _torch()
@@ -315,15 +319,15 @@ def test_error_paths():
     op = gop.IEOperator(8, 8, 2, 100.0, 100.0, [0, 10, 20])
     with pytest.raises(L.G2GError, match="not ready"):
         op.apply_host(np.zeros(3 * 128, np.complex128))
-    # matrix-core contraction and two-vector applies need Nz = 16, 32, 48 or 64
-    with pytest.raises(L.G2GError, match="Nz = 16, 32, 48 or 64"):
+    # matrix-core contraction and two-vector applies need Nz from 8 to 64
+    with pytest.raises(L.G2GError, match="Nz from 8 to 64"):
         op.set_contraction(2)
     with pytest.raises(L.G2GError, match="contraction mode"):
         op.set_contraction(5)
     op.close()
     import torch
     from giem2g_amd import models
-    m = models.small(8, 8, 12, seed=2)
+    m = models.small(8, 8, 4, seed=2)
     op = gop.operator_for(m)
     v = torch.zeros(2, 3 * m.ncell, dtype=torch.complex128, device="cuda")
     o = torch.empty_like(v)
@@ -333,7 +337,7 @@ def test_error_paths():
         L.check(op.lib.giem2g_op_apply_batch_dev(op.h, 0, 0, v.data_ptr(), o.data_ptr()))
     with pytest.raises(L.G2GError, match="precision"):
         L.check(op.lib.giem2g_op_apply_batch_dev(op.h, 2, 2, v.data_ptr(), o.data_ptr()))
-    op.apply_batch_dev(v, o)  # Nz = 12: one vector at a time
+    op.apply_batch_dev(v, o)  # Nz = 4: one vector at a time
     op.sync()
     assert torch.count_nonzero(o) == 0
     op.close()
