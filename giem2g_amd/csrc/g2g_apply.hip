@@ -1103,8 +1103,10 @@ __global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S, 
   contract3_body<R, NZ, NRHS, DEEP, NW, false>(S, sst, G, g, BS);
 }
 // other Nz from 8 to 64: the tiles of the next multiple of 16, zero-padded
+// (at least two waves per SIMD: the padded paired kernels for FP64 at NZ = 48
+// and complex64 at NZ = 64 otherwise take more than 256 registers a lane)
 template <typename R, int NZ, int NRHS, int DEEP, int NW>
-__global__ __launch_bounds__(64 * NW) void k_contract3_pad(cx_t<R>* __restrict__ S, int64_t sst,
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2))) void k_contract3_pad(cx_t<R>* __restrict__ S, int64_t sst,
                                                        const cx_t<R>* __restrict__ G, Geo g, int64_t BS) {
   contract3_body<R, NZ, NRHS, DEEP, NW, true>(S, sst, G, g, BS);
 }
@@ -1154,13 +1156,14 @@ static bool launch_contract3(const Operator& op, const cx_t<R>* G, cx_t<R>* S, i
 }
 // Automatic kernel for one vector: the matrix cores, except complex64 at
 // padded layer counts where the vector-ALU kernel measured faster (128x128,
-// contraction alone: Nz = 20, 24, 50, 56 by 12-19%; the padded tiles won at
-// Nz = 8, 12, 28, 40 and in FP64 at every Nz measured), i.e. where padding
-// adds more than half the tile work or pads to 64 layers.
+// contraction alone: Nz = 20 by 19%, Nz = 50 and 56 by 5-8%; the padded tiles
+// won at Nz = 8, 12, 28, 40, tied at 24, and won in FP64 at every Nz
+// measured), i.e. where padding adds more than 80% of the tile work or pads
+// to 64 layers.
 template <typename R> bool c3_auto(int nz) {
   const int nzp = (nz + 15) / 16 * 16;
   if (sizeof(R) == 8 || nzp == nz || nzp == 16) return true;
-  return nzp < 64 && 2 * nzp * nzp <= 3 * nz * nz;
+  return nzp < 64 && 5 * nzp * nzp <= 9 * nz * nz;
 }
 template <typename R, template <typename, int> class KF> struct Dummy {};
 

@@ -178,7 +178,7 @@ class IEOperator:
 
     def set_contraction(self, mode):
         """0 automatic (2 for Nz from 8 to 64, otherwise 1; complex64 at
-        Nz = 17-26, 33-39, 49-63 uses 1), 1 vector ALU
+        Nz = 17-23, 33-35, 49-63 uses 1), 1 vector ALU
         (two components in flight), 2 matrix cores (MFMA), 3 vector ALU (one
         component in flight)."""
         L.check(self.lib.giem2g_op_set_contraction(self.h, mode))
