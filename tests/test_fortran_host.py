@@ -218,3 +218,65 @@ def test_fortran_host_receivers(tmp_path, oracle):
     assert not any(l.startswith("freq") for l in r2.stdout.splitlines())
     for p in ("PY", "PX"):
         assert (tmp_path / ("%s_F%s.dat" % (p, fnum_of(freqs[0])))).read_text() == texts[p]
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(300)
+def test_fortran_host_reference_deck_geometry(tmp_path):
+    """The reference's own deck (test/giem2g_input.json): 128 x 128 x 50 cells
+    of 312.5 m on its graded z borders, its background, frequencies,
+    receivers and FGMRES settings, with a synthetic random-box conductivity
+    file in place of the deck's (not part of the reference).  Nz = 50 runs
+    the zero-padded matrix-core contraction.  Every solve converges; each
+    saved solution of f = 0.1 Hz satisfies the equation with the vector-ALU
+    contraction too (restarted from it, the FP64 solver stops at once) and
+    equals the Python API's solve warm-started the same way; the receiver
+    files hold finite rows for both receivers."""
+    import re
+    from giem2g_amd import models, operator as gop
+    host = _host()
+    with open(os.path.join(HERE, "golden", "reference_deck_giem2g_input.json")) as fh:
+        deck = json.loads(re.sub(r",\s*([\]}])", r"\1", fh.read()))  # the deck has trailing commas
+    an, bg, fg = deck["Anomaly"], deck["Background"], deck["FGMRES"]
+    assert (an["Nx"], an["Ny"], an["Nz"]) == (128, 128, 50)
+    rng = np.random.Generator(np.random.PCG64(50))
+    siga = models.random_boxes(rng, an["Nx"], an["Ny"], an["Nz"], 12, bg["Conductivity"][0], 0.01, 1.0)
+    m = models.Model("deck", an["Nx"], an["Ny"], an["Nz"], an["dx"], an["dy"], np.array(an["zborders"]),
+                     list(bg["Conductivity"]), list(bg["Thickness"]), 0.1, siga)
+    sig = np.transpose(m.siga, (1, 2, 0)).reshape(-1)
+    np.savetxt(str(tmp_path / "sigma.dat"), sig, fmt="%.17e")
+    an["Conductivity"] = "sigma.dat"
+    with open(str(tmp_path / "deck.json"), "w") as f:
+        json.dump(deck, f, indent=1)
+    print("host run reference deck", flush=True)
+    r = subprocess.run([host, "deck.json"], cwd=str(tmp_path), capture_output=True, text=True, timeout=280)
+    print(r.stdout[-3000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    runs = [l for l in r.stdout.splitlines() if l.startswith("freq")]
+    assert len(runs) == 2 * len(deck["Frequencies"]) and all(" status   1" in l for l in runs), runs
+    recv = deck["Recievers"]
+    for f in deck["Frequencies"]:
+        for p in ("PY", "PX"):
+            assert (tmp_path / ("SOL_%s_F%s.bin" % (p, fnum_of(f)))).exists()
+            _, rows = read_dat(str(tmp_path / ("%s_F%s.dat" % (p, fnum_of(f)))))
+            assert rows.shape == (len(recv) * m.Nx * m.Ny, 28) and np.isfinite(rows).all()
+            assert np.abs(rows[:, 4:]).max() > 0
+    f = 0.1
+    m.freq = f
+    op = gop.operator_for(m)
+    prev = deck["Frequencies"][deck["Frequencies"].index(f) - 1]
+    for p, pol in (("PY", 1), ("PX", 0)):
+        sol = read_sol(str(tmp_path / ("SOL_%s_F%s.bin" % (p, fnum_of(f)))), m)
+        warm = read_sol(str(tmp_path / ("SOL_%s_F%s.bin" % (p, fnum_of(prev)))), m)
+        E = op.plane_wave(pol)
+        op.set_contraction(0)
+        Es, info = op.solve(E, E0=warm, outer_m=fg["OuterBufferSize"], inner_m=fg["InnerBufferSize"],
+                            max_restart=fg["MaxRestart"], tol=fg["Tolerance"])
+        assert info["status"] == 1 and rel(Es, sol) < 1e-12, (p, info)
+        op.set_contraction(1)
+        Ev, iv = op.solve(E, E0=sol, outer_m=fg["OuterBufferSize"], inner_m=fg["InnerBufferSize"],
+                          max_restart=fg["MaxRestart"], tol=fg["Tolerance"])
+        print(p, "Python solve", info, "restart on the vector ALU", iv, flush=True)
+        assert iv["status"] == 1 and iv["outer_iterations"] <= 1 and iv["true_residual"] < fg["Tolerance"], iv
+        assert rel(Ev, sol) < 1e-7
+    op.close()
