@@ -231,7 +231,8 @@ def test_fortran_host_reference_deck_geometry(tmp_path):
     saved solution of f = 0.1 Hz satisfies the equation with the vector-ALU
     contraction too (restarted from it, the FP64 solver stops at once) and
     equals the Python API's solve warm-started the same way; the receiver
-    files hold finite rows for both receivers."""
+    files hold finite rows for both receivers.  The same deck with complex64
+    inner solves gives every field within 1e-6 of the FP64 one."""
     import re
     from giem2g_amd import models, operator as gop
     host = _host()
@@ -280,3 +281,28 @@ def test_fortran_host_reference_deck_geometry(tmp_path):
         assert iv["status"] == 1 and iv["outer_iterations"] <= 1 and iv["true_residual"] < fg["Tolerance"], iv
         assert rel(Ev, sol) < 1e-7
     op.close()
+    # the same deck with complex64 inner solves (Solver.InnerPrecision = 1):
+    # every solve converges in FP64 terms and its field is within 1e-6 of the
+    # FP64 solve's
+    mixed = tmp_path / "mixed"
+    mixed.mkdir()
+    deck["Solver"] = {"InnerPrecision": 1}
+    deck["Recalculate Fields"] = False
+    an["Conductivity"] = "../sigma.dat"
+    with open(str(mixed / "deck.json"), "w") as f:
+        json.dump(deck, f, indent=1)
+    print("host run reference deck, complex64 inner", flush=True)
+    r = subprocess.run([host, "deck.json"], cwd=str(mixed), capture_output=True, text=True, timeout=200)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0
+    mruns = [l for l in r.stdout.splitlines() if l.startswith("freq")]
+    assert len(mruns) == len(runs) and all(" status   1" in l for l in mruns), mruns
+    assert all(int(l.split("mv32")[1].split()[0]) > 0 for l in mruns)
+    t64 = sum(float(l.split()[-2]) for l in runs)
+    t32 = sum(float(l.split()[-2]) for l in mruns)
+    print("reference deck: FP64 solves %.3f s, complex64 inner %.3f s" % (t64, t32), flush=True)
+    for f in deck["Frequencies"]:
+        for p in ("PY", "PX"):
+            a = read_sol(str(tmp_path / ("SOL_%s_F%s.bin" % (p, fnum_of(f)))), m)
+            b = read_sol(str(mixed / ("SOL_%s_F%s.bin" % (p, fnum_of(f)))), m)
+            assert rel(b, a) < 1e-6, (f, p, rel(b, a))
