@@ -50,9 +50,11 @@ def test_apply_matches_oracle_on_oracle_tensor(oracle, Nx, Ny, Nz):
 
 
 @pytest.mark.parametrize("Nx,Ny,Nz", [(16, 16, 16), (8, 16, 32), (8, 8, 48), (8, 8, 64),
-                                      (8, 8, 8), (8, 8, 12), (16, 8, 24), (8, 8, 50)])
+                                      (8, 8, 8), (8, 8, 12), (16, 8, 24), (8, 8, 50), (8, 8, 33),
+                                      (8, 16, 63)])
 def test_matrix_core_contraction_matches_oracle(oracle, Nx, Ny, Nz):
-    """Nz = 16, 32, 48, 64 and zero-padded layer counts (8, 12, 24, 50): the
+    """Nz = 16, 32, 48, 64 and zero-padded layer counts (8, 12, 24, 33, 50,
+    63; one layer past or short of a multiple of 16 at 33 and 63): the
     MFMA contraction (mode 2), the vector-ALU one (mode 1)
     and the automatic choice (mode 0) against the oracle, FP64 at 1e-12;
     complex64 within 2e-6."""
@@ -80,7 +82,7 @@ def test_matrix_core_contraction_matches_oracle(oracle, Nx, Ny, Nz):
 
 
 @pytest.mark.parametrize("Nx,Ny,Nz", [(16, 8, 16), (8, 16, 32), (8, 8, 48), (8, 8, 64),
-                                      (8, 8, 12), (8, 16, 50)])
+                                      (8, 8, 12), (8, 16, 50), (16, 8, 9)])
 def test_paired_apply_matches_oracle(oracle, Nx, Ny, Nz):
     """Two vectors through one tensor read (giem2g_op_apply_pair_dev): each
     result against the oracle, FP64 at 1e-12 and complex64 within 2e-6, and
@@ -118,7 +120,7 @@ def test_paired_apply_matches_oracle(oracle, Nx, Ny, Nz):
 
 @pytest.mark.parametrize("Nx,Ny,Nz,nrhs", [(8, 8, 16, 8), (16, 8, 32, 8), (8, 16, 32, 5), (8, 8, 32, 3),
                                             (8, 8, 64, 3), (8, 8, 12, 2), (8, 8, 24, 8), (8, 8, 50, 3),
-                                            (8, 8, 4, 3)])
+                                            (8, 8, 4, 3), (8, 8, 31, 8), (8, 8, 47, 2)])
 def test_batched_apply_matches_oracle(oracle, Nx, Ny, Nz, nrhs):
     """nrhs vectors through giem2g_op_apply_batch_dev (one tensor read for up
     to 8 at Nz 8 to 32; pairs at Nz 33 to 64; one by one below 8): every
