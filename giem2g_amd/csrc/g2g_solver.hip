@@ -48,15 +48,19 @@ __global__ __launch_bounds__(BT) void k_mdot(const T2* __restrict__ V, int64_t N
   A re[KC], im[KC];
 #pragma unroll
   for (int k = 0; k < KC; k++) re[k] = im[k] = 0;
+  // all KC loads issued before the products: the column index is clamped to
+  // kc - 1 instead of a branch per column (with a branch each load waited for
+  // the one before, about 2 TB/s); sums past kc are never read
   for (int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x; i < N; i += (int64_t)gridDim.x * BT) {
     const T2 wv = w[i];
+    T2 v[KC];
 #pragma unroll
-    for (int k = 0; k < KC; k++)
-      if (k < kc) {
-        const T2 v = ld_stream(V + (int64_t)k * N + i);
-        re[k] += (A)v.x * (A)wv.x + (A)v.y * (A)wv.y;
-        im[k] += (A)v.x * (A)wv.y - (A)v.y * (A)wv.x;
-      }
+    for (int k = 0; k < KC; k++) v[k] = ld_stream(V + (int64_t)(k < kc ? k : kc - 1) * N + i);
+#pragma unroll
+    for (int k = 0; k < KC; k++) {
+      re[k] += (A)v[k].x * (A)wv.x + (A)v[k].y * (A)wv.y;
+      im[k] += (A)v[k].x * (A)wv.y - (A)v[k].y * (A)wv.x;
+    }
   }
   __shared__ A red[BT / 64][2 * KC];
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -104,16 +108,25 @@ template <typename T2, typename A>
 __global__ __launch_bounds__(BT) void k_update_norm(T2* __restrict__ w, const T2* __restrict__ V,
                                                     int64_t N, int K, const zd* __restrict__ h,
                                                     A* __restrict__ part) {
-  __shared__ T2 hs[64];
-  for (int k = threadIdx.x; k < K; k += BT) hs[k] = mk<T2>(h[k].x, h[k].y);
+  // coefficients past K are zero, so the columns go in groups of KC with
+  // all loads of a group in flight (index clamped to K - 1); the terms are
+  // added in the same order as one at a time, and the zero ones change nothing
+  __shared__ T2 hs[MAXK];
+  for (int k = threadIdx.x; k < MAXK; k += BT) hs[k] = k < K ? mk<T2>(h[k].x, h[k].y) : mk<T2>(0, 0);
   __syncthreads();
   A nrm = 0;
   for (int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x; i < N; i += (int64_t)gridDim.x * BT) {
     T2 acc = w[i];
-    for (int k = 0; k < K; k++) {
-      const T2 v = ld_stream(V + (int64_t)k * N + i), hk = hs[k];
-      acc.x -= v.x * hk.x - v.y * hk.y;
-      acc.y -= v.x * hk.y + v.y * hk.x;
+    for (int k0 = 0; k0 < K; k0 += KC) {
+      T2 v[KC];
+#pragma unroll
+      for (int j = 0; j < KC; j++) v[j] = ld_stream(V + (int64_t)(k0 + j < K ? k0 + j : K - 1) * N + i);
+#pragma unroll
+      for (int j = 0; j < KC; j++) {
+        const T2 hk = hs[k0 + j];
+        acc.x -= v[j].x * hk.x - v[j].y * hk.y;
+        acc.y -= v[j].x * hk.y + v[j].y * hk.x;
+      }
     }
     w[i] = acc;
     nrm += (A)acc.x * (A)acc.x + (A)acc.y * (A)acc.y;
