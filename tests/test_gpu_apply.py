@@ -120,7 +120,8 @@ def test_paired_apply_matches_oracle(oracle, Nx, Ny, Nz):
 
 @pytest.mark.parametrize("Nx,Ny,Nz,nrhs", [(8, 8, 16, 8), (16, 8, 32, 8), (8, 16, 32, 5), (8, 8, 32, 3),
                                             (8, 8, 64, 3), (8, 8, 12, 2), (8, 8, 24, 8), (8, 8, 50, 3),
-                                            (8, 8, 4, 3), (8, 8, 31, 8), (8, 8, 47, 2)])
+                                            (8, 8, 4, 3), (8, 8, 31, 8), (8, 8, 47, 2), (8, 8, 12, 8),
+                                            (16, 8, 9, 5)])
 def test_batched_apply_matches_oracle(oracle, Nx, Ny, Nz, nrhs):
     """nrhs vectors through giem2g_op_apply_batch_dev (one tensor read for up
     to 8 at Nz 8 to 32; pairs at Nz 33 to 64; one by one below 8): every
