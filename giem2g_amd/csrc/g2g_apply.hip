@@ -1096,9 +1096,18 @@ __device__ __forceinline__ void contract3_body(cx_t<R>* __restrict__ S, int64_t 
 #undef PY
 #undef VAL
 }
+// Minimum waves per SIMD asked of the compiler for the unpadded kernel: four
+// for complex64 at NZ = 48 (one or two right-hand sides fit in 128 registers
+// a lane without spilling, against three waves otherwise); no request for
+// the others, whose register use the plain launch bounds set (a request for
+// more waves made the FP64 NZ = 32 kernel, the complex64 NZ = 32 and 64
+// kernels and the padded complex64 NZ = 48 kernels spill)
+template <typename R, int NZ, int NRHS> constexpr int c3_waves() {
+  return (sizeof(R) == 4 && NZ == 48 && NRHS <= 2) ? 4 : 1;
+}
 // Nz a multiple of 16 (NZ = Nz)
 template <typename R, int NZ, int NRHS, int DEEP, int NW>
-__global__ __launch_bounds__(64 * NW) void k_contract3(cx_t<R>* __restrict__ S, int64_t sst,
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(c3_waves<R, NZ, NRHS>()))) void k_contract3(cx_t<R>* __restrict__ S, int64_t sst,
                                                    const cx_t<R>* __restrict__ G, Geo g, int64_t BS) {
   contract3_body<R, NZ, NRHS, DEEP, NW, false>(S, sst, G, g, BS);
 }
