@@ -1289,9 +1289,15 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
   G2G_CHECK(op.have_tensor && op.have_sigma, -4, "operator not ready: tensor or conductivity missing");
   const int Nz3 = 3 * op.Nz;
   Geo g = make_geo(op, b.NR);
+  // profiling: events around the three stages (flushed into op.cnt later)
+  const bool prof = op.prof.on && !timed;
+  const int prec = sizeof(R) == 8 ? 0 : 1;
+  int pe[4] = {0, 0, 0, 0};
+  if (prof) pe[0] = op.prof.record(s);
   if (timed) G2G_HIP(hipEventRecord(op.ev[0], s));
   fft_forward<R>(op, b, vin, b.S, g, s);
   if (timed) G2G_HIP(hipEventRecord(op.ev[1], s));
+  if (prof) pe[1] = op.prof.record(s);
   {
     // vector ALU: rows = Nz layers, split lanes per row (power of two)
     int split_log = 0;
@@ -1334,8 +1340,15 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
     }
   }
   if (timed) G2G_HIP(hipEventRecord(op.ev[2], s));
+  if (prof) pe[2] = op.prof.record(s);
   fft_inverse<R>(op, b, b.S, vin, vout, g, s);
   if (timed) G2G_HIP(hipEventRecord(op.ev[3], s));
+  if (prof) {
+    pe[3] = op.prof.record(s);
+    op.prof.mark(Prof::FWD, prec, 1, pe[0], pe[1]);
+    op.prof.mark(Prof::CONTR, prec, 1, pe[1], pe[2]);
+    op.prof.mark(Prof::BWD, prec, 1, pe[2], pe[3]);
+  }
   G2G_HIP(hipGetLastError());
 }
 
@@ -1377,7 +1390,12 @@ static void op_apply_k(Operator& op, OpBuffers<R>& b, int k, const cx_t<R>* cons
                        b.SB + nS * k, nz);
   }
   Geo g = make_geo(op, b.NR);
+  const bool prof = op.prof.on;
+  const int prec = sizeof(R) == 8 ? 0 : 1;
+  int pe[4] = {0, 0, 0, 0};
+  if (prof) pe[0] = op.prof.record(s);
   for (int r = 0; r < k; r++) fft_forward<R>(op, b, vins[r], b.SB + nS * r, g, s);
+  if (prof) pe[1] = op.prof.record(s);
   bool ok = false;
   switch (kk) {
     case 1: ok = launch_contract3<R, 1>(op, b.G, b.SB, nS, g, s); break;
@@ -1389,7 +1407,14 @@ static void op_apply_k(Operator& op, OpBuffers<R>& b, int k, const cx_t<R>* cons
       break;
   }
   G2G_CHECK(ok, -3, "batched contraction not available for this Nz");
+  if (prof) pe[2] = op.prof.record(s);
   for (int r = 0; r < k; r++) fft_inverse<R>(op, b, b.SB + nS * r, vins[r], vouts[r], g, s);
+  if (prof) {
+    pe[3] = op.prof.record(s);
+    op.prof.mark(Prof::FWD, prec, k, pe[0], pe[1]);
+    op.prof.mark(Prof::CONTR, prec, k, pe[1], pe[2]);
+    op.prof.mark(Prof::BWD, prec, k, pe[2], pe[3]);
+  }
   G2G_HIP(hipGetLastError());
 }
 
@@ -1783,6 +1808,9 @@ void op_apply_adjoint(Operator& op, const zd* vin, zd* vout, hipStream_t s) {
 void op_set_sigma(Operator& op, const zd* d_csiga) {
   G2G_CHECK(op.have_bkg, -4, "background not set");
   const int Nz = op.Nz;
+  if (!op.csiga) G2G_HIP(hipMalloc(&op.csiga, sizeof(zd) * op.ncell()));
+  if (d_csiga != op.csiga)
+    G2G_HIP(hipMemcpyAsync(op.csiga, d_csiga, sizeof(zd) * op.ncell(), hipMemcpyDeviceToDevice, op.stream));
   zd* d_csigb;
   double* d_sq;
   G2G_HIP(hipMalloc(&d_csigb, sizeof(zd) * Nz));
@@ -1791,7 +1819,7 @@ void op_set_sigma(Operator& op, const zd* d_csiga) {
   G2G_HIP(hipMemcpy(d_sq, op.sqsigb.data(), sizeof(double) * Nz, hipMemcpyHostToDevice));
   const int64_t n = op.ncell(), ncxy = (int64_t)op.Nx * op.Ny;
   const int blocks = (int)((n + 255) / 256);
-  hipLaunchKernelGGL((k_sigma<double>), dim3(blocks), dim3(256), 0, op.stream, d_csiga, d_csigb,
+  hipLaunchKernelGGL((k_sigma<double>), dim3(blocks), dim3(256), 0, op.stream, op.csiga, d_csigb,
                      d_sq, op.d64.gsig, op.d64.asiga, ncxy, Nz);
   G2G_HIP(hipMemcpyAsync(op.d64.sq, d_sq, sizeof(double) * Nz, hipMemcpyDeviceToDevice, op.stream));
   op.have_sigma = true;
@@ -1800,6 +1828,30 @@ void op_set_sigma(Operator& op, const zd* d_csiga) {
   G2G_HIP(hipStreamSynchronize(op.stream));
   (void)hipFree(d_csigb);
   (void)hipFree(d_sq);
+}
+
+template <typename R> static void upload_dzn(const Operator& op, OpBuffers<R>& b) {
+  if (!b.dzn) return;
+  std::vector<R> dzn(op.Nz);
+  for (int z = 0; z < op.Nz; z++) dzn[z] = (R)(1.0 / (op.dz[z] * 4.0 * op.Nx * op.Ny));
+  G2G_HIP(hipMemcpy(b.dzn, dzn.data(), sizeof(R) * op.Nz, hipMemcpyHostToDevice));
+}
+
+void op_set_geometry(Operator& op, double dx, double dy, const double* z) {
+  G2G_CHECK(dx > 0 && dy > 0, -3, "dx, dy must be positive");
+  std::vector<double> dz(op.Nz);
+  for (int i = 0; i < op.Nz; i++) {
+    dz[i] = z[i + 1] - z[i];
+    G2G_CHECK(dz[i] > 0, -3, "z borders must increase");
+  }
+  G2G_HIP(hipStreamSynchronize(op.stream));
+  if (dx != op.dx || dy != op.dy) op.have_wts = false;
+  op.dx = dx;
+  op.dy = dy;
+  op.z.assign(z, z + op.Nz + 1);
+  op.dz = dz;
+  upload_dzn<double>(op, op.d64);
+  upload_dzn<float>(op, op.d32);
 }
 
 __global__ void k_g_to_f32(const zd* __restrict__ a, zf* __restrict__ b, int64_t n) {

@@ -42,6 +42,10 @@ template <typename F> int guard(F&& f) {
 struct Handle {
   Operator op;
   int receivers = 0;  // receiver operators created on this operator (they hold a pointer to it)
+  // the caller's conductivity array given to giem2g_set_anomaly_conductivity:
+  // aliased as in the reference (giem2g_interfaces.F90:199), so every
+  // giem2g_apply_ie_operator reads its current values
+  const double* sigma_alias = nullptr;
   std::vector<double> hist;
   PrecondHook learned;
   UNet* net = nullptr;  // attached learned preconditioner (not owned)
@@ -117,6 +121,9 @@ void op_init(Operator& op, int Nx, int Ny, int Nz, double dx, double dy, const d
 void op_fini(Operator& op) {
   if (op.adj) (void)hipFree(op.adj);
   op.adj = nullptr;
+  if (op.csiga) (void)hipFree(op.csiga);
+  op.csiga = nullptr;
+  op.prof.release();
   op_free_buffers<double>(op.d64);
   op_free_buffers<float>(op.d32);
   op_free_tensor_scratch(op);
@@ -148,6 +155,9 @@ void set_background(Operator& op, int Nl, const double* csigb, const double* thi
   op.bkg_version++;
   op.have_tensor = false;
   op.have_g32 = false;
+  // the per-cell scalings depend on the background: derive them again from
+  // the conductivity last set
+  if (op.have_sigma) op_set_sigma(op, op.csiga);
 }
 
 void build_tensor(Operator& op, int stage) {
@@ -179,17 +189,18 @@ void set_sigma_dev(Operator& op, const void* d) {
 }
 
 void set_sigma_host(Operator& op, const double* h) {
-  zd* d;
+  G2G_CHECK(h != nullptr, -2, "null conductivity");
+  G2G_CHECK(op.have_bkg, -4, "background not set");
   const size_t n = op.ncell();
-  G2G_HIP(hipMalloc(&d, sizeof(zd) * n));
-  G2G_HIP(hipMemcpy(d, h, sizeof(zd) * n, hipMemcpyHostToDevice));
-  try {
-    op_set_sigma(op, d);
-  } catch (...) {
-    (void)hipFree(d);
-    throw;
-  }
-  (void)hipFree(d);
+  if (!op.csiga) G2G_HIP(hipMalloc(&op.csiga, sizeof(zd) * n));
+  G2G_HIP(hipStreamSynchronize(op.stream));
+  G2G_HIP(hipMemcpy(op.csiga, h, sizeof(zd) * n, hipMemcpyHostToDevice));
+  op_set_sigma(op, op.csiga);
+}
+
+// merge the event log of the profiling mode into the counters
+void flush_prof(Operator& op) {
+  if (!op.prof.marks.empty()) op.prof.flush(op.cnt);
 }
 
 void apply_host(Operator& op, const double* in, double* out) {
@@ -385,6 +396,7 @@ void solve_dev(Handle& h, const zd* Ebkg, const zd* E0, const giem2g_solver_para
     solve_scale(op, Esol, Esol, 2);  // E = asiga x (:85-97)
     G2G_HIP(hipEventRecord(b, op.stream));
     G2G_HIP(hipEventSynchronize(b));
+    flush_prof(op);
   } catch (...) {
     (void)hipFree(rhs);
     (void)hipFree(guess);
@@ -484,11 +496,71 @@ int giem2g_op_put_tensor(void* p, const double* G, int spectral) {
 }
 
 int giem2g_op_set_sigma(void* p, const double* csiga) {
-  return guard([&] { set_sigma_host(H(p)->op, csiga); });
+  return guard([&] {
+    Handle* h = H(p);
+    set_sigma_host(h->op, csiga);
+    h->sigma_alias = nullptr;
+  });
 }
 
 int giem2g_op_set_sigma_dev(void* p, const void* csiga) {
-  return guard([&] { set_sigma_dev(H(p)->op, csiga); });
+  return guard([&] {
+    Handle* h = H(p);
+    G2G_CHECK(csiga != nullptr, -2, "null conductivity");
+    set_sigma_dev(h->op, csiga);
+    h->sigma_alias = nullptr;
+  });
+}
+
+int giem2g_op_set_geometry(void* p, double dx, double dy, const double* z) {
+  return guard([&] {
+    G2G_CHECK(z != nullptr, -2, "null layer borders");
+    Operator& op = H(p)->op;
+    op_set_geometry(op, dx, dy, z);
+    if (op.have_bkg) {
+      // Lnumber, csigb and the scalings follow the new layer borders; the tensor must be built again
+      std::vector<double> cs(2 * (op.Nl + 1)), th(op.Nl > 1 ? op.Nl - 1 : 1, 0.0);
+      for (int i = 0; i <= op.Nl; i++) {
+        cs[2 * i] = op.csigma[i].x;
+        cs[2 * i + 1] = op.csigma[i].y;
+      }
+      for (int i = 1; i <= op.Nl - 1; i++) th[i - 1] = op.thick[i];
+      set_background(op, op.Nl, cs.data(), th.data(), op.omega);
+    }
+  });
+}
+
+int giem2g_op_set_profiling(void* p, int on) {
+  return guard([&] {
+    Operator& op = H(p)->op;
+    G2G_HIP(hipStreamSynchronize(op.stream));
+    flush_prof(op);
+    op.prof.on = on != 0;
+  });
+}
+
+int giem2g_op_counters(void* p, double* o, int n) {
+  return guard([&] {
+    G2G_CHECK(o != nullptr && n >= 0, -2, "bad output array");
+    Operator& op = H(p)->op;
+    flush_prof(op);
+    const Counters& c = op.cnt;
+    const double v[20] = {(double)c.matvecs, (double)c.matvecs32, (double)c.dotprods, c.t_apply,
+                          c.t_fwd, c.t_contr, c.t_bwd, c.t_dot, c.t_tensor, c.t_tensor_fft,
+                          (double)(op.d64.bytes + op.d32.bytes), (double)c.gs_passes, c.krylov_bytes,
+                          (double)c.contr_launches[0], (double)c.contr_rhs[0], c.t_contr_p[0],
+                          (double)c.contr_launches[1], (double)c.contr_rhs[1], c.t_contr_p[1],
+                          (double)(op.prof.on ? 1 : 0)};
+    for (int i = 0; i < n && i < 20; i++) o[i] = v[i];
+  });
+}
+
+int giem2g_op_reset_counters(void* p) {
+  return guard([&] {
+    Operator& op = H(p)->op;
+    flush_prof(op);
+    op.cnt = Counters();
+  });
 }
 
 int giem2g_op_apply_dev(void* p, const void* in, void* out) {
@@ -559,6 +631,7 @@ int giem2g_op_sync(void* p) {
 int giem2g_op_stats(void* p, double* o) {
   return guard([&] {
     Operator& op = H(p)->op;
+    flush_prof(op);
     const Counters& c = op.cnt;
     const double v[11] = {(double)c.matvecs, (double)c.matvecs32, (double)c.dotprods, c.t_apply,
                           c.t_fwd, c.t_contr, c.t_bwd, c.t_dot, c.t_tensor, c.t_tensor_fft,
@@ -638,6 +711,7 @@ int giem2g_solve_pair_dev(void* p, const void* Ebkg0, const void* Ebkg1, const v
       for (int k = 0; k < 2; k++) solve_scale(op, es[k], es[k], 2);
       G2G_HIP(hipEventRecord(b, op.stream));
       G2G_HIP(hipEventSynchronize(b));
+      flush_prof(op);
     } catch (...) {
       (void)hipFree(work);
       throw;
@@ -889,9 +963,14 @@ void giem2g_prepare_ie_kernel(GIEM2G_ANOMALY_TYPE a, GIEM2G_DATA_TYPE data) {
 }
 
 void giem2g_calc_ie_kernel(void* p, GIEM2G_BACKGROUND_TYPE bkg, GIEM2G_ANOMALY_TYPE a, double omega) {
-  (void)a;
   const int rc = guard([&] {
     Operator& op = H(p)->op;
+    // the anomaly given here defines the geometry of this build: the layer
+    // of every cell (Lnumber) is derived from its borders at each call
+    // (giem2g_interfaces.F90:158-166)
+    G2G_CHECK(a.Nx == op.Nx && a.Ny == op.Ny && a.Nz == op.Nz, -3,
+              "anomaly dimensions differ from the prepared operator's");
+    if (a.z) op_set_geometry(op, a.dx, a.dy, static_cast<const double*>(a.z));
     set_background(op, bkg.Nl, static_cast<const double*>(bkg.csigb),
                    static_cast<const double*>(bkg.thickness), omega);
     build_tensor(op, 0);
@@ -905,12 +984,21 @@ void giem2g_calc_fft_of_ie_kernel(void* p) {
 }
 
 void giem2g_set_anomaly_conductivity(void* p, void* csiga) {
-  const int rc = guard([&] { set_sigma_host(H(p)->op, static_cast<const double*>(csiga)); });
+  const int rc = guard([&] {
+    Handle* h = H(p);
+    set_sigma_host(h->op, static_cast<const double*>(csiga));
+    h->sigma_alias = static_cast<const double*>(csiga);
+  });
   if (rc) log_msg(std::string("giem2g_set_anomaly_conductivity: ") + g_last_error);
 }
 
 void giem2g_apply_ie_operator(void* p, void* in, void* out) {
-  const int rc = guard([&] { apply_host(H(p)->op, static_cast<const double*>(in), static_cast<double*>(out)); });
+  const int rc = guard([&] {
+    Handle* h = H(p);
+    // the aliased conductivity may have changed since the last call
+    if (h->sigma_alias) set_sigma_host(h->op, h->sigma_alias);
+    apply_host(h->op, static_cast<const double*>(in), static_cast<double*>(out));
+  });
   if (rc) log_msg(std::string("giem2g_apply_ie_operator: ") + g_last_error);
 }
 
@@ -919,15 +1007,27 @@ void giem2g_set_logger(void (*logger)(const char*)) { g_logger = logger; }
 void giem2g_print_stats(void* p) {
   const int rc = guard([&] {
     Operator& op = H(p)->op;
+    G2G_HIP(hipStreamSynchronize(op.stream));
+    flush_prof(op);
     Counters& c = op.cnt;
-    const double n = c.matvecs > 0 ? (double)c.matvecs : 1.0;
-    char buf[512];
+    // the reference's split (PRINT_STATS, Integral_Equation_Module.F90:473-484,
+    // and SolveEquation's, ie_solver_module.F90:98-113); times are those
+    // measured (giem2g_op_time_apply, or every apply and Gram-Schmidt pass
+    // while profiling is on)
+    const double mv = (double)(c.matvecs + c.matvecs32);
+    const double n = mv > 0 ? mv : 1.0;
+    const double nd = c.dotprods > 0 ? (double)c.dotprods : 1.0;
+    char buf[1024];
     std::snprintf(buf, sizeof buf,
-                  "Time for multiplications: %.3e s | average fwd FFT %.3e s, contraction %.3e s, "
-                  "bwd FFT %.3e s, mult %.3e s | matvecs %lld (+%lld complex64) | tensor %.3e s, "
+                  "Time for multiplications: %.3e s | Time for dot products: %.3e s | "
+                  "Average dot product: %.3e s | Average fft forward: %.3e s | "
+                  "Average fft backward: %.3e s | Average contraction: %.3e s | Average mult: %.3e s | "
+                  "Number of matrix-vector multiplications: %lld (%lld FP64, %lld complex64) | "
+                  "Number of dot products: %lld in %lld Gram-Schmidt passes | tensor %.3e s, "
                   "tensor FFT %.3e s",
-                  c.t_apply, c.t_fwd / n, c.t_contr / n, c.t_bwd / n, c.t_apply / n,
-                  (long long)c.matvecs, (long long)c.matvecs32, c.t_tensor, c.t_tensor_fft);
+                  c.t_apply, c.t_dot, c.t_dot / nd, c.t_fwd / n, c.t_bwd / n, c.t_contr / n, c.t_apply / n,
+                  (long long)mv, (long long)c.matvecs, (long long)c.matvecs32, (long long)c.dotprods,
+                  (long long)c.gs_passes, c.t_tensor, c.t_tensor_fft);
     log_msg(buf);
     c = Counters();
   });

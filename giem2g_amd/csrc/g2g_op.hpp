@@ -12,16 +12,92 @@
 //     S (full 2Nx x 2Ny spectrum, [kx][ky][c*Nz+z]).
 #pragma once
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "g2g_common.hpp"
 
 namespace g2g {
 
+// Operator counters (the reference's split: Integral_Equation_Module.F90:
+// 44-56, ie_solver_module.F90:98-113).  Times of the FFT passes, the
+// contraction and the Krylov dot products come from HIP events and are
+// gathered while profiling is on (giem2g_op_set_profiling) or from
+// giem2g_op_time_apply; counts are always kept.
 struct Counters {
   int64_t matvecs = 0, matvecs32 = 0, dotprods = 0;
   double t_apply = 0, t_fwd = 0, t_contr = 0, t_bwd = 0, t_dot = 0;
   double t_tensor = 0, t_tensor_fft = 0;
+  // per precision (0 FP64, 1 complex64): contraction launches and their
+  // total time, and how many right-hand sides those launches served
+  int64_t contr_launches[2] = {0, 0}, contr_rhs[2] = {0, 0};
+  double t_contr_p[2] = {0, 0};
+  // classical Gram-Schmidt passes and the bytes the Krylov kernels move
+  // (multi-dot, update, norms, linear combinations, copies, conversions)
+  int64_t gs_passes = 0;
+  double krylov_bytes = 0;
+};
+
+// Event log of the profiling mode: pairs of HIP events recorded around the
+// stages of each apply and around each Gram-Schmidt pass, read back (one
+// synchronisation) by flush().  Shared by the threads of a paired solve.
+struct Prof {
+  enum Kind { FWD = 0, CONTR = 1, BWD = 2, DOT = 3 };
+  bool on = false;
+  std::mutex mu;
+  std::vector<hipEvent_t> pool;
+  struct Mark {
+    int kind, prec, nrhs;
+    int a, b;
+  };
+  std::vector<Mark> marks;
+  size_t next = 0;
+  // record one event on stream s, returning its index in the pool
+  int record(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(mu);
+    if (next == pool.size()) {
+      hipEvent_t e;
+      G2G_HIP(hipEventCreate(&e));
+      pool.push_back(e);
+    }
+    G2G_HIP(hipEventRecord(pool[next], s));
+    return (int)next++;
+  }
+  void mark(int kind, int prec, int nrhs, int a, int b) {
+    std::lock_guard<std::mutex> lk(mu);
+    marks.push_back(Mark{kind, prec, nrhs, a, b});
+  }
+  // wait for the recorded events and add their intervals to c; called
+  // between API calls only (never while a solve still records)
+  void flush(Counters& c) {
+    std::lock_guard<std::mutex> lk(mu);
+    for (const Mark& m : marks) {
+      G2G_HIP(hipEventSynchronize(pool[m.b]));
+      float ms = 0;
+      G2G_HIP(hipEventElapsedTime(&ms, pool[m.a], pool[m.b]));
+      const double t = ms * 1e-3;
+      switch (m.kind) {
+        case FWD: c.t_fwd += t; c.t_apply += t; break;
+        case CONTR:
+          c.t_contr += t;
+          c.t_apply += t;
+          c.t_contr_p[m.prec] += t;
+          c.contr_launches[m.prec]++;
+          c.contr_rhs[m.prec] += m.nrhs;
+          break;
+        case BWD: c.t_bwd += t; c.t_apply += t; break;
+        default: c.t_dot += t; break;
+      }
+    }
+    marks.clear();
+    next = 0;
+  }
+  void release() {
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    pool.clear();
+    marks.clear();
+    next = 0;
+  }
 };
 
 template <typename R> struct OpBuffers {
@@ -57,6 +133,12 @@ struct Operator {
   std::vector<double> sqsigb;
   bool have_bkg = false, have_tensor = false, have_sigma = false, have_g32 = false;
   int bkg_version = 0;  // incremented by every background (frequency) change
+  // the anomaly conductivity last set (complex (Nx,Ny,Nz)): the per-cell
+  // scalings derive from it and the background, so a background change
+  // recomputes them from this copy, as the reference derives them from the
+  // current csigb at every apply (apply_ie_operator_module.F90:66-68)
+  zd* csiga = nullptr;
+  Prof prof;
   int64_t BS = 0;
   OpBuffers<double> d64;
   OpBuffers<float> d32;
@@ -135,7 +217,13 @@ int op_batch_max(const Operator& op);
 template <typename R>
 void op_apply_batch(Operator& op, OpBuffers<R>& b, int nrhs, const cx_t<R>* vin, cx_t<R>* vout,
                     hipStream_t s);
+// per-cell scalings from the conductivity d_csiga (HBM, complex (Nx,Ny,Nz));
+// keeps a copy in op.csiga (d_csiga may be op.csiga itself)
 void op_set_sigma(Operator& op, const zd* d_csiga);
+// new cell sizes and layer borders (same Nx, Ny, Nz): the 1/(dz 4NxNy)
+// factors of the inverse transform are uploaded again and the cached
+// filter weights (which depend on dx, dy) are dropped
+void op_set_geometry(Operator& op, double dx, double dy, const double* z);
 // out = A^H in (FP64), from the reciprocity of the contraction kernel
 void op_apply_adjoint(Operator& op, const zd* vin, zd* vout, hipStream_t s);
 void op_make_g32(Operator& op);

@@ -202,7 +202,14 @@ struct DevBlas {
   double* res = nullptr; // 2*MAXK dot products (complex pairs) + 2KC for a norm
   zd* hbuf = nullptr; // 64
   double hres[2 * MAXK + 2 * KC];
-  int64_t* dotcount = nullptr;
+  // counts of this solver (merged into the operator's counters by the caller)
+  int64_t dots = 0, passes = 0;
+  double bytes = 0;  // bytes the Krylov kernels read and write
+  Prof* prof = nullptr;  // event log of the profiling mode (dot products), or null
+  int prof_start() { return prof && prof->on ? prof->record(s) : -1; }
+  void prof_end(int a) {
+    if (a >= 0) prof->mark(Prof::DOT, 0, 0, a, prof->record(s));
+  }
   void init(hipStream_t st) {
     s = st;
     G2G_HIP(hipMalloc(&part, sizeof(double) * (MAXK / KC) * NBLK * 2 * KC));
@@ -220,6 +227,7 @@ struct DevBlas {
   template <typename T2, typename A> void mdot_dev(const T2* V, int64_t N, int K, const T2* w) {
     G2G_CHECK(K >= 1 && K <= MAXK, -3, "multi-dot of more than 64 columns");
     const int nb = nblocks(N), nrow = (K + KC - 1) / KC;
+    bytes += (double)sizeof(T2) * N * (K + 1);
     hipLaunchKernelGGL((k_mdot<T2, A>), dim3(nb, nrow), dim3(BT), 0, s, V, N, K, w, (A*)part);
     hipLaunchKernelGGL((k_reduce<A>), dim3(nrow), dim3(BT), 0, s, (const A*)part, nb, res);
   }
@@ -230,23 +238,29 @@ struct DevBlas {
   // out[k] = V[:,k]^H w for k < K
   template <typename T2, typename A>
   void mdot(const T2* V, int64_t N, int K, const T2* w, zc* out) {
+    const int pa = prof_start();
     mdot_dev<T2, A>(V, N, K, w);
+    prof_end(pa);
     fetch(2 * K);
     for (int k = 0; k < K; k++) out[k] = zc(hres[2 * k], hres[2 * k + 1]);
-    if (dotcount) *dotcount += K;
+    dots += K;
   }
   // One classical Gram-Schmidt pass: h = V^H w, w -= V h, returns |w|; h goes
   // from the reduction straight to the update kernel, one host round trip.
   template <typename T2, typename A>
   double ortho_pass(T2* w, const T2* V, int64_t N, int K, zc* h) {
+    const int pa = prof_start();
     mdot_dev<T2, A>(V, N, K, w);
     const int nb = nblocks(N);
     hipLaunchKernelGGL((k_update_norm<T2, A>), dim3(nb), dim3(BT), 0, s, w, V, N, K, (const zd*)res,
                        (A*)part);
     hipLaunchKernelGGL((k_reduce<A>), dim3(1), dim3(BT), 0, s, (const A*)part, nb, res + 2 * MAXK);
+    prof_end(pa);
+    bytes += (double)sizeof(T2) * N * (K + 2);
     fetch(2 * MAXK + 2 * KC);
     for (int k = 0; k < K; k++) h[k] = zc(hres[2 * k], hres[2 * k + 1]);
-    if (dotcount) *dotcount += K + 1;
+    dots += K + 1;
+    passes++;
     return std::sqrt(hres[2 * MAXK]);
   }
   template <typename T2, typename A> double norm(const T2* v, int64_t N) {
@@ -256,22 +270,28 @@ struct DevBlas {
   }
   template <typename T2>
   void lincomb(T2* x, const T2* B, int64_t N, int K, const zc* y, bool acc) {
+    bytes += (double)sizeof(T2) * N * (K + (acc ? 2 : 1));
     G2G_HIP(hipMemcpyAsync(hbuf, y, sizeof(zd) * K, hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL((k_lincomb<T2>), gridN(N), dim3(BT), 0, s, x, B, N, K, hbuf, acc ? 1 : 0);
   }
   template <typename T2, typename R> void scale(T2* d, const T2* src, int64_t N, R f) {
+    bytes += 2.0 * sizeof(T2) * N;
     hipLaunchKernelGGL((k_scale<T2, R>), gridN(N), dim3(BT), 0, s, d, src, N, f);
   }
   template <typename T2> void copy(T2* d, const T2* src, int64_t N) {
+    bytes += 2.0 * sizeof(T2) * N;
     G2G_HIP(hipMemcpyAsync(d, src, sizeof(T2) * N, hipMemcpyDeviceToDevice, s));
   }
   template <typename T2> void bminus(T2* r, const T2* b, int64_t N) {
+    bytes += 3.0 * sizeof(T2) * N;
     hipLaunchKernelGGL((k_bminus<T2>), gridN(N), dim3(BT), 0, s, r, b, N);
   }
   template <typename T2> void add(T2* x, const T2* a, int64_t N) {
+    bytes += 3.0 * sizeof(T2) * N;
     hipLaunchKernelGGL((k_add<T2>), gridN(N), dim3(BT), 0, s, x, a, N);
   }
   template <typename Ti, typename To> void convert(To* o, const Ti* a, int64_t N) {
+    bytes += (double)(sizeof(Ti) + sizeof(To)) * N;
     hipLaunchKernelGGL((k_convert<Ti, To>), gridN(N), dim3(BT), 0, s, o, a, N);
   }
 };
@@ -524,7 +544,7 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
   G2G_CHECK(!env || p.precond == 0, -3, "the learned preconditioner runs only in single solves");
   DevBlas blas;
   blas.init(st);
-  blas.dotcount = env ? nullptr : &op.cnt.dotprods;
+  blas.prof = &op.prof;
   Gmres<double> outer;
   outer.blas = &blas;
   outer.maxit = p.max_restart;
@@ -640,6 +660,14 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
     r.precond_calls = res.precond_calls;
     r.precond_fallbacks = res.precond_fallbacks;
     res = r;
+    res.dots = blas.dots;
+    res.passes = blas.passes;
+    res.krylov_bytes = blas.bytes;
+    if (!env) {
+      op.cnt.dotprods += blas.dots;
+      op.cnt.gs_passes += blas.passes;
+      op.cnt.krylov_bytes += blas.bytes;
+    }
   } catch (...) {
     outer.fini(); in64.fini(); in32.fini(); blas.fini();
     for (void* q : {(void*)b32, (void*)x32, (void*)r32, (void*)tmp, (void*)tmp2})
@@ -722,9 +750,13 @@ struct PairBatcher {
       cv.wait(lk);
     }
   }
+  // a worker that returns (or throws) withdraws any request it left, so the
+  // other never runs a product on buffers that are freed
   void finish(int me) {
     std::lock_guard<std::mutex> lk(mu);
     active[me] = false;
+    r[me].pending = false;
+    r[me].done = false;
     cv.notify_all();
   }
 };
@@ -771,6 +803,11 @@ PairResult nested_solve_pair(Operator& op, const zd* const b[2], const zd* const
   out.alone = pb.alone;
   op.cnt.matvecs += out.mv64[0] + out.mv64[1];
   op.cnt.matvecs32 += out.mv32[0] + out.mv32[1];
+  for (int k = 0; k < 2; k++) {
+    op.cnt.dotprods += out.r[k].dots;
+    op.cnt.gs_passes += out.r[k].passes;
+    op.cnt.krylov_bytes += out.r[k].krylov_bytes;
+  }
   return out;
 }
 

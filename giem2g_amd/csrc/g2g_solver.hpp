@@ -19,6 +19,9 @@ struct SolveResult {
   int stat = -1;
   double bea = -1, be = -1;
   long long precond_calls = 0, precond_fallbacks = 0;
+  // Krylov work of the solve: dot products, Gram-Schmidt passes, bytes moved
+  long long dots = 0, passes = 0;
+  double krylov_bytes = 0;
 };
 
 // A learned right preconditioner: z = M(v) on HBM vectors (complex128).

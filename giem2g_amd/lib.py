@@ -34,6 +34,8 @@ EXTENDED_SYMBOLS = [
     "giem2g_rc_receivers",
     "giem2g_rc_set_anomaly_conductivity", "giem2g_rc_set_anomaly_conductivity_dev",
     "giem2g_rc_recalculate", "giem2g_rc_recalculate_dev", "giem2g_rc_plane_wave", "giem2g_rc_stats",
+    "giem2g_op_set_geometry", "giem2g_op_set_profiling", "giem2g_op_counters",
+    "giem2g_op_reset_counters",
 ]
 
 
@@ -153,6 +155,10 @@ def load():
         "giem2g_rc_recalculate_dev": (i, [vp, vp, vp, vp]),
         "giem2g_rc_plane_wave": (i, [vp, i, vp]),
         "giem2g_rc_stats": (i, [vp, vp]),
+        "giem2g_op_set_geometry": (i, [vp, d, d, vp]),
+        "giem2g_op_set_profiling": (i, [vp, i]),
+        "giem2g_op_counters": (i, [vp, vp, i]),
+        "giem2g_op_reset_counters": (i, [vp]),
         "giem2g_calc_data_sizes": (None, [Anomaly, C.POINTER(Data)]),
         "giem2g_prepare_ie_kernel": (None, [Anomaly, Data]),
         "giem2g_calc_ie_kernel": (None, [vp, Background, Anomaly, d]),

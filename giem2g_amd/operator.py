@@ -19,14 +19,60 @@ ORDER_YXZ = 1
 def _ptr(x):
     """Raw address of a numpy array or torch tensor (contiguous)."""
     if isinstance(x, np.ndarray):
-        assert x.flags["C_CONTIGUOUS"]
+        if not x.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be C-contiguous")
         return x.ctypes.data
     if hasattr(x, "data_ptr"):
-        assert x.is_contiguous()
+        if not x.is_contiguous():
+            raise ValueError("tensor must be contiguous")
         return x.data_ptr()
     if x is None:
         return None
     return int(x)
+
+
+_DTYPE_NAMES = {"c128": "complex128", "c64": "complex64"}
+
+
+def dev_ptrs(n, dtype, *tensors, allow_none=()):
+    """Addresses of torch tensors handed to a device entry point, after the
+    checks that keep a kernel from reading or writing out of bounds: each is
+    a CUDA (HIP) tensor, contiguous, of the given dtype ("c128" or "c64")
+    and with exactly n elements.  Positions listed in allow_none may be None
+    (a nullable argument).  Raises ValueError (not assert, which python -O
+    removes)."""
+    import torch
+    want = {"c128": torch.complex128, "c64": torch.complex64}[dtype]
+    out = []
+    for k, t in enumerate(tensors):
+        if t is None:
+            if k in allow_none:
+                out.append(None)
+                continue
+            raise ValueError("argument %d must be a tensor" % k)
+        if not isinstance(t, torch.Tensor):
+            raise ValueError("argument %d must be a torch tensor on the GPU, got %s" % (k, type(t).__name__))
+        if t.dtype != want:
+            raise ValueError("argument %d must be %s, got %s" % (k, _DTYPE_NAMES[dtype], t.dtype))
+        if t.numel() != n:
+            raise ValueError("argument %d must have %d elements, got %d" % (k, n, t.numel()))
+        if not t.is_contiguous():
+            raise ValueError("argument %d must be contiguous" % k)
+        if not t.is_cuda:
+            raise ValueError("argument %d must be on the GPU (device %s)" % (k, t.device))
+        out.append(t.data_ptr())
+    return out
+
+
+def _prec_of(t):
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError("expected a torch tensor on the GPU, got %s" % type(t).__name__)
+    if t.dtype == torch.complex128:
+        return 0, "c128"
+    if t.dtype == torch.complex64:
+        return 1, "c64"
+    raise ValueError("vectors must be complex128 or complex64, got %s" % t.dtype)
 
 
 class IEOperator:
@@ -62,6 +108,10 @@ class IEOperator:
         return 4 * self.Nz * (self.Nz + 1) // 2 + 2 * self.Nz * self.Nz
 
     @property
+    def ncell(self):
+        return self.Nx * self.Ny * self.Nz
+
+    @property
     def nvec(self):
         return 3 * self.Nx * self.Ny * self.Nz
 
@@ -85,59 +135,79 @@ class IEOperator:
 
     def put_tensor(self, G, spectral=True):
         G = np.ascontiguousarray(G, np.complex128)
-        assert G.size == self.Nx * self.Ny * self.block_size
+        if G.size != self.Nx * self.Ny * self.block_size:
+            raise ValueError("tensor must have Nx*Ny*block_size = %d elements"
+                             % (self.Nx * self.Ny * self.block_size))
         L.check(self.lib.giem2g_op_put_tensor(self.h, G.ctypes.data, 1 if spectral else 0))
 
     # --- conductivity --------------------------------------------------------
     def set_sigma(self, csiga):
         if isinstance(csiga, np.ndarray):
             c = np.ascontiguousarray(csiga, np.complex128)
+            if c.size != self.ncell:
+                raise ValueError("conductivity must have Nx*Ny*Nz = %d elements" % self.ncell)
             L.check(self.lib.giem2g_op_set_sigma(self.h, c.ctypes.data))
         else:
-            L.check(self.lib.giem2g_op_set_sigma_dev(self.h, _ptr(csiga)))
+            (p,) = dev_ptrs(self.ncell, "c128", csiga)
+            L.check(self.lib.giem2g_op_set_sigma_dev(self.h, p))
+
+    def set_geometry(self, dx, dy, z):
+        """New cell sizes and layer borders (same Nx, Ny, Nz); the tensor must
+        be built again."""
+        z = np.ascontiguousarray(z, np.float64)
+        if z.size != self.Nz + 1:
+            raise ValueError("z must hold Nz + 1 borders")
+        L.check(self.lib.giem2g_op_set_geometry(self.h, float(dx), float(dy), z.ctypes.data))
+        self.dx, self.dy, self.z = float(dx), float(dy), z
 
     # --- apply ---------------------------------------------------------------
     def apply_host(self, v):
         v = np.ascontiguousarray(v, np.complex128)
+        if v.size != self.nvec:
+            raise ValueError("vector must have 3*Nx*Ny*Nz = %d elements" % self.nvec)
         out = np.empty_like(v)
         L.check(self.lib.giem2g_op_apply_host(self.h, v.ctypes.data, out.ctypes.data))
         return out
 
     def apply_dev(self, vin, vout):
         """FP64 matvec on HBM (asynchronous on the operator stream)."""
-        L.check(self.lib.giem2g_op_apply_dev(self.h, _ptr(vin), _ptr(vout)))
+        a, b = dev_ptrs(self.nvec, "c128", vin, vout)
+        L.check(self.lib.giem2g_op_apply_dev(self.h, a, b))
 
     def apply_adjoint_dev(self, vin, vout):
         """vout = A^H vin (complex128 torch tensors on the GPU)."""
-        L.check(self.lib.giem2g_op_apply_adjoint_dev(self.h, _ptr(vin), _ptr(vout)))
+        a, b = dev_ptrs(self.nvec, "c128", vin, vout)
+        L.check(self.lib.giem2g_op_apply_adjoint_dev(self.h, a, b))
 
     def apply_dev_f32(self, vin, vout):
-        L.check(self.lib.giem2g_op_apply_dev_f32(self.h, _ptr(vin), _ptr(vout)))
+        a, b = dev_ptrs(self.nvec, "c64", vin, vout)
+        L.check(self.lib.giem2g_op_apply_dev_f32(self.h, a, b))
 
     def apply_batch_dev(self, vin, vout):
         """vout[r] = A vin[r] for the rows of two (nrhs, 3 Nx Ny Nz) torch
         tensors on the GPU (complex128 or complex64); one tensor read serves up
         to 8 complex64 or 4 complex128 rows (Nz 8 to 32) or 2 (Nz 33 to 64)."""
-        import torch
-        assert vin.dim() == 2 and vin.shape == vout.shape and vin.is_contiguous() and vout.is_contiguous()
-        prec = 1 if vin.dtype == torch.complex64 else 0
-        L.check(self.lib.giem2g_op_apply_batch_dev(self.h, prec, int(vin.shape[0]), _ptr(vin), _ptr(vout)))
+        prec, dt = _prec_of(vin)
+        if vin.dim() != 2 or vin.shape[1] != self.nvec or vin.shape[0] < 1:
+            raise ValueError("vin must be (nrhs, %d)" % self.nvec)
+        nrhs = int(vin.shape[0])
+        a, b = dev_ptrs(nrhs * self.nvec, dt, vin, vout)
+        L.check(self.lib.giem2g_op_apply_batch_dev(self.h, prec, nrhs, a, b))
 
     def apply_pair_dev(self, in0, in1, out0, out1):
         """out0 = A in0, out1 = A in1 with one tensor read (torch tensors on
         the GPU, all complex128 or all complex64; Nz from 8 to 64)."""
-        import torch
-        prec = 1 if in0.dtype == torch.complex64 else 0
-        L.check(self.lib.giem2g_op_apply_pair_dev(self.h, prec, _ptr(in0), _ptr(in1), _ptr(out0),
-                                                   _ptr(out1)))
+        prec, dt = _prec_of(in0)
+        a, b, c, d = dev_ptrs(self.nvec, dt, in0, in1, out0, out1)
+        L.check(self.lib.giem2g_op_apply_pair_dev(self.h, prec, a, b, c, d))
 
     def sync(self):
         L.check(self.lib.giem2g_op_sync(self.h))
 
     def time_apply(self, vin, vout, reps=10, precision=0):
         ms = np.zeros(3)
-        L.check(self.lib.giem2g_op_time_apply(self.h, precision, _ptr(vin), _ptr(vout), reps,
-                                              ms.ctypes.data))
+        a, b = dev_ptrs(self.nvec, "c64" if precision == 1 else "c128", vin, vout)
+        L.check(self.lib.giem2g_op_time_apply(self.h, precision, a, b, reps, ms.ctypes.data))
         return ms
 
     def stats(self):
@@ -146,6 +216,25 @@ class IEOperator:
         keys = ["matvecs", "matvecs32", "dotprods", "t_apply", "t_fwd", "t_contr", "t_bwd", "t_dot",
                 "t_tensor", "t_tensor_fft", "device_bytes"]
         return dict(zip(keys, o.tolist()))
+
+    COUNTER_KEYS = ["matvecs", "matvecs32", "dotprods", "t_apply", "t_fwd", "t_contr", "t_bwd", "t_dot",
+                    "t_tensor", "t_tensor_fft", "device_bytes", "gs_passes", "krylov_bytes",
+                    "contr_launches_fp64", "contr_rhs_fp64", "t_contr_fp64",
+                    "contr_launches_c64", "contr_rhs_c64", "t_contr_c64", "profiling"]
+
+    def counters(self):
+        """All operator counters (giem2g_op_counters); times in seconds."""
+        o = np.zeros(len(self.COUNTER_KEYS))
+        L.check(self.lib.giem2g_op_counters(self.h, o.ctypes.data, len(o)))
+        return dict(zip(self.COUNTER_KEYS, o.tolist()))
+
+    def reset_counters(self):
+        L.check(self.lib.giem2g_op_reset_counters(self.h))
+
+    def set_profiling(self, on):
+        """Record HIP events around every apply stage and Gram-Schmidt pass
+        (their times appear in counters())."""
+        L.check(self.lib.giem2g_op_set_profiling(self.h, 1 if on else 0))
 
     # --- sources / solve -----------------------------------------------------
     def plane_wave(self, pol):
@@ -161,20 +250,23 @@ class IEOperator:
             Ebkg = np.ascontiguousarray(Ebkg, np.complex128)
             out = np.empty_like(Ebkg)
             e0 = None if E0 is None else np.ascontiguousarray(E0, np.complex128)
+            if Ebkg.size != self.nvec or (e0 is not None and e0.size != self.nvec):
+                raise ValueError("fields must have 3*Nx*Ny*Nz = %d elements" % self.nvec)
             L.check(self.lib.giem2g_solve(self.h, Ebkg.ctypes.data,
                                           None if e0 is None else e0.ctypes.data, C.byref(p),
                                           out.ctypes.data, C.byref(info)))
         else:
             import torch
             out = torch.empty_like(Ebkg)
-            L.check(self.lib.giem2g_solve_dev(self.h, _ptr(Ebkg), _ptr(E0), C.byref(p), _ptr(out),
-                                              C.byref(info)))
+            a, b, c = dev_ptrs(self.nvec, "c128", Ebkg, E0, out, allow_none=(1,))
+            L.check(self.lib.giem2g_solve_dev(self.h, a, b, C.byref(p), c, C.byref(info)))
         return out, info.as_dict()
 
     def scale(self, mode, vin, vout):
         """mode 0: vout = sqrt(Re sb) vin (rhs), 1: vin / a (field -> x),
         2: a vin (x -> field); torch complex128 tensors on the GPU."""
-        L.check(self.lib.giem2g_op_scale(self.h, mode, _ptr(vin), _ptr(vout)))
+        a, b = dev_ptrs(self.nvec, "c128", vin, vout)
+        L.check(self.lib.giem2g_op_scale(self.h, mode, a, b))
 
     def set_contraction(self, mode):
         """0 automatic (2 for Nz from 8 to 64, otherwise 1; complex64 at
@@ -192,10 +284,21 @@ class IEOperator:
     def learned_features(self, r, e, x, t=None):
         """Network input x [vox][8] of residual r and target t [vox][6] of
         correction e (torch tensors on the GPU)."""
-        L.check(self.lib.giem2g_op_learned_features(self.h, _ptr(r), _ptr(e), _ptr(x), _ptr(t)))
+        import torch
+        a, b = dev_ptrs(self.nvec, "c128", r, e, allow_none=(1,))
+        for k, (arr, ch) in enumerate(((x, 8), (t, 6))):
+            if arr is None and k == 1:
+                continue
+            if (not isinstance(arr, torch.Tensor) or not arr.is_cuda or arr.dtype != torch.float32
+                    or not arr.is_contiguous() or arr.numel() != ch * self.ncell):
+                raise ValueError("feature arrays must be contiguous float32 GPU tensors of %d x %d"
+                                 % (self.ncell, ch))
+        L.check(self.lib.giem2g_op_learned_features(self.h, a, b, x.data_ptr(),
+                                                    None if t is None else t.data_ptr()))
 
     def learned_apply(self, v, z):
-        L.check(self.lib.giem2g_op_learned_apply(self.h, _ptr(v), _ptr(z)))
+        a, b = dev_ptrs(self.nvec, "c128", v, z)
+        L.check(self.lib.giem2g_op_learned_apply(self.h, a, b))
 
     def solve_pair(self, Eb0, Eb1, E00=None, E01=None, outer_m=30, inner_m=30, max_restart=100,
                    tol=1e-8, inner_precision=0):
@@ -206,9 +309,9 @@ class IEOperator:
         infos = (L.SolveInfo * 2)()
         cnt = (C.c_longlong * 2)()
         o0, o1 = torch.empty_like(Eb0), torch.empty_like(Eb1)
+        a, b, c, d, e, f = dev_ptrs(self.nvec, "c128", Eb0, Eb1, E00, E01, o0, o1, allow_none=(2, 3))
         torch.cuda.synchronize()
-        L.check(self.lib.giem2g_solve_pair_dev(self.h, _ptr(Eb0), _ptr(Eb1), _ptr(E00), _ptr(E01),
-                                               C.byref(p), _ptr(o0), _ptr(o1), infos, cnt))
+        L.check(self.lib.giem2g_solve_pair_dev(self.h, a, b, c, d, C.byref(p), e, f, infos, cnt))
         return o0, o1, infos[0].as_dict(), infos[1].as_dict(), [cnt[0], cnt[1]]
 
     def history(self, maxlen=100000):
@@ -298,7 +401,8 @@ class ReceiverOperator:
 
     def set_sigma(self, csiga):
         if hasattr(csiga, "data_ptr"):
-            L.check(self.lib.giem2g_rc_set_anomaly_conductivity_dev(self.h, _ptr(csiga)))
+            (p,) = dev_ptrs(self.ie.ncell, "c128", csiga)
+            L.check(self.lib.giem2g_rc_set_anomaly_conductivity_dev(self.h, p))
         else:
             cs = np.ascontiguousarray(csiga, np.complex128)
             L.check(self.lib.giem2g_rc_set_anomaly_conductivity(self.h, cs.ctypes.data))
@@ -313,7 +417,9 @@ class ReceiverOperator:
 
     def recalculate_dev(self, E, Ea, Ha):
         """HBM tensors, ordered on the IE operator's stream."""
-        L.check(self.lib.giem2g_rc_recalculate_dev(self.h, _ptr(E), _ptr(Ea), _ptr(Ha)))
+        (e,) = dev_ptrs(self.ie.nvec, "c128", E)
+        a, b = dev_ptrs(self.nfield, "c128", Ea, Ha)
+        L.check(self.lib.giem2g_rc_recalculate_dev(self.h, e, a, b))
 
     def plane_wave(self, pol):
         """Normal field at the prepared receiver depths: complex (Nr, 6)."""

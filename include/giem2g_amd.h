@@ -75,7 +75,11 @@ void giem2g_calc_data_sizes(GIEM2G_ANOMALY_TYPE anomaly, GIEM2G_DATA_TYPE *data)
 void giem2g_prepare_ie_kernel(GIEM2G_ANOMALY_TYPE anomaly, GIEM2G_DATA_TYPE data);
 
 /* Replaces GIEM2G_CALC_INTEGRAL_GREEN_TENSOR (giem2g_interfaces.F90:144-174):
- * FP64 Hankel-filter tensor build on the GPU for angular frequency omega. */
+ * FP64 Hankel-filter tensor build on the GPU for angular frequency omega.
+ * As in the reference, the anomaly passed here sets the geometry of the
+ * build (dx, dy and the layer borders z, from which each layer's background
+ * layer Lnumber is derived, giem2g_interfaces.F90:158-166); its Nx, Ny, Nz
+ * must be those of the prepared operator. */
 void giem2g_calc_ie_kernel(void *ie_op, GIEM2G_BACKGROUND_TYPE bkg,
                            GIEM2G_ANOMALY_TYPE anomaly, double omega);
 
@@ -83,21 +87,30 @@ void giem2g_calc_ie_kernel(void *ie_op, GIEM2G_BACKGROUND_TYPE bkg,
 void giem2g_calc_fft_of_ie_kernel(void *ie_op);
 
 /* Replaces GIEM2G_SET_ANOMALY_CONDUCTIVITY (giem2g_interfaces.F90:190-201).
- * csiga: complex (Nx,Ny,Nz) host array.  The values are copied to the GPU
- * at this call (the reference aliases the array; call again after changing
- * it). */
+ * csiga: complex (Nx,Ny,Nz) host array, aliased as in the reference (199):
+ * the library keeps the pointer and every giem2g_apply_ie_operator reads the
+ * array's current values, so it must stay alive while the operator is
+ * applied.  The values are also uploaded at this call for the extension
+ * entry points of section 2, which use the last uploaded values;
+ * giem2g_op_set_sigma(_dev) replaces the alias by a copy. */
 void giem2g_set_anomaly_conductivity(void *ie_op, void *csiga);
 
 /* Replaces GIEM2G_APPLY_INTEGRAL_OPERATOR (giem2g_interfaces.F90:203-232):
- * out = A*in, both complex (Nx,Ny,Nz,3) HOST arrays (copied over PCIe). */
+ * out = A*in, both complex (Nx,Ny,Nz,3) HOST arrays (copied over PCIe),
+ * with the current values of the aliased conductivity. */
 void giem2g_apply_ie_operator(void *ie_op, void *in, void *out);
 
 /* Replaces GIEM2G_SET_LOGGER (giem2g_interfaces.F90:234-238). */
 void giem2g_set_logger(void (*logger)(const char *));
 
 /* Replaces GIEM2G_PRINT_STATS (giem2g_interfaces.F90:240-247): logs the
- * operator counters (fwd FFT / contraction / bwd FFT / apply averages) and
- * resets them. */
+ * operator counters in the reference's split (time for multiplications and
+ * for dot products, average dot product, forward FFT, backward FFT,
+ * contraction and apply, numbers of matrix-vector products and dot
+ * products; Integral_Equation_Module.F90:473-484, ie_solver_module.F90:
+ * 98-113) and resets them.  Times are HIP-event times of the applies and
+ * Gram-Schmidt passes made while profiling was on (giem2g_op_set_profiling)
+ * or timed by giem2g_op_time_apply. */
 void giem2g_print_stats(void *ie_op);
 
 /* Replaces GIEM2G_DELETE_OPERATOR (giem2g_interfaces.F90:248-254): frees
@@ -132,9 +145,15 @@ int giem2g_op_tensor_stage(void *ie_op, int kind);
  * 4*Nz*(Nz+1)/2 + 2*Nz*Nz complex, block b = kx + Nx*ky) to/from the host. */
 int giem2g_op_get_tensor(void *ie_op, double *G_host);
 int giem2g_op_put_tensor(void *ie_op, const double *G_host, int spectral);
-/* Copy the complex (Nx,Ny,Nz) conductivity from host memory / HBM. */
+/* Copy the complex (Nx,Ny,Nz) conductivity from host memory / HBM.  The
+ * per-cell scalings are derived from it and the background; a later
+ * background change derives them again from this copy. */
 int giem2g_op_set_sigma(void *ie_op, const double *csiga_host);
 int giem2g_op_set_sigma_dev(void *ie_op, const void *csiga_dev);
+/* New cell sizes dx, dy and layer borders z (Nz+1) for the same Nx, Ny, Nz
+ * (what giem2g_calc_ie_kernel takes from its anomaly argument): Lnumber and
+ * the scalings follow; the tensor must be built again. */
+int giem2g_op_set_geometry(void *ie_op, double dx, double dy, const double *z);
 /* out = A*in on HBM pointers, complex128 (the FP64 drop-in) or complex64. */
 int giem2g_op_apply_dev(void *ie_op, const void *in_dev, void *out_dev);
 int giem2g_op_apply_dev_f32(void *ie_op, const void *in_dev, void *out_dev);
@@ -160,6 +179,17 @@ int giem2g_op_sync(void *ie_op);
 /* Counters: [matvecs, matvecs32, dotprods, t_apply, t_fwd, t_contr, t_bwd,
  * t_dot, t_tensor, t_tensor_fft, device_bytes] */
 int giem2g_op_stats(void *ie_op, double *out11);
+/* The first n (at most 20) of: the 11 values of giem2g_op_stats, then
+ * Gram-Schmidt passes, bytes moved by the Krylov kernels, FP64 contraction
+ * launches, right-hand sides they served, their time (s), the same three for
+ * complex64, and 1 while profiling is on.  Times in seconds. */
+int giem2g_op_counters(void *ie_op, double *out, int n);
+int giem2g_op_reset_counters(void *ie_op);
+/* on != 0: record HIP events around the stages of every apply (forward
+ * FFT, contraction, inverse FFT) and around every Gram-Schmidt pass, on the
+ * streams they run on; the times reach the counters at the next query.
+ * Costs a few microseconds of host time per apply. */
+int giem2g_op_set_profiling(void *ie_op, int on);
 /* Per-kernel timing of one apply: [fwd FFT, contraction, bwd FFT] in ms,
  * measured with HIP events on the operator stream (precision 0 FP64, 1 FP32). */
 int giem2g_op_time_apply(void *ie_op, int precision, const void *in_dev,

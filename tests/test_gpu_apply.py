@@ -307,7 +307,8 @@ def test_reference_abi_lifecycle(oracle):
     lib.giem2g_set_logger(cb)
     lib.giem2g_print_stats(h)
     lib.giem2g_delete_ie_operator(h)
-    assert msgs and "matvecs 1" in msgs[0]
+    assert msgs and "Number of matrix-vector multiplications: 1 (1 FP64" in msgs[0]
+    assert "Number of dot products: 0" in msgs[0]
     om = oracle_model(oracle, m)
     Gs = om.tensor_fft(om.tensor_realspace())
     assert rel(out, om.apply(Gs, sa, v)) < 1e-12
