@@ -797,6 +797,68 @@ void or_tensor_element(const or_model *m, int lx, int ly, double *blk_d) {
   free(G1);
 }
 
+/* Rounding scale of or_tensor_element.  G is linear in the six weights of
+ * every tap, G = sum_K sum_c WT_c(K) s_c(lambda_K) B_c(lambda_K), with the
+ * scalings s_c = 1/lambda (DXX, DYY, DXY) or lambda (DX, DY, D0) of
+ * calc_ie_tensor_module.F90:269-274.  Two FP64 evaluations of the weights
+ * (the FFT deconvolution of VolumeBesselTransforms) differ by a few eps times
+ * the largest |WT_c| of each vector, at every tap alike (at taps of tiny
+ * lambda that is O(1) relative), so per packed entry this returns
+ *   S = sum over unmasked taps K and over c of
+ *       max_K' |WT_c(K')| |s_c(lambda_K)| |B_c(lambda_K)| / (4 pi):
+ * weight errors of delta max|WT_c| change G by at most delta S, however much
+ * the far-offset tap sums cancel. */
+void or_tensor_element_abssum(const or_model *m, int lx, int ly, double *out) {
+  int Nz = m->Nz, P = Nz * (Nz + 1) / 2, K, c, i, j;
+  double rmin = sqrt(m->dx * m->dx + m->dy * m->dy) / 2.0;
+  double lms[OR_NTAPS], WT[OR_NTAPS * 6], W0[6], Ws[OR_NTAPS * 6], Wmax[6] = {0};
+  int keep[OR_NTAPS];
+  cplx *G1 = malloc((size_t)6 * Nz * Nz * sizeof(cplx));
+  double *A = calloc((size_t)6 * Nz * Nz, sizeof(double));
+  or_vbt_lms(rmin, lms);
+  or_compute_weights(lx, ly, m->dx, m->dy, rmin, WT, W0);
+  for (K = 0; K < OR_NTAPS; K++) {
+    double Wm = 0, lm = lms[K];
+    for (c = 0; c < 6; c++) {
+      double t = fabs(WT[K + OR_NTAPS * c] / W0[c]);
+      if (c == 0 || t > Wm) Wm = t;
+    }
+    keep[K] = Wm > WT_THRESHOLD;
+    /* |scaling| of each weight at this tap */
+    Ws[6 * K + IE_DXX] = Ws[6 * K + IE_DYY] = Ws[6 * K + IE_DXY] = 1.0 / lm;
+    Ws[6 * K + IE_DX] = Ws[6 * K + IE_DY] = Ws[6 * K + IE_D0] = lm;
+    for (c = 0; c < 6; c++)
+      if (fabs(WT[K + OR_NTAPS * c]) > Wmax[c]) Wmax[c] = fabs(WT[K + OR_NTAPS * c]);
+  }
+  for (K = 0; K < OR_NTAPS; K++) {
+    if (!keep[K]) continue;
+    for (c = 0; c < 6; c++) {
+      double W[6] = {0, 0, 0, 0, 0, 0};
+      int64_t e;
+      if (Wmax[c] == 0) continue;
+      W[c] = 1.0;
+      memset(G1, 0, (size_t)6 * Nz * Nz * sizeof(cplx));
+      double_integral_U(m, lms[K], W, G1);
+      for (e = 0; e < (int64_t)6 * Nz * Nz; e++) A[e] += Wmax[c] * Ws[6 * K + c] * cabs(G1[e]);
+    }
+  }
+  for (j = 0; j < Nz; j++)
+    for (i = 0; i < Nz; i++) {
+      out[4 * P + i + Nz * j] = A[GIDX(EXZ, i, j)] / PI / 4.0;
+      out[4 * P + Nz * Nz + i + Nz * j] = A[GIDX(EYZ, i, j)] / PI / 4.0;
+    }
+  for (j = 0; j < Nz; j++)
+    for (i = 0; i <= j; i++) {
+      int pidx = i + j * (j + 1) / 2;
+      out[0 * P + pidx] = A[GIDX(EXX, i, j)] / PI / 4.0;
+      out[1 * P + pidx] = A[GIDX(EXY, i, j)] / PI / 4.0;
+      out[2 * P + pidx] = A[GIDX(EYY, i, j)] / PI / 4.0;
+      out[3 * P + pidx] = A[GIDX(EZZ, i, j)] / PI / 4.0;
+    }
+  free(G1);
+  free(A);
+}
+
 /* CalcIntegralGreenTensor (:54-192), single rank: every (lx,ly) offset */
 void or_tensor_realspace(const or_model *m, double *Greal, int nthreads) {
   int Nx = m->Nx, Ny = m->Ny;

@@ -66,6 +66,9 @@ void or_double_integral_U(const or_model *m, double lm, const double *W /*6*/,
 /* Real-space tensor (calc_ie_tensor_module.F90). */
 int64_t or_block_size(int Nz);
 void or_tensor_element(const or_model *m, int lx, int ly, double *blk);
+/* per entry, sum over taps K and weights c of max|WT_c| |dG/dWT_c(K)|: the change
+   of or_tensor_element per weight error of one unit of each vector's largest weight */
+void or_tensor_element_abssum(const or_model *m, int lx, int ly, double *out);
 void or_tensor_realspace(const or_model *m, double *Greal, int nthreads);
 
 /* Tensor spectrum (Integral_Equation_Module.F90:319-471). */

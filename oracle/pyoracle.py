@@ -60,6 +60,7 @@ def lib():
         L.or_block_size.restype = C.c_int64
         L.or_block_size.argtypes = [C.c_int]
         L.or_tensor_element.argtypes = [C.c_void_p, C.c_int, C.c_int, cdp]
+        L.or_tensor_element_abssum.argtypes = [C.c_void_p, C.c_int, C.c_int, dp]
         L.or_tensor_realspace.argtypes = [C.c_void_p, cdp, C.c_int]
         L.or_tensor_fft.argtypes = [C.c_void_p, cdp, cdp, C.c_int]
         L.or_apply.argtypes = [C.c_void_p, cdp, cdp, cdp, cdp, C.c_int]
@@ -162,6 +163,16 @@ class Model:
     def tensor_element(self, lx, ly):
         out = np.zeros(self.block_size, np.complex128)
         lib().or_tensor_element(self.ptr, lx, ly, out)
+        return out
+
+    def tensor_element_abssum(self, lx, ly):
+        """Per entry (real, block layout): sum over unmasked taps K and the
+        six weights c of max|WT_c| |dG/dWT_c(K)|, with WT the unscaled filter
+        weights.  Weight errors of delta times each vector's largest weight
+        (how two FP64 weight codes differ) change the element by at most
+        delta times this, however much the far-offset tap sums cancel."""
+        out = np.zeros(self.block_size, np.float64)
+        lib().or_tensor_element_abssum(self.ptr, lx, ly, out)
         return out
 
     def tensor_realspace(self, nthreads=None):

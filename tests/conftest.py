@@ -25,6 +25,16 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+def tensor_block_tol(om, lx, ly):
+    """Bound on the difference of two FP64 codes' real-space tensor block at
+    offset (lx, ly): 16 eps |S|, with S the oracle's weight-error scale
+    (tensor_element_abssum: the change of every entry per weight error of one
+    unit of each weight vector's largest value).  Against blocks built with
+    the reference's quad-precision weights, the reference's own FP64 blocks
+    and the oracle's stay within 2.2 eps |S| at every fixture offset."""
+    return 16 * np.finfo(np.float64).eps * np.linalg.norm(om.tensor_element_abssum(lx, ly))
+
+
 @pytest.fixture(scope="session")
 def golden():
     path = os.path.join(ROOT, "tests", "golden", "reference_vectors.npz")
