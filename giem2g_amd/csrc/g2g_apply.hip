@@ -1462,6 +1462,101 @@ void op_apply_batch(Operator& op, OpBuffers<R>& b, int nrhs, const cx_t<R>* vin,
 template void op_apply_batch<double>(Operator&, OpBuffers<double>&, int, const zd*, zd*, hipStream_t);
 template void op_apply_batch<float>(Operator&, OpBuffers<float>&, int, const zf*, zf*, hipStream_t);
 
+// ------------------------------------------------ allocation-free batch ---
+// Batched applies for callers that must not allocate device memory (code
+// under graph capture, serving loops): the x-transformed rows X and the
+// spectrum slots live in one file-scope device array, `scratch`, instead of
+// the operator's buffers, which the first batched apply allocates.  Each GPU
+// holds its own copy of the array from the moment the library's code is
+// loaded on it until the process ends.  Host code never passes the array's
+// name to a kernel (on the host it is a stand-in, not a device address): the
+// address of each device's copy is looked up once, when the first operator
+// is created on that device, and kept in scratch_dev[device].
+static __device__ __align__(256) zd scratch[NOALLOC_SCRATCH_BYTES / sizeof(zd)];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void* scratch_dev[MAX_DEVICES] = {};
+}  // namespace
+
+void op_noalloc_setup(int device) {
+  G2G_CHECK(device >= 0 && device < MAX_DEVICES, -3, "device index beyond the library's table");
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (scratch_dev[device]) return;
+  void* p = nullptr;
+  G2G_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  G2G_CHECK(p != nullptr, -100, "no device address for the scratch array");
+  scratch_dev[device] = p;
+}
+
+template <typename R> int op_noalloc_batch_max(const Operator& op, const OpBuffers<R>& b, size_t* need1) {
+  const size_t es = sizeof(cx_t<R>);
+  const size_t nX = (size_t)b.nchunk * b.NR * op.Ny * 2 * op.Nx;
+  const size_t nS = (size_t)4 * op.Nx * op.Ny * 3 * op.Nz;
+  if (need1) *need1 = es * (nX + nS);
+  const int kmax = std::min(op_batch_max(op), sizeof(R) == 8 ? 4 : 8);
+  int k = 0;
+  for (int kk : {1, 2, 4, 8})
+    if (kk <= kmax && es * (nX + nS * kk) <= NOALLOC_SCRATCH_BYTES) k = kk;
+  return k;
+}
+
+template <typename R>
+void op_apply_batch_noalloc(Operator& op, OpBuffers<R>& b, int nrhs, const cx_t<R>* vin, cx_t<R>* vout,
+                            hipStream_t s) {
+  using T2 = cx_t<R>;
+  G2G_CHECK(nrhs >= 1, -3, "batched apply needs at least one vector");
+  G2G_CHECK(b.X != nullptr, -4,
+            "complex64 operator not made yet: make it with a complex64 apply or batch before the "
+            "allocation-free apply");
+  size_t need1 = 0;
+  const int kcap = op_noalloc_batch_max<R>(op, b, &need1);
+  if (kcap < 1) {
+    char msg[200];
+    std::snprintf(msg, sizeof msg,
+                  "grid too large for the allocation-free apply: one vector needs %zu bytes of scratch, "
+                  "the library holds %zu",
+                  need1, (size_t)NOALLOC_SCRATCH_BYTES);
+    throw Error(-3, msg);
+  }
+  G2G_CHECK(op.have_tensor && op.have_sigma, -4, "operator not ready: tensor or conductivity missing");
+  G2G_CHECK(!op.prof.on, -3, "the profiling mode creates events; turn it off for the allocation-free apply");
+  int dev = 0;
+  G2G_HIP(hipGetDevice(&dev));
+  G2G_CHECK(dev >= 0 && dev < MAX_DEVICES && scratch_dev[dev] != nullptr, -4,
+            "no operator was created on the current device");
+  // the operator's buffers with X and the spectrum slots moved into scratch;
+  // the single-vector path's spectrum S is the first slot
+  OpBuffers<R> bl = b;
+  T2* base = static_cast<T2*>(scratch_dev[dev]);
+  const size_t nX = (size_t)b.nchunk * b.NR * op.Ny * 2 * op.Nx;
+  bl.X = base;
+  bl.SB = base + nX;
+  bl.S = bl.SB;
+  bl.sb_cap = kcap;
+  const int64_t nv = 3 * (int64_t)op.ncell();
+  for (int r0 = 0; r0 < nrhs;) {
+    const int k = std::min(kcap, nrhs - r0);
+    if (k == 1) {
+      op_apply<R>(op, bl, vin + nv * r0, vout + nv * r0, s, false);
+    } else {
+      const T2* vins[8];
+      T2* vouts[8];
+      for (int r = 0; r < k; r++) {
+        vins[r] = vin + nv * (r0 + r);
+        vouts[r] = vout + nv * (r0 + r);
+      }
+      op_apply_k<R>(op, bl, k, vins, vouts, s);
+    }
+    r0 += k;
+  }
+}
+template int op_noalloc_batch_max<double>(const Operator&, const OpBuffers<double>&, size_t*);
+template int op_noalloc_batch_max<float>(const Operator&, const OpBuffers<float>&, size_t*);
+template void op_apply_batch_noalloc<double>(Operator&, OpBuffers<double>&, int, const zd*, zd*, hipStream_t);
+template void op_apply_batch_noalloc<float>(Operator&, OpBuffers<float>&, int, const zf*, zf*, hipStream_t);
+
 // ============================================================ receivers ==
 // Continuation of the solution to receivers (ReCalculation, APPLY_RC_E_OP,
 // APPLY_RC_H_OP, Continuation_Function_Module.F90:198-481): j = dsig * E is

@@ -116,6 +116,9 @@ void op_init(Operator& op, int Nx, int Ny, int Nz, double dx, double dy, const d
   op.stream = nullptr;  // the default stream orders with torch's default stream
   for (auto& e : op.ev) G2G_HIP(hipEventCreate(&e));
   op_alloc_buffers<double>(op, op.d64, true);
+  int dev = 0;
+  G2G_HIP(hipGetDevice(&dev));
+  op_noalloc_setup(dev);  // the allocation-free apply's scratch address on this device
 }
 
 void op_fini(Operator& op) {
@@ -611,6 +614,26 @@ int giem2g_op_apply_batch_dev(void* p, int precision, int nrhs, const void* in, 
     } else {
       if (!op.have_g32) op_make_g32(op);
       op_apply_batch<float>(op, op.d32, nrhs, static_cast<const zf*>(in), static_cast<zf*>(out), op.stream);
+      op.cnt.matvecs32 += nrhs;
+    }
+  });
+}
+
+int giem2g_op_apply_batch_dev_noalloc(void* p, int precision, int nrhs, const void* in, void* out) {
+  return guard([&] {
+    Operator& op = H(p)->op;
+    G2G_CHECK(precision == 0 || precision == 1, -3, "precision must be 0 or 1");
+    G2G_CHECK(nrhs >= 1, -3, "nrhs must be at least 1");
+    if (precision == 0) {
+      op_apply_batch_noalloc<double>(op, op.d64, nrhs, static_cast<const zd*>(in), static_cast<zd*>(out),
+                                     op.stream);
+      op.cnt.matvecs += nrhs;
+    } else {
+      G2G_CHECK(op.have_g32, -4,
+                "complex64 operator not made yet: make it with a complex64 apply or batch before the "
+                "allocation-free apply");
+      op_apply_batch_noalloc<float>(op, op.d32, nrhs, static_cast<const zf*>(in), static_cast<zf*>(out),
+                                    op.stream);
       op.cnt.matvecs32 += nrhs;
     }
   });

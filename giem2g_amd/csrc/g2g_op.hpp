@@ -217,6 +217,20 @@ int op_batch_max(const Operator& op);
 template <typename R>
 void op_apply_batch(Operator& op, OpBuffers<R>& b, int nrhs, const cx_t<R>* vin, cx_t<R>* vout,
                     hipStream_t s);
+// Allocation-free batched apply: the same products as op_apply_batch with
+// the FFT intermediate and the spectrum slots in a file-scope device array
+// of NOALLOC_SCRATCH_BYTES per GPU (batches of as many vectors per tensor
+// read as fit in it).  op_noalloc_setup(device) looks up that device's copy;
+// op_init calls it with the operator's device current.  Calls that overlap
+// on one GPU share the array and race.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(256) << 20;
+void op_noalloc_setup(int device);
+// vectors per tensor read the array allows (0 when one vector does not fit;
+// *need1 = bytes one vector needs)
+template <typename R> int op_noalloc_batch_max(const Operator& op, const OpBuffers<R>& b, size_t* need1);
+template <typename R>
+void op_apply_batch_noalloc(Operator& op, OpBuffers<R>& b, int nrhs, const cx_t<R>* vin, cx_t<R>* vout,
+                            hipStream_t s);
 // per-cell scalings from the conductivity d_csiga (HBM, complex (Nx,Ny,Nz));
 // keeps a copy in op.csiga (d_csiga may be op.csiga itself)
 void op_set_sigma(Operator& op, const zd* d_csiga);

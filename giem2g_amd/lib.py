@@ -21,7 +21,7 @@ EXTENDED_SYMBOLS = [
     "giem2g_op_set_background", "giem2g_op_build_tensor", "giem2g_op_tensor_stage",
     "giem2g_op_get_tensor", "giem2g_op_put_tensor", "giem2g_op_set_sigma",
     "giem2g_op_set_sigma_dev", "giem2g_op_apply_dev", "giem2g_op_apply_dev_f32",
-    "giem2g_op_apply_pair_dev", "giem2g_op_apply_batch_dev",
+    "giem2g_op_apply_pair_dev", "giem2g_op_apply_batch_dev", "giem2g_op_apply_batch_dev_noalloc",
     "giem2g_op_apply_host", "giem2g_op_stream", "giem2g_op_sync", "giem2g_op_stats",
     "giem2g_op_time_apply", "giem2g_plane_wave", "giem2g_solve", "giem2g_solve_dev",
     "giem2g_last_history", "giem2g_unet_create", "giem2g_unet_destroy", "giem2g_unet_nparams",
@@ -115,6 +115,7 @@ def load():
         "giem2g_op_apply_dev_f32": (i, [vp, vp, vp]),
         "giem2g_op_apply_pair_dev": (i, [vp, i, vp, vp, vp, vp]),
         "giem2g_op_apply_batch_dev": (i, [vp, i, i, vp, vp]),
+        "giem2g_op_apply_batch_dev_noalloc": (i, [vp, i, i, vp, vp]),
         "giem2g_op_apply_host": (i, [vp, vp, vp]),
         "giem2g_op_stream": (i, [vp, C.POINTER(vp)]),
         "giem2g_op_sync": (i, [vp]),

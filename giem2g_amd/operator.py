@@ -194,6 +194,18 @@ class IEOperator:
         a, b = dev_ptrs(nrhs * self.nvec, dt, vin, vout)
         L.check(self.lib.giem2g_op_apply_batch_dev(self.h, prec, nrhs, a, b))
 
+    def apply_batch_dev_noalloc(self, vin, vout):
+        """As apply_batch_dev, with the FFT intermediate and the spectrum slots
+        in the library's fixed 256 MiB device array instead of allocations
+        (no hipMalloc in the call).  Grids whose one vector does not fit are
+        rejected; complex64 needs the complex64 operator made beforehand."""
+        prec, dt = _prec_of(vin)
+        if vin.dim() != 2 or vin.shape[1] != self.nvec or vin.shape[0] < 1:
+            raise ValueError("vin must be (nrhs, %d)" % self.nvec)
+        nrhs = int(vin.shape[0])
+        a, b = dev_ptrs(nrhs * self.nvec, dt, vin, vout)
+        L.check(self.lib.giem2g_op_apply_batch_dev_noalloc(self.h, prec, nrhs, a, b))
+
     def apply_pair_dev(self, in0, in1, out0, out1):
         """out0 = A in0, out1 = A in1 with one tensor read (torch tensors on
         the GPU, all complex128 or all complex64; Nz from 8 to 64)."""

@@ -171,6 +171,21 @@ int giem2g_op_apply_pair_dev(void *ie_op, int precision, const void *in0_dev, co
  * zero layers inside the kernel); Nz below 8 applies the vectors one by one.  No reference counterpart (the
  * reference applies the operator to one vector at a time). */
 int giem2g_op_apply_batch_dev(void *ie_op, int precision, int nrhs, const void *in_dev, void *out_dev);
+/* Allocation-free variant of giem2g_op_apply_batch_dev, with the same
+ * arguments and products.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The call allocates and
+ * frees nothing.  Its FFT intermediate and spectrum slots live in a file-scope
+ * device array of 256 MiB per GPU, held from library load to process end, and
+ * as many vectors share one tensor read as fit in it (128x128x32: two
+ * complex128 or four complex64).  A grid whose single vector does not fit
+ * (for example 256x256x32 complex128) is rejected with -3, and so is a call
+ * with the profiling mode on.  For precision 1 the complex64 operator must
+ * already exist (any earlier complex64 apply makes it); otherwise -4.  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (two operators on different streams) race. */
+int giem2g_op_apply_batch_dev_noalloc(void *ie_op, int precision, int nrhs, const void *in_dev,
+                                      void *out_dev);
 /* Same on host pointers (PCIe copies included). */
 int giem2g_op_apply_host(void *ie_op, const double *in, double *out);
 /* Stream used by the operator (hipStream_t) and device synchronisation. */
