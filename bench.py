@@ -135,7 +135,7 @@ def solve_record(m, dt, steps, info, c0, c1):
                          "gram_schmidt": (c1["t_dot"] - c0["t_dot"]) / steps}}
 
 
-def c2_headline(torch, gop, models, steps, warmup):
+def c2_headline(torch, gop, models, steps, warmup, clock_s=CLOCK_WARMUP_S):
     """The metric: K timed solves each of plain FP64, mixed (complex64 inner)
     and mixed+learned on the C2 model, EY polarization, profiling on (HIP
     events around every apply stage and Gram-Schmidt pass)."""
@@ -149,7 +149,7 @@ def c2_headline(torch, gop, models, steps, warmup):
     E = torch.from_numpy(op.plane_wave(1)).cuda()
     v = torch.randn(op.nvec, dtype=torch.complex128, device="cuda")
     out = torch.empty_like(v)
-    nwarm = clock_warmup(torch, op, v, out)
+    nwarm = clock_warmup(torch, op, v, out, clock_s)
     del v, out
     runs = [("fp64", 0, 0), ("mixed", 1, 0)]
     net = None
@@ -483,6 +483,8 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the 128x128x64 multi-frequency run")
     ap.add_argument("--c3-freqs", type=int, default=16, help="number of C3 frequencies")
     ap.add_argument("--headline-only", action="store_true", help="only the C2 solves (profiling runs)")
+    ap.add_argument("--clock-warmup", type=float, default=CLOCK_WARMUP_S,
+                    help="seconds of FP64 applies before the timed C2 solves (0 for counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 1 GiB device copy (HBM counter calibration)")
     a = ap.parse_args()
@@ -500,7 +502,7 @@ def main():
         dst.copy_(src)  # reads and writes exactly 1 GiB
         torch.cuda.synchronize()
         del src, dst
-    m2, op2, E2, head = c2_headline(torch, gop, models, a.steps, a.warmup)
+    m2, op2, E2, head = c2_headline(torch, gop, models, a.steps, a.warmup, a.clock_warmup)
     top = head.get("mixed+learned", head["mixed"])
     top_name = "mixed+learned" if "mixed+learned" in head else "mixed"
     b32 = algorithmic_bytes(m2.Nx, m2.Ny, m2.Nz, 8)

@@ -35,7 +35,9 @@ template <typename R> constexpr int nr_for(int L) {
   const int n = kBytes / (L * 2 * (int)sizeof(R));
   return n < 1 ? 1 : (n > 8 ? 8 : n);
 }
-template <typename R> int nr_rt(int L) { return nr_for<R>(L); }
+// rows per block of a pass over rows of length L: that of L's buffer
+// capacity (L itself for powers of two, the next power of two otherwise)
+template <typename R> int nr_rt(int L) { return nr_for<R>(len_capacity(L)); }
 template <typename R> int nr_half_rt(int L) { return nr_for<R>(L / 2); }
 // Nz for which the matrix-core contraction of nrhs right-hand sides exists:
 // up to 64 layers for one or two, up to 32 for four or eight; at least C3_MIN_NZ
@@ -45,14 +47,17 @@ inline bool c3_supported(int nz, int nrhs) { return nz >= C3_MIN_NZ && nz <= (nr
 // Half-length passes (two N-point transforms per zero-padded 2N-point one)
 // for rows of 32 points or more.  Measured against the 2N-point passes: at
 // 256x256 (512-point rows) the FP64 and complex64 passes together 15% faster,
-// at 128x128 FP64 5-7% faster and complex64 within 4%.
-template <typename R> bool use_half_fft(int Lx, int Ly) { return Lx >= 32 && Ly >= 32; }
+// at 128x128 FP64 5-7% faster and complex64 within 4%.  Powers of two only.
+template <typename R> bool use_half_fft(int Lx, int Ly) {
+  return Lx >= 32 && Ly >= 32 && is_pow2(Lx) && is_pow2(Ly);
+}
 
 struct Geo {
   int Nx, Ny, Nz, Nz3;
   int nrx_log;  // X tile rows = 1 << nrx_log
   int64_t nc;   // cells
   int slot[3];
+  FftPlan px, py;  // factors of 2Nx and 2Ny (used where they are not powers of two)
 };
 
 __device__ inline int64_t xidx(const Geo& g, int cz, int y, int kx) {
@@ -63,13 +68,15 @@ __device__ inline int64_t xidx(const Geo& g, int cz, int y, int kx) {
 // Tensor elements are loaded with ld_stream (each apply reads them once):
 // FP64, 128x128x32, contraction 6% and inverse y pass 17% faster.
 
-template <typename T2, int L>
-__device__ inline void load_tw(T2* tw, const T2* g) {
+template <typename T2>
+__device__ inline void load_tw(T2* tw, const T2* g, int L) {
   for (int i = threadIdx.x; i < L; i += NT) tw[i] = g[i];
 }
 
 // ---------------------------------------------------------------- K1 ----
-template <typename R, int L>
+// L is the buffer capacity; with RT the rows are 2Nx <= L points long
+// (a length that is not a power of two, transformed by its factor plan).
+template <typename R, int L, bool RT = false>
 __global__ __launch_bounds__(NT) void k_fwd_x(const cx_t<R>* __restrict__ v,
                                               const cx_t<R>* __restrict__ gsig,
                                               cx_t<R>* __restrict__ X,
@@ -80,7 +87,8 @@ __global__ __launch_bounds__(NT) void k_fwd_x(const cx_t<R>* __restrict__ v,
   __shared__ T2 buf[NR * LS];
   __shared__ T2 tw[L];
   const int y = blockIdx.x, chunk = blockIdx.y, Nx = g.Nx;
-  load_tw<T2, L>(tw, twg);
+  const int Lr = RT ? 2 * Nx : L;
+  load_tw<T2>(tw, twg, Lr);
   for (int i = threadIdx.x; i < NR * Nx; i += NT) {
     const int s = i / Nx, x = i - s * Nx, cz = chunk * NR + s;
     T2 val = mk<T2>(0, 0);
@@ -93,10 +101,10 @@ __global__ __launch_bounds__(NT) void k_fwd_x(const cx_t<R>* __restrict__ v,
     buf[s * LS + x + Nx] = mk<T2>(0, 0);
   }
   __syncthreads();
-  lds_fft<T2, L, NR, LS, NT, -1>(buf, tw);
+  lds_fft_any<T2, L, NR, LS, NT, -1, RT>(buf, tw, g.px);
   // X tile for this chunk is contiguous: [y][kx][s]
-  T2* dst = X + (((int64_t)chunk * g.Ny + y) * L) * NR;
-  for (int i = threadIdx.x; i < NR * L; i += NT) {
+  T2* dst = X + (((int64_t)chunk * g.Ny + y) * Lr) * NR;
+  for (int i = threadIdx.x; i < NR * Lr; i += NT) {
     const int kx = i / NR, s = i - kx * NR;
     dst[i] = buf[s * LS + kx];
   }
@@ -106,7 +114,7 @@ __global__ __launch_bounds__(NT) void k_fwd_x(const cx_t<R>* __restrict__ v,
 // FULL: every point of the 2Nx x 2Ny spectrum (the receiver continuation);
 // otherwise the x- and y-Nyquist lines, which the IE operator never uses,
 // are skipped.
-template <typename R, int L, bool FULL = false>
+template <typename R, int L, bool FULL = false, bool RT = false>
 __global__ __launch_bounds__(NT) void k_fwd_y(const cx_t<R>* __restrict__ X,
                                               cx_t<R>* __restrict__ S,
                                               const cx_t<R>* __restrict__ twg, Geo g) {
@@ -116,23 +124,24 @@ __global__ __launch_bounds__(NT) void k_fwd_y(const cx_t<R>* __restrict__ X,
   __shared__ T2 buf[NR * LS];
   __shared__ T2 tw[L];
   const int kx = blockIdx.x, chunk = blockIdx.y, Ny = g.Ny;
+  const int Lr = RT ? 2 * Ny : L;
   if (!FULL && kx == g.Nx) return;  // x-Nyquist column is never used
-  load_tw<T2, L>(tw, twg);
+  load_tw<T2>(tw, twg, Lr);
   for (int i = threadIdx.x; i < NR * Ny; i += NT) {
     const int yy = i / NR, s = i - yy * NR, cz = chunk * NR + s;
     buf[s * LS + yy] = (cz < g.Nz3) ? X[xidx(g, cz, yy, kx)] : mk<T2>(0, 0);
     buf[s * LS + yy + Ny] = mk<T2>(0, 0);
   }
   __syncthreads();
-  lds_fft<T2, L, NR, LS, NT, -1>(buf, tw);
-  for (int i = threadIdx.x; i < NR * L; i += NT) {
+  lds_fft_any<T2, L, NR, LS, NT, -1, RT>(buf, tw, g.py);
+  for (int i = threadIdx.x; i < NR * Lr; i += NT) {
     const int ky = i / NR, s = i - ky * NR, cz = chunk * NR + s;
-    if ((FULL || ky != Ny) && cz < g.Nz3) S[((int64_t)kx * L + ky) * g.Nz3 + cz] = buf[s * LS + ky];
+    if ((FULL || ky != Ny) && cz < g.Nz3) S[((int64_t)kx * Lr + ky) * g.Nz3 + cz] = buf[s * LS + ky];
   }
 }
 
 // ---------------------------------------------------------------- K4 ----
-template <typename R, int L, bool FULL = false>
+template <typename R, int L, bool FULL = false, bool RT = false>
 __global__ __launch_bounds__(NT) void k_inv_y(const cx_t<R>* __restrict__ S,
                                               cx_t<R>* __restrict__ X,
                                               const cx_t<R>* __restrict__ twg, Geo g) {
@@ -142,6 +151,7 @@ __global__ __launch_bounds__(NT) void k_inv_y(const cx_t<R>* __restrict__ S,
   __shared__ T2 buf[NR * LS];
   __shared__ T2 tw[L];
   const int kx = blockIdx.x, chunk = blockIdx.y, Ny = g.Ny;
+  const int Lr = RT ? 2 * Ny : L;
   if (!FULL && kx == g.Nx) {  // x-Nyquist: zero (InitialTransposeRepack, Distributed_FFT.F90:723-724)
     for (int i = threadIdx.x; i < NR * Ny; i += NT) {
       const int yy = i / NR, s = i - yy * NR, cz = chunk * NR + s;
@@ -149,13 +159,13 @@ __global__ __launch_bounds__(NT) void k_inv_y(const cx_t<R>* __restrict__ S,
     }
     return;
   }
-  load_tw<T2, L>(tw, twg);
-  for (int i = threadIdx.x; i < NR * L; i += NT) {
+  load_tw<T2>(tw, twg, Lr);
+  for (int i = threadIdx.x; i < NR * Lr; i += NT) {
     const int ky = i / NR, s = i - ky * NR, cz = chunk * NR + s;
-    buf[s * LS + ky] = (cz < g.Nz3) ? S[((int64_t)kx * L + ky) * g.Nz3 + cz] : mk<T2>(0, 0);
+    buf[s * LS + ky] = (cz < g.Nz3) ? S[((int64_t)kx * Lr + ky) * g.Nz3 + cz] : mk<T2>(0, 0);
   }
   __syncthreads();
-  lds_fft<T2, L, NR, LS, NT, +1>(buf, tw);
+  lds_fft_any<T2, L, NR, LS, NT, +1, RT>(buf, tw, g.py);
   for (int i = threadIdx.x; i < NR * Ny; i += NT) {
     const int yy = i / NR, s = i - yy * NR, cz = chunk * NR + s;
     if (cz < g.Nz3) X[xidx(g, cz, yy, kx)] = buf[s * LS + yy];
@@ -163,7 +173,7 @@ __global__ __launch_bounds__(NT) void k_inv_y(const cx_t<R>* __restrict__ S,
 }
 
 // ---------------------------------------------------------------- K5 ----
-template <typename R, int L>
+template <typename R, int L, bool RT = false>
 __global__ __launch_bounds__(NT) void k_inv_x(const cx_t<R>* __restrict__ X,
                                               const cx_t<R>* __restrict__ vin,
                                               const cx_t<R>* __restrict__ asiga,
@@ -177,14 +187,15 @@ __global__ __launch_bounds__(NT) void k_inv_x(const cx_t<R>* __restrict__ X,
   __shared__ T2 buf[NR * LS];
   __shared__ T2 tw[L];
   const int y = blockIdx.x, chunk = blockIdx.y, Nx = g.Nx;
-  load_tw<T2, L>(tw, twg);
-  const T2* src = X + (((int64_t)chunk * g.Ny + y) * L) * NR;
-  for (int i = threadIdx.x; i < NR * L; i += NT) {
+  const int Lr = RT ? 2 * Nx : L;
+  load_tw<T2>(tw, twg, Lr);
+  const T2* src = X + (((int64_t)chunk * g.Ny + y) * Lr) * NR;
+  for (int i = threadIdx.x; i < NR * Lr; i += NT) {
     const int kx = i / NR, s = i - kx * NR;
     buf[s * LS + kx] = src[i];
   }
   __syncthreads();
-  lds_fft<T2, L, NR, LS, NT, +1>(buf, tw);
+  lds_fft_any<T2, L, NR, LS, NT, +1, RT>(buf, tw, g.px);
   for (int i = threadIdx.x; i < NR * Nx; i += NT) {
     const int s = i / Nx, x = i - s * Nx, cz = chunk * NR + s;
     if (cz < g.Nz3) {
@@ -594,6 +605,156 @@ __global__ __launch_bounds__(NT) void k_contract(cx_t<R>* __restrict__ S,
 #undef G2G_PREFETCH
 #undef G2G_ST
 #undef G2G_STASH
+}
+
+// Layer counts above 64 (up to MAX_NZ): the products of k_contract with
+// every tensor component staged through LDS in tiles of TJ source columns,
+// which are contiguous in the packed-upper and in the column-major storage
+// alike.  Thread (row i, part) owns receiver layer i of all four points.
+// For a symmetric component S a tile of columns [j0, j1) gives row i the
+// upper terms S(i,j) v(j), j >= i, from column j, and, when j0 <= i < j1,
+// the lower terms S(k,i) v(k), k < i, from column i; for a full component A
+// (xz, yz) row i takes A(i,j) v_z(j) of the tile's columns and, when i is in
+// the tile, the transposed terms A(k,i) v(k) over all k from column i.  So
+// every stored entry enters each product it belongs to exactly once.
+template <typename R>
+__global__ __launch_bounds__(NT) void k_contract_tiled(cx_t<R>* __restrict__ S,
+                                                       const cx_t<R>* __restrict__ G, Geo g,
+                                                       int64_t BS, int split_log, int TJ) {
+  using T2 = cx_t<R>;
+  extern __shared__ unsigned char smem_raw[];
+  const int Nz = g.Nz, Nz3 = g.Nz3;
+  const int64_t P = (int64_t)Nz * (Nz + 1) / 2;
+  T2* vin = reinterpret_cast<T2*>(smem_raw);  // [4][3Nz]
+  T2* tile = vin + 4 * Nz3;                   // at most TJ*Nz entries
+  const int b = blockIdx.x, kx = b % g.Nx, ky = b / g.Nx;
+  const int Nx2 = 2 * g.Nx, Ny2 = 2 * g.Ny;
+  const int gNx = g.Nx, gNy = g.Ny;
+#define PX(p) (((p) & 1) ? (kx == 0 ? gNx : Nx2 - kx) : kx)
+#define PY(p) (((p) & 2) ? (ky == 0 ? gNy : Ny2 - ky) : ky)
+#define VAL(p) ((!((p) & 1) || kx > 0) && (!((p) & 2) || ky > 0))
+  const T2* blk = G + (int64_t)b * BS;
+  const T2 zero = mk<T2>(0, 0);
+  for (int t = threadIdx.x; t < 4 * Nz3; t += NT) {
+    const int p = t / Nz3, e = t - p * Nz3;
+    T2 vv = zero;
+    if (VAL(p)) vv = S[((int64_t)PX(p) * Ny2 + PY(p)) * Nz3 + e];
+    vin[t] = vv;
+  }
+  const int split = 1 << split_log;
+  const int part = threadIdx.x & (split - 1);
+  const int i = threadIdx.x >> split_log;
+  const bool active = i < Nz;
+#define G2G_SG(p) (((p) & 1) ? R(-1) : R(1))
+#define G2G_MR(p) (((p) & 2) ? R(-1) : R(1))
+#define G2G_ACC(p) T2 ax##p = zero, ay##p = zero, az##p = zero;
+  G2G_FOR4(G2G_ACC)
+#undef G2G_ACC
+  for (int comp = 0; comp < 6; comp++) {
+    const bool sym = comp < 4;
+    const T2* src = sym ? blk + comp * P : blk + 4 * P + (int64_t)(comp - 4) * Nz * Nz;
+    // t1: the component acting on its source (x for xx, y for xy and yy, z
+    // for zz, xz and yz); t2: xy on x, or the transposed xz / yz
+    const int off1 = comp == 0 ? 0 : (comp == 3 || comp >= 4 ? 2 * Nz : Nz);
+    const int off2 = comp == 5 ? Nz : 0;
+#define G2G_T(p) T2 t1##p = zero, t2##p = zero;
+    G2G_FOR4(G2G_T)
+#undef G2G_T
+    for (int j0 = 0; j0 < Nz; j0 += TJ) {
+      const int j1 = min(j0 + TJ, Nz);
+      const int64_t base = sym ? (int64_t)j0 * (j0 + 1) / 2 : (int64_t)j0 * Nz;
+      const int64_t len = sym ? (int64_t)j1 * (j1 + 1) / 2 - base : (int64_t)(j1 - j0) * Nz;
+      __syncthreads();  // the previous tile (and, first time, vin) is consumed / written
+      for (int64_t t = threadIdx.x; t < len; t += NT) tile[t] = ld_stream(src + base + t);
+      __syncthreads();
+      if (active) {
+        if (sym) {
+          for (int j = max(j0, i) + part; j < j1; j += split) {
+            const T2 sv = tile[i + (int64_t)j * (j + 1) / 2 - base];
+#define G2G_UP(p)                                     \
+  {                                                   \
+    const T2* v = vin + (p) * Nz3;                    \
+    cmac(t1##p, sv, v[off1 + j]);                     \
+    if (comp == 1) cmac(t2##p, sv, v[j]);             \
+  }
+            G2G_FOR4(G2G_UP)
+#undef G2G_UP
+          }
+          if (i >= j0 && i < j1) {
+            const T2* col = tile + (int64_t)i * (i + 1) / 2 - base;
+            for (int k = part; k < i; k += split) {
+              const T2 sv = col[k];
+#define G2G_LO(p)                                     \
+  {                                                   \
+    const T2* v = vin + (p) * Nz3;                    \
+    cmac(t1##p, sv, v[off1 + k]);                     \
+    if (comp == 1) cmac(t2##p, sv, v[k]);             \
+  }
+              G2G_FOR4(G2G_LO)
+#undef G2G_LO
+            }
+          }
+        } else {
+          for (int j = j0 + part; j < j1; j += split) {
+            const T2 a = tile[(int64_t)(j - j0) * Nz + i];  // A(i,j)
+#define G2G_AF(p) cmac(t1##p, a, vin[(p) * Nz3 + off1 + j]);
+            G2G_FOR4(G2G_AF)
+#undef G2G_AF
+          }
+          if (i >= j0 && i < j1) {
+            const T2* col = tile + (int64_t)(i - j0) * Nz;
+            for (int k = part; k < Nz; k += split) {
+              const T2 at = col[k];  // A(k,i)
+#define G2G_AT(p) cmac(t2##p, at, vin[(p) * Nz3 + off2 + k]);
+              G2G_FOR4(G2G_AT)
+#undef G2G_AT
+            }
+          }
+        }
+      }
+    }
+#define G2G_FOLD(p)                                                  \
+  if (comp == 0) ax##p = ax##p + t1##p;                              \
+  else if (comp == 1) {                                              \
+    ax##p = ax##p + t1##p * (G2G_SG(p) * G2G_MR(p));                 \
+    ay##p = ay##p + t2##p * (G2G_SG(p) * G2G_MR(p));                 \
+  } else if (comp == 2) ay##p = ay##p + t1##p;                       \
+  else if (comp == 3) az##p = az##p + t1##p;                         \
+  else if (comp == 4) {                                              \
+    ax##p = ax##p + t1##p * G2G_SG(p);                               \
+    az##p = az##p - t2##p * G2G_SG(p);                               \
+  } else {                                                           \
+    ay##p = ay##p + t1##p * G2G_MR(p);                               \
+    az##p = az##p - t2##p * G2G_MR(p);                               \
+  }
+    G2G_FOR4(G2G_FOLD)
+#undef G2G_FOLD
+  }
+  for (int o = 1; o < split; o <<= 1) {
+#define G2G_RED(p)                                                       \
+  ax##p.x += __shfl_xor(ax##p.x, o); ax##p.y += __shfl_xor(ax##p.y, o);  \
+  ay##p.x += __shfl_xor(ay##p.x, o); ay##p.y += __shfl_xor(ay##p.y, o);  \
+  az##p.x += __shfl_xor(az##p.x, o); az##p.y += __shfl_xor(az##p.y, o);
+    G2G_FOR4(G2G_RED)
+#undef G2G_RED
+  }
+  if (active && part == 0) {
+#define G2G_OUT(p)                                                  \
+  {                                                                 \
+    T2* dst = S + ((int64_t)PX(p) * Ny2 + PY(p)) * Nz3;             \
+    if (!VAL(p)) ax##p = ay##p = az##p = zero;                      \
+    dst[i] = ax##p;                                                 \
+    dst[Nz + i] = ay##p;                                            \
+    dst[2 * Nz + i] = az##p;                                        \
+  }
+    G2G_FOR4(G2G_OUT)
+#undef G2G_OUT
+  }
+#undef G2G_SG
+#undef G2G_MR
+#undef PX
+#undef PY
+#undef VAL
 }
 
 // Two-deep variant of k_contract: two tensor components in flight in
@@ -1185,6 +1346,8 @@ static Geo make_geo(const Operator& op, int nrx) {
   g.nrx_log = ilog2(nrx);
   g.nc = op.ncell();
   for (int c = 0; c < 3; c++) g.slot[c] = op.comp_slot[c];
+  g.px = op.px;
+  g.py = op.py;
   return g;
 }
 
@@ -1243,19 +1406,22 @@ static void fft_forward(const Operator& op, OpBuffers<R>& b, const cx_t<R>* vin,
   const int Nx2 = 2 * op.Nx, Ny2 = 2 * op.Ny, Nz3 = 3 * op.Nz;
   const int nry = b.half ? nr_half_rt<R>(Ny2) : nr_rt<R>(Ny2);
   const int chy = (Nz3 + nry - 1) / nry;
-  pow2_dispatch(Nx2, [&](auto LC) {
+  len_dispatch(Nx2, [&](auto LC) {
     constexpr int LL = decltype(LC)::value;
-    if (b.half)
+    constexpr bool RT = decltype(LC)::rt;
+    if (!RT && b.half)
       hipLaunchKernelGGL((k_fwd_x_h<R, LL>), dim3(op.Ny, b.nchunk), dim3(NT), 0, s, vin, b.gsig, b.X, b.twx, g);
     else
-      hipLaunchKernelGGL((k_fwd_x<R, LL>), dim3(op.Ny, b.nchunk), dim3(NT), 0, s, vin, b.gsig, b.X, b.twx, g);
+      hipLaunchKernelGGL((k_fwd_x<R, LL, RT>), dim3(op.Ny, b.nchunk), dim3(NT), 0, s, vin, b.gsig, b.X, b.twx,
+                         g);
   });
-  pow2_dispatch(Ny2, [&](auto LC) {
+  len_dispatch(Ny2, [&](auto LC) {
     constexpr int LL = decltype(LC)::value;
-    if (b.half)
+    constexpr bool RT = decltype(LC)::rt;
+    if (!RT && b.half)
       hipLaunchKernelGGL((k_fwd_y_h<R, LL>), dim3(Nx2, chy), dim3(NT), 0, s, b.X, S, b.twy, g);
     else
-      hipLaunchKernelGGL((k_fwd_y<R, LL>), dim3(Nx2, chy), dim3(NT), 0, s, b.X, S, b.twy, g);
+      hipLaunchKernelGGL((k_fwd_y<R, LL, false, RT>), dim3(Nx2, chy), dim3(NT), 0, s, b.X, S, b.twy, g);
   });
 }
 
@@ -1265,20 +1431,22 @@ static void fft_inverse(const Operator& op, OpBuffers<R>& b, const cx_t<R>* S, c
   const int Nx2 = 2 * op.Nx, Ny2 = 2 * op.Ny, Nz3 = 3 * op.Nz;
   const int nry = b.half ? nr_half_rt<R>(Ny2) : nr_rt<R>(Ny2);
   const int chy = (Nz3 + nry - 1) / nry;
-  pow2_dispatch(Ny2, [&](auto LC) {
+  len_dispatch(Ny2, [&](auto LC) {
     constexpr int LL = decltype(LC)::value;
-    if (b.half)
+    constexpr bool RT = decltype(LC)::rt;
+    if (!RT && b.half)
       hipLaunchKernelGGL((k_inv_y_h<R, LL>), dim3(Nx2, chy), dim3(NT), 0, s, S, b.X, b.twy, g);
     else
-      hipLaunchKernelGGL((k_inv_y<R, LL>), dim3(Nx2, chy), dim3(NT), 0, s, S, b.X, b.twy, g);
+      hipLaunchKernelGGL((k_inv_y<R, LL, false, RT>), dim3(Nx2, chy), dim3(NT), 0, s, S, b.X, b.twy, g);
   });
-  pow2_dispatch(Nx2, [&](auto LC) {
+  len_dispatch(Nx2, [&](auto LC) {
     constexpr int LL = decltype(LC)::value;
-    if (b.half)
+    constexpr bool RT = decltype(LC)::rt;
+    if (!RT && b.half)
       hipLaunchKernelGGL((k_inv_x_h<R, LL>), dim3(op.Ny, b.nchunk), dim3(NT), 0, s, b.X, vin, b.asiga, b.sq,
                          b.dzn, vout, b.twx, g);
     else
-      hipLaunchKernelGGL((k_inv_x<R, LL>), dim3(op.Ny, b.nchunk), dim3(NT), 0, s, b.X, vin, b.asiga, b.sq,
+      hipLaunchKernelGGL((k_inv_x<R, LL, RT>), dim3(op.Ny, b.nchunk), dim3(NT), 0, s, b.X, vin, b.asiga, b.sq,
                          b.dzn, vout, b.twx, g);
   });
 }
@@ -1298,7 +1466,18 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
   fft_forward<R>(op, b, vin, b.S, g, s);
   if (timed) G2G_HIP(hipEventRecord(op.ev[1], s));
   if (prof) pe[1] = op.prof.record(s);
-  {
+  if (op.Nz > 64) {
+    // tiles of TJ source columns; the four points' rows and one tile in LDS
+    int split_log = 0;
+    while ((op.Nz << (split_log + 1)) <= NT) split_log++;
+    const size_t es = sizeof(cx_t<R>), vb = es * 4 * (size_t)Nz3;
+    const int TJ = (int)std::max<size_t>(4, std::min<size_t>(32, (96 * 1024 - vb) / (es * op.Nz)));
+    const size_t sm = vb + es * (size_t)TJ * op.Nz;
+    auto kern = k_contract_tiled<R>;
+    if (sm > 65536)
+      G2G_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    hipLaunchKernelGGL(kern, dim3(op.Nx * op.Ny), dim3(NT), sm, s, b.S, b.G, g, op.BS, split_log, TJ);
+  } else {
     // vector ALU: rows = Nz layers, split lanes per row (power of two)
     int split_log = 0;
     while ((op.Nz << (split_log + 1)) <= NT) split_log++;
@@ -1573,17 +1752,13 @@ template void op_apply_batch_noalloc<float>(Operator&, OpBuffers<float>&, int, c
 // One P-lane segment per (point, receiver) pair, lane = source layer z;
 // partial products summed over the segment with lane exchanges.  All lanes
 // stay active through the exchanges (loads and stores are predicated).
-template <int P>
-__global__ __launch_bounds__(NT) void k_rc_contract(const zd* __restrict__ S, const zd* __restrict__ G,
-                                                    zd* __restrict__ So, int64_t npairs, int Nr, int Nz) {
-  const int lane = threadIdx.x & (P - 1);
-  const int64_t q = (int64_t)blockIdx.x * (NT / P) + threadIdx.x / P;
-  const bool pair_ok = q < npairs;
-  const bool ok = pair_ok && lane < Nz;
-  const int64_t pt = pair_ok ? q / Nr : 0;
-  const int r = pair_ok ? (int)(q - pt * Nr) : 0;
-  const zd* s = S + pt * 3 * Nz + lane;
-  const zd* g = G + (pt * 15 * Nr + r) * Nz + lane;  // entry e at g[e * Nr * Nz]
+// The six output products of one (point, receiver) pair over the source
+// layers z0 + lane (zero where that is past Nz or the pair is out of range).
+__device__ inline void rc_products(const zd* __restrict__ S, const zd* __restrict__ G, int64_t pt, int r,
+                                   int Nr, int Nz, int z, bool ok, zd& o0, zd& o1, zd& o2, zd& o3, zd& o4,
+                                   zd& o5) {
+  const zd* s = S + pt * 3 * Nz + z;
+  const zd* g = G + (pt * 15 * Nr + r) * Nz + z;  // entry e at g[e * Nr * Nz]
   const int64_t es = (int64_t)Nr * Nz;
   // predicated loads into zero-initialised registers (a select between a
   // global element and a local constant would put the constant in scratch)
@@ -1612,12 +1787,38 @@ __global__ __launch_bounds__(NT) void k_rc_contract(const zd* __restrict__ S, co
   }
   // APPLY_RC_E_OP (:420-434) and APPLY_RC_H_OP (:461-473); RHYY = -RHXX (:468).
   // Named registers, not an array, so that nothing is placed in scratch.
-  zd o0 = rexx * jx + rexy * jy + rexz * jz;
-  zd o1 = rexy * jx + reyy * jy + reyz * jz;
-  zd o2 = rezx * jx + rezy * jy + rezz * jz;
-  zd o3 = rhxx * jx + rhxy * jy + rhxz * jz;
-  zd o4 = rhyx * jx - rhxx * jy + rhyz * jz;
-  zd o5 = rhzx * jx + rhzy * jy;
+  o0 = rexx * jx + rexy * jy + rexz * jz;
+  o1 = rexy * jx + reyy * jy + reyz * jz;
+  o2 = rezx * jx + rezy * jy + rezz * jz;
+  o3 = rhxx * jx + rhxy * jy + rhxz * jz;
+  o4 = rhyx * jx - rhxx * jy + rhyz * jz;
+  o5 = rhzx * jx + rhzy * jy;
+}
+
+// CHUNKS: more than P (= 64) source layers; a lane then also takes the
+// layers lane + P, lane + 2P, ... (one round otherwise, the case's own code)
+template <int P, bool CHUNKS = false>
+__global__ __launch_bounds__(NT) void k_rc_contract(const zd* __restrict__ S, const zd* __restrict__ G,
+                                                    zd* __restrict__ So, int64_t npairs, int Nr, int Nz) {
+  const int lane = threadIdx.x & (P - 1);
+  const int64_t q = (int64_t)blockIdx.x * (NT / P) + threadIdx.x / P;
+  const bool pair_ok = q < npairs;
+  const int64_t pt = pair_ok ? q / Nr : 0;
+  const int r = pair_ok ? (int)(q - pt * Nr) : 0;
+  zd o0, o1, o2, o3, o4, o5;
+  rc_products(S, G, pt, r, Nr, Nz, lane, pair_ok && lane < Nz, o0, o1, o2, o3, o4, o5);
+  if constexpr (CHUNKS) {
+    for (int z0 = P; z0 < Nz; z0 += P) {
+      zd p0, p1, p2, p3, p4, p5;
+      rc_products(S, G, pt, r, Nr, Nz, z0 + lane, pair_ok && z0 + lane < Nz, p0, p1, p2, p3, p4, p5);
+      o0 = o0 + p0;
+      o1 = o1 + p1;
+      o2 = o2 + p2;
+      o3 = o3 + p3;
+      o4 = o4 + p4;
+      o5 = o5 + p5;
+    }
+  }
 #pragma unroll
   for (int off = P / 2; off > 0; off >>= 1) {
 #define G2G_XR(v) v.x += __shfl_xor(v.x, off); v.y += __shfl_xor(v.y, off);
@@ -1637,7 +1838,7 @@ __global__ __launch_bounds__(NT) void k_rc_contract(const zd* __restrict__ S, co
 
 // Inverse x-FFT of the output rows (row = component*Nr + receiver), keep
 // x < Nx, scale (RC_OP_FFTW_BWD, :344-348) and store Ea, Ha (Nr,3,Nx,Ny).
-template <int L>
+template <int L, bool RT = false>
 __global__ __launch_bounds__(NT) void k_rc_inv_x(const zd* __restrict__ X, zd* __restrict__ Ea,
                                                  zd* __restrict__ Ha, const zd* __restrict__ twg, Geo g,
                                                  int Nr, double scale) {
@@ -1646,14 +1847,15 @@ __global__ __launch_bounds__(NT) void k_rc_inv_x(const zd* __restrict__ X, zd* _
   __shared__ zd buf[NR * LS];
   __shared__ zd tw[L];
   const int y = blockIdx.x, chunk = blockIdx.y, Nx = g.Nx;
-  load_tw<zd, L>(tw, twg);
-  const zd* src = X + (((int64_t)chunk * g.Ny + y) * L) * NR;
-  for (int i = threadIdx.x; i < NR * L; i += NT) {
+  const int Lr = RT ? 2 * Nx : L;
+  load_tw<zd>(tw, twg, Lr);
+  const zd* src = X + (((int64_t)chunk * g.Ny + y) * Lr) * NR;
+  for (int i = threadIdx.x; i < NR * Lr; i += NT) {
     const int kx = i / NR, s = i - kx * NR;
     buf[s * LS + kx] = src[i];
   }
   __syncthreads();
-  lds_fft<zd, L, NR, LS, NT, +1>(buf, tw);
+  lds_fft_any<zd, L, NR, LS, NT, +1, RT>(buf, tw, g.px);
   for (int i = threadIdx.x; i < NR * Nx; i += NT) {
     const int s = i / Nx, x = i - s * Nx, row = chunk * NR + s;
     if (row < g.Nz3) {
@@ -1748,20 +1950,20 @@ void rc_apply(RcOperator& rc, const zd* E, zd* Ea, zd* Ha, hipStream_t s) {
   const zd* twx = op.d64.twx;
   const zd* twy = op.d64.twy;
   G2G_HIP(hipEventRecord(op.ev[4], s));
-  pow2_dispatch(Nx2, [&](auto LC) {
+  len_dispatch(Nx2, [&](auto LC) {
     constexpr int LL = decltype(LC)::value;
-    hipLaunchKernelGGL((k_fwd_x<double, LL>), dim3(op.Ny, rc.nchunk_f), dim3(NT), 0, s, E, rc.dsig, rc.Xf,
-                       twx, gf);
+    hipLaunchKernelGGL((k_fwd_x<double, LL, decltype(LC)::rt>), dim3(op.Ny, rc.nchunk_f), dim3(NT), 0, s, E,
+                       rc.dsig, rc.Xf, twx, gf);
   });
-  pow2_dispatch(Ny2, [&](auto LC) {
+  len_dispatch(Ny2, [&](auto LC) {
     constexpr int LL = decltype(LC)::value;
-    hipLaunchKernelGGL((k_fwd_y<double, LL, true>), dim3(Nx2, (3 * op.Nz + nry - 1) / nry), dim3(NT), 0, s,
-                       rc.Xf, rc.S, twy, gf);
+    hipLaunchKernelGGL((k_fwd_y<double, LL, true, decltype(LC)::rt>), dim3(Nx2, (3 * op.Nz + nry - 1) / nry),
+                       dim3(NT), 0, s, rc.Xf, rc.S, twy, gf);
   });
   G2G_HIP(hipEventRecord(op.ev[7], s));
   {
     int P = 1;
-    while (P < op.Nz) P <<= 1;
+    while (P < op.Nz && P < 64) P <<= 1;  // above 64 layers a lane takes several
     const int64_t npairs = rc.NP * Nr;
     const unsigned blocks = (unsigned)((npairs + NT / P - 1) / (NT / P));
     auto go = [&](auto kern) {
@@ -1774,19 +1976,22 @@ void rc_apply(RcOperator& rc, const zd* E, zd* Ea, zd* Ha, hipStream_t s) {
       case 8: go(k_rc_contract<8>); break;
       case 16: go(k_rc_contract<16>); break;
       case 32: go(k_rc_contract<32>); break;
-      default: go(k_rc_contract<64>); break;
+      default:
+        if (op.Nz > 64) go(k_rc_contract<64, true>);
+        else go(k_rc_contract<64>);
+        break;
     }
   }
   G2G_HIP(hipEventRecord(op.ev[5], s));
-  pow2_dispatch(Ny2, [&](auto LC) {
+  len_dispatch(Ny2, [&](auto LC) {
     constexpr int LL = decltype(LC)::value;
-    hipLaunchKernelGGL((k_inv_y<double, LL, true>), dim3(Nx2, (6 * Nr + nry - 1) / nry), dim3(NT), 0, s,
-                       rc.So, rc.Xi, twy, gi);
+    hipLaunchKernelGGL((k_inv_y<double, LL, true, decltype(LC)::rt>), dim3(Nx2, (6 * Nr + nry - 1) / nry),
+                       dim3(NT), 0, s, rc.So, rc.Xi, twy, gi);
   });
-  pow2_dispatch(Nx2, [&](auto LC) {
+  len_dispatch(Nx2, [&](auto LC) {
     constexpr int LL = decltype(LC)::value;
-    hipLaunchKernelGGL((k_rc_inv_x<LL>), dim3(op.Ny, rc.nchunk_i), dim3(NT), 0, s, rc.Xi, Ea, Ha, twx, gi, Nr,
-                       1.0 / (4.0 * op.Nx * op.Ny));
+    hipLaunchKernelGGL((k_rc_inv_x<LL, decltype(LC)::rt>), dim3(op.Ny, rc.nchunk_i), dim3(NT), 0, s, rc.Xi, Ea,
+                       Ha, twx, gi, Nr, 1.0 / (4.0 * op.Nx * op.Ny));
   });
   G2G_HIP(hipEventRecord(op.ev[6], s));
   G2G_HIP(hipGetLastError());

@@ -97,9 +97,12 @@ struct DevCopy {
 int get_layer(const Operator& op, double z) { return op_layer(op, z); }
 
 void op_init(Operator& op, int Nx, int Ny, int Nz, double dx, double dy, const double* z) {
-  G2G_CHECK(Nx >= 8 && Ny >= 8 && is_pow2(Nx) && is_pow2(Ny) && Nx <= 512 && Ny <= 512, -3,
-            "Nx and Ny must be powers of two in [8,512]");
-  G2G_CHECK(Nz >= 1 && Nz <= 64, -3, "Nz must be in [1,64]");
+  // any even Nx, Ny (CalcLocalKernelSize, Integral_Equation_Module.F90:105-129,
+  // rejects odd ones): powers of two run the radix-4 passes, other lengths
+  // their factor plans
+  G2G_CHECK(Nx >= 8 && Ny >= 8 && Nx % 2 == 0 && Ny % 2 == 0 && Nx <= 512 && Ny <= 512, -3,
+            "Nx and Ny must be even and in [8,512]");
+  G2G_CHECK(Nz >= 1 && Nz <= MAX_NZ, -3, "Nz must be in [1,256]");
   G2G_CHECK(dx > 0 && dy > 0, -3, "dx, dy must be positive");
   op.Nx = Nx;
   op.Ny = Ny;
@@ -113,6 +116,8 @@ void op_init(Operator& op, int Nx, int Ny, int Nz, double dx, double dy, const d
     G2G_CHECK(op.dz[i] > 0, -3, "z borders must increase");
   }
   op.BS = 4 * (int64_t)Nz * (Nz + 1) / 2 + 2 * (int64_t)Nz * Nz;
+  op.px = fft_plan(2 * Nx);
+  op.py = fft_plan(2 * Ny);
   op.stream = nullptr;  // the default stream orders with torch's default stream
   for (auto& e : op.ev) G2G_HIP(hipEventCreate(&e));
   op_alloc_buffers<double>(op, op.d64, true);
@@ -122,6 +127,7 @@ void op_init(Operator& op, int Nx, int Ny, int Nz, double dx, double dy, const d
 }
 
 void op_fini(Operator& op) {
+  for (Arena& a : op.ws) a.release();
   if (op.adj) (void)hipFree(op.adj);
   op.adj = nullptr;
   if (op.csiga) (void)hipFree(op.csiga);
@@ -367,6 +373,7 @@ SolveParams params_from(const giem2g_solver_params* p) {
     s.tol = p->tol;
     s.inner_precision = p->inner_precision;
     s.precond = p->precond;
+    s.learned_patience = p->learned_patience == 0 ? 2 : (p->learned_patience < 0 ? 0 : p->learned_patience);
   }
   G2G_CHECK(s.outer_m >= 1 && s.outer_m <= 63 && s.inner_m >= 1 && s.inner_m <= 63, -3,
             "buffer sizes must be in [1,63]");
@@ -380,9 +387,8 @@ void solve_dev(Handle& h, const zd* Ebkg, const zd* E0, const giem2g_solver_para
   G2G_CHECK(op.have_tensor && op.have_sigma, -4, "operator not ready");
   const SolveParams p = params_from(pp);
   const int64_t N = op.nvec();
-  zd *rhs, *guess;
-  G2G_HIP(hipMalloc(&rhs, sizeof(zd) * N));
-  G2G_HIP(hipMalloc(&guess, sizeof(zd) * N));
+  zd* rhs = (zd*)op.ws[0].get(WS_RHS, sizeof(zd) * N);
+  zd* guess = (zd*)op.ws[0].get(WS_GUESS, sizeof(zd) * N);
   hipEvent_t a, b;
   G2G_HIP(hipEventCreate(&a));
   G2G_HIP(hipEventCreate(&b));
@@ -401,16 +407,14 @@ void solve_dev(Handle& h, const zd* Ebkg, const zd* E0, const giem2g_solver_para
     G2G_HIP(hipEventSynchronize(b));
     flush_prof(op);
   } catch (...) {
-    (void)hipFree(rhs);
-    (void)hipFree(guess);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
     throw;
   }
   float ms = 0;
   G2G_HIP(hipEventElapsedTime(&ms, a, b));
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
-  (void)hipFree(rhs);
-  (void)hipFree(guess);
   if (info) {
     info->outer_iterations = r.iterations;
     info->status = r.stat;
@@ -710,8 +714,7 @@ int giem2g_solve_pair_dev(void* p, const void* Ebkg0, const void* Ebkg1, const v
     G2G_CHECK(Ebkg0 && Ebkg1 && Esol0 && Esol1, -2, "null vector");
     const SolveParams prm = params_from(pp);
     const int64_t N = op.nvec();
-    zd* work = nullptr;
-    G2G_HIP(hipMalloc(&work, sizeof(zd) * N * 4));
+    zd* work = (zd*)op.ws[0].get(WS_PAIR, sizeof(zd) * N * 4);
     zd* rhs[2] = {work, work + N};
     zd* guess[2] = {work + 2 * N, work + 3 * N};
     const zd* eb[2] = {(const zd*)Ebkg0, (const zd*)Ebkg1};
@@ -736,14 +739,14 @@ int giem2g_solve_pair_dev(void* p, const void* Ebkg0, const void* Ebkg1, const v
       G2G_HIP(hipEventSynchronize(b));
       flush_prof(op);
     } catch (...) {
-      (void)hipFree(work);
+      (void)hipEventDestroy(a);
+      (void)hipEventDestroy(b);
       throw;
     }
     float ms = 0;
     G2G_HIP(hipEventElapsedTime(&ms, a, b));
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    (void)hipFree(work);
     if (info2) {
       for (int k = 0; k < 2; k++) {
         giem2g_solve_info& in = info2[k];

@@ -10,6 +10,10 @@
 
 namespace g2g {
 
+// Most layers an anomaly may have (the contraction above 64 layers stages
+// tensor tiles through LDS with the four points' rows, 12*Nz complex)
+constexpr int MAX_NZ = 256;
+
 // ---------------------------------------------------------------- errors --
 struct Error : std::runtime_error {
   int code;
@@ -116,5 +120,30 @@ inline int ilog2(int n) {
   return l;
 }
 inline bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// A transform length's compile-time class: a power of two in [16,1024] runs
+// as itself (rt = false); any other even length up to 1024 runs in the
+// buffers of the next power of two with a runtime factor plan (rt = true).
+template <int LC_, bool RT_> struct LenClass {
+  static constexpr int value = LC_;
+  static constexpr bool rt = RT_;
+};
+template <typename F> inline void len_dispatch(int L, F&& f) {
+  if (L < 16 || L > 1024 || (L & 1)) throw Error(-3, "FFT length must be even and in [16,1024]");
+  if (is_pow2(L)) {
+    pow2_dispatch(L, [&](auto LC) { f(LenClass<decltype(LC)::value, false>{}); });
+    return;
+  }
+  switch (1 << ilog2(L)) {
+    case 32: f(LenClass<32, true>{}); break;
+    case 64: f(LenClass<64, true>{}); break;
+    case 128: f(LenClass<128, true>{}); break;
+    case 256: f(LenClass<256, true>{}); break;
+    case 512: f(LenClass<512, true>{}); break;
+    default: f(LenClass<1024, true>{}); break;
+  }
+}
+// the buffer capacity len_dispatch gives L
+inline int len_capacity(int L) { return 1 << ilog2(L); }
 
 }  // namespace g2g

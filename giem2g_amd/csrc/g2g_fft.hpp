@@ -1,4 +1,5 @@
-// g2g_fft.hpp -- batched power-of-two complex FFTs held in LDS.
+// g2g_fft.hpp -- batched complex FFTs held in LDS: radix-4 Stockham for
+// powers of two, and a factor-plan Stockham for other lengths (below).
 //
 // NR independent rows of length L sit in LDS at buf[row*LS + i] (LS >= L, a
 // padded row stride).  The transform is an in-place Stockham autosort FFT
@@ -82,6 +83,91 @@ __device__ inline void lds_fft(T2* buf, const T2* tw) {
     Ns *= 4;
   }
   if constexpr (LOG & 1) stockham_stage<T2, L, NR, LS, NT, 2, DIR>(buf, tw, Ns);
+}
+
+// ------------------------------------------------------- any length ------
+// Lengths that are not powers of two (2Nx, 2Ny for any even Nx, Ny) run a
+// Stockham autosort FFT over the length's factors -- fours, then twos, then
+// the odd primes -- in the same LDS layout.  A stage of radix R computes
+// every output as the R-term sum over its butterfly's inputs, each times one
+// entry of the twiddle table tw[t] = exp(-2 pi i t/L): output slot
+// t = b Ns R + r Ns + k of a row takes the inputs j + q L/R, j = b Ns + k,
+// with factors tw[(q e) mod L], e = k L/(Ns R) + r L/R (the stage twiddle
+// w_{Ns R}^{q k} and the radix-R DFT in one).  That is O(L R) work per
+// stage and row for any prime R.  The plan is made on the host.
+struct FftPlan {
+  int L = 0, nst = 0;
+  int rad[24] = {};
+};
+
+inline FftPlan fft_plan(int L) {
+  FftPlan p;
+  p.L = L;
+  int n = L;
+  auto push = [&](int r) {
+    if (p.nst >= 24) throw Error(-3, "FFT length has too many factors");
+    p.rad[p.nst++] = r;
+    n /= r;
+  };
+  while (n % 4 == 0) push(4);
+  while (n % 2 == 0) push(2);
+  for (int f = 3; n > 1; f += 2)
+    while (n % f == 0) push(f);
+  return p;
+}
+
+template <typename T2, int PER, int NT, int DIR>
+__device__ inline void stockham_stage_any(T2* buf, const T2* tw, int L, int NR, int LS, int R, int Ns) {
+  const int NBR = L / R, NO = NR * L, sK = L / (Ns * R);
+  T2 o[PER];
+#pragma unroll
+  for (int p = 0; p < PER; p++) {
+    const int oi = threadIdx.x + p * NT;
+    o[p] = mk<T2>(0, 0);
+    if (oi < NO) {
+      const int row = oi / L, t = oi - row * L;
+      const int k = t % Ns, r = (t / Ns) % R, b = t / (Ns * R);
+      const T2* src = buf + row * LS + b * Ns + k;
+      const int e = k * sK + r * NBR;
+      T2 acc = src[0];
+      int idx = e;
+      for (int q = 1; q < R; q++) {
+        T2 w = tw[idx];
+        if (DIR > 0) w.y = -w.y;
+        acc = acc + src[q * NBR] * w;
+        idx += e;
+        if (idx >= L) idx -= L;
+      }
+      o[p] = acc;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int p = 0; p < PER; p++) {
+    const int oi = threadIdx.x + p * NT;
+    if (oi < NO) {
+      const int row = oi / L;
+      buf[row * LS + (oi - row * L)] = o[p];
+    }
+  }
+  __syncthreads();
+}
+
+// Full transform of NR rows of length plan.L <= LC (the capacity the
+// buffers were sized for).  RT = false: LC itself, a power of two, with
+// lds_fft.  Caller must __syncthreads() after filling buf and tw.
+template <typename T2, int LC, int NR, int LS, int NT, int DIR, bool RT>
+__device__ inline void lds_fft_any(T2* buf, const T2* tw, const FftPlan& plan) {
+  if constexpr (!RT) {
+    lds_fft<T2, LC, NR, LS, NT, DIR>(buf, tw);
+  } else {
+    constexpr int PER = (NR * LC + NT - 1) / NT;
+    int Ns = 1;
+    for (int s = 0; s < plan.nst; s++) {
+      stockham_stage_any<T2, PER, NT, DIR>(buf, tw, plan.L, NR, LS, plan.rad[s], Ns);
+      Ns *= plan.rad[s];
+    }
+  }
 }
 
 }  // namespace g2g

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "g2g_common.hpp"
+#include "g2g_fft.hpp"
 
 namespace g2g {
 
@@ -100,6 +101,40 @@ struct Prof {
   }
 };
 
+// Device buffers that repeated solves reuse: each slot keeps the largest
+// allocation asked of it until release(), so a solve after the first of
+// the same size allocates and frees nothing (at 256x256x32 the Krylov
+// bases of one solve are several GB).
+struct Arena {
+  struct Buf {
+    void* p = nullptr;
+    size_t n = 0;
+  };
+  std::vector<Buf> b;
+  void* get(int slot, size_t bytes) {
+    if ((size_t)slot >= b.size()) b.resize(slot + 1);
+    Buf& x = b[slot];
+    if (x.n < bytes) {
+      if (x.p) G2G_HIP(hipFree(x.p));
+      x.p = nullptr;
+      x.n = 0;
+      G2G_HIP(hipMalloc(&x.p, bytes));
+      x.n = bytes;
+    }
+    return x.p;
+  }
+  size_t bytes() const {
+    size_t t = 0;
+    for (const Buf& x : b) t += x.n;
+    return t;
+  }
+  void release() {
+    for (Buf& x : b)
+      if (x.p) (void)hipFree(x.p);
+    b.clear();
+  }
+};
+
 template <typename R> struct OpBuffers {
   using T2 = cx_t<R>;
   T2* G = nullptr;      // spectral tensor, Nx*Ny*BS
@@ -123,6 +158,7 @@ struct Operator {
   double dx = 0, dy = 0;
   std::vector<double> z, dz;
   int comp_slot[3] = {0, 1, 2};  // memory slot of physical component x,y,z
+  FftPlan px, py;                // factor plans of the 2Nx- and 2Ny-point transforms
   // background of the current frequency
   int Nl = 0;
   double omega = 0;
@@ -150,6 +186,9 @@ struct Operator {
   bool have_wts = false;
   double* tabs = nullptr;  // per-lambda kernel tables
   zd* adj = nullptr;       // adjoint-apply scratch (allocated on first use)
+  // Krylov workspaces of the solves: one per concurrent solve (a paired
+  // solve runs two); kept between solves, freed with the operator
+  Arena ws[2];
   size_t tabs_bytes = 0;
   hipEvent_t ev[8] = {};
 

@@ -32,6 +32,7 @@ constexpr int NBLK = 1024;  // blocks for reductions
 constexpr int KC = 8;       // columns per multi-dot block row
 constexpr int MAXK = 64;    // columns of one multi-dot call (Krylov basis size bound)
 
+
 template <typename A> __device__ inline A wave_sum(A v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
@@ -210,17 +211,11 @@ struct DevBlas {
   void prof_end(int a) {
     if (a >= 0) prof->mark(Prof::DOT, 0, 0, a, prof->record(s));
   }
-  void init(hipStream_t st) {
+  void init(hipStream_t st, Arena& ws) {
     s = st;
-    G2G_HIP(hipMalloc(&part, sizeof(double) * (MAXK / KC) * NBLK * 2 * KC));
-    G2G_HIP(hipMalloc(&res, sizeof(double) * (2 * MAXK + 2 * KC)));
-    G2G_HIP(hipMalloc(&hbuf, sizeof(zd) * 64));
-  }
-  void fini() {
-    if (part) (void)hipFree(part);
-    if (res) (void)hipFree(res);
-    if (hbuf) (void)hipFree(hbuf);
-    part = nullptr;
+    part = ws.get(WS_PART, sizeof(double) * (MAXK / KC) * NBLK * 2 * KC);
+    res = (double*)ws.get(WS_RES, sizeof(double) * (2 * MAXK + 2 * KC));
+    hbuf = (zd*)ws.get(WS_HBUF, sizeof(zd) * 64);
   }
   static int nblocks(int64_t N) { return (int)std::min<int64_t>(NBLK, (N + BT - 1) / BT); }
   // res[0 .. 2K) = V[:,k]^H w for k < K, on the device (no host round trip)
@@ -344,31 +339,25 @@ template <typename R> struct Gmres {
 
   zc& HH(int i, int j) { return H[(size_t)i + (size_t)(M + 1) * j]; }
 
-  void alloc(int64_t n, int m, bool f) {
+  // buffers from arena slots slot0 .. slot0+4 (kept for the next solve)
+  void alloc(int64_t n, int m, bool f, Arena& ws, int slot0) {
     N = n;
     M = m;
     flex = f;
-    auto Al = [&](T2** p, int64_t cnt) {
-      G2G_HIP(hipMalloc(p, sizeof(T2) * cnt));
+    auto Al = [&](T2** p, int64_t cnt, int k) {
+      *p = (T2*)ws.get(slot0 + k, sizeof(T2) * cnt);
       bytes += sizeof(T2) * cnt;
     };
-    Al(&x, N);
-    Al(&r0, N);
-    Al(&w, N);
-    Al(&V, N * (M + 1));
-    if (flex) Al(&Z, N * M);
+    Al(&x, N, 0);
+    Al(&r0, N, 1);
+    Al(&w, N, 2);
+    Al(&V, N * (M + 1), 3);
+    if (flex) Al(&Z, N * M, 4);
     H.assign((size_t)(M + 1) * (M + 1), 0);
     gs.assign(M + 1, 0);
     gc.assign(M + 1, 0);
     y.assign(M + 1, 0);
     dp.assign(M + 1, 0);
-  }
-  void fini() {
-    for (T2** p : {&x, &r0, &w, &V, &Z})
-      if (*p) {
-        (void)hipFree(*p);
-        *p = nullptr;
-      }
   }
   void apply(const T2* in, T2* out) {
     A(in, out);
@@ -542,14 +531,15 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
   const int64_t N = op.nvec();
   const hipStream_t st = env ? env->stream : op.stream;
   G2G_CHECK(!env || p.precond == 0, -3, "the learned preconditioner runs only in single solves");
+  Arena& ws = env ? *env->ws : op.ws[0];
   DevBlas blas;
-  blas.init(st);
+  blas.init(st, ws);
   blas.prof = &op.prof;
   Gmres<double> outer;
   outer.blas = &blas;
   outer.maxit = p.max_restart;
   outer.tol = p.tol;
-  outer.alloc(N, p.outer_m, true);
+  outer.alloc(N, p.outer_m, true, ws, WS_OUTER);
   outer.matvecs = env ? env->mv64 : &op.cnt.matvecs;
   outer.hist = hist;
   if (env) outer.A = env->A64;
@@ -564,7 +554,7 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
       in64.blas = &blas;
       in64.maxit = 1;
       in64.tol = p.tol;
-      in64.alloc(N, p.inner_m, false);
+      in64.alloc(N, p.inner_m, false, ws, WS_IN64);
       in64.matvecs = env ? env->mv64 : &op.cnt.matvecs;
       in64.A = outer.A;
     } else {
@@ -572,16 +562,16 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
       in32.blas = &blas;
       in32.maxit = 1;
       in32.tol = p.tol;
-      in32.alloc(N, p.inner_m, false);
+      in32.alloc(N, p.inner_m, false, ws, WS_IN32);
       in32.matvecs = env ? env->mv32 : &op.cnt.matvecs32;
       if (env) in32.A = env->A32;
       else in32.A = [&](const zf* in, zf* out) { op_apply<float>(op, op.d32, in, out, op.stream, false); };
-      G2G_HIP(hipMalloc(&b32, sizeof(zf) * N));
-      G2G_HIP(hipMalloc(&x32, sizeof(zf) * N));
-      if (p.precond == 1) G2G_HIP(hipMalloc(&r32, sizeof(zf) * N));
+      b32 = (zf*)ws.get(WS_B32, sizeof(zf) * N);
+      x32 = (zf*)ws.get(WS_X32, sizeof(zf) * N);
+      if (p.precond == 1) r32 = (zf*)ws.get(WS_R32, sizeof(zf) * N);
     }
-    G2G_HIP(hipMalloc(&tmp, sizeof(zd) * N));
-    if (p.precond == 1 && p.inner_precision == 0) G2G_HIP(hipMalloc(&tmp2, sizeof(zd) * N));
+    tmp = (zd*)ws.get(WS_TMP, sizeof(zd) * N);
+    if (p.precond == 1 && p.inner_precision == 0) tmp2 = (zd*)ws.get(WS_TMP2, sizeof(zd) * N);
     // FGMRES_BASED_PRECONDITIONER (seq_fgmres.F90:216-226): inner GMRES with x0 = b = v
     auto inner_gmres = [&](const zd* v, zd* z) {
       if (p.inner_precision == 0) {
@@ -594,12 +584,25 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
     };
     G2G_CHECK(p.precond == 0 || (learned && learned->apply), -4,
               "learned preconditioner requested but none is attached");
+    // after `patience` learned steps in a row are rejected, the network is
+    // not called again in this solve (the steps it would propose cost an
+    // operator application and a network application each)
+    int rejected_run = 0;
+    bool learned_on = true, learned_used = false;
+    auto note = [&](bool accepted) {
+      rejected_run = accepted ? 0 : rejected_run + 1;
+      if (p.learned_patience > 0 && rejected_run >= p.learned_patience) learned_on = false;
+    };
     if (p.precond == 1) {
       // learned start + inner GMRES: c = M(v), scaled by the least-squares
       // factor alpha = <A c, v>/<A c, A c>; accepted when it cuts the
       // residual, ||v - alpha A c|| < accept ||v|| (inner precision), then
       // z = alpha c + inner GMRES on v - alpha A c; else plain inner GMRES.
       outer.precond = [&](const zd* v, zd* z) {
+        if (!learned_on) {
+          inner_gmres(v, z);
+          return;
+        }
         res.precond_calls++;
         learned->apply(learned->ctx, v, z);
         zc alpha;
@@ -610,9 +613,11 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
           const double vn = blas.norm<zd, double>(v, N);
           if (!(rn < learned->accept * vn)) {
             res.precond_fallbacks++;
+            note(false);
             in64.solve(z, v, v);
             return;
           }
+          note(true);
           blas.lincomb(z, z, N, 1, &alpha, false);
           in64.solve(tmp2, tmp, tmp);
           blas.add(z, tmp2, N);
@@ -625,10 +630,12 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
           const double vn = blas.norm<zf, float>(b32, N);
           if (!(rn < learned->accept * vn)) {
             res.precond_fallbacks++;
+            note(false);
             in32.solve(x32, b32, b32);
             blas.convert<zf, zd>(z, x32, N);
             return;
           }
+          note(true);
           blas.lincomb(z, z, N, 1, &alpha, false);
           in32.solve(x32, r32, r32);
           blas.convert<zf, zd>(tmp, x32, N);
@@ -641,16 +648,23 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
       // (FGMRES does not depend on the scale of z) and otherwise replaces
       // z by inner GMRES(v).
       outer.precond = [&](const zd* v, zd* z) {
+        learned_used = learned_on;
+        if (!learned_on) {
+          inner_gmres(v, z);
+          return;
+        }
         res.precond_calls++;
         learned->apply(learned->ctx, v, z);
       };
       outer.accept_step = [&](const zd* v, const zd* Az) {
+        if (!learned_used) return true;  // the step came from the inner GMRES
         zc alpha;
         const double rn = fit_residual<zd, double>(blas, v, Az, tmp, N, &alpha);
         const double vn = blas.norm<zd, double>(v, N);
-        if (rn < learned->accept * vn) return true;
-        res.precond_fallbacks++;
-        return false;
+        const bool ok = rn < learned->accept * vn;
+        note(ok);
+        if (!ok) res.precond_fallbacks++;
+        return ok;
       };
       outer.fallback = inner_gmres;
     } else {
@@ -669,15 +683,10 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
       op.cnt.krylov_bytes += blas.bytes;
     }
   } catch (...) {
-    outer.fini(); in64.fini(); in32.fini(); blas.fini();
-    for (void* q : {(void*)b32, (void*)x32, (void*)r32, (void*)tmp, (void*)tmp2})
-      if (q) (void)hipFree(q);
+    (void)hipStreamSynchronize(st);  // nothing may still use the arena's buffers
     throw;
   }
   G2G_HIP(hipStreamSynchronize(st));
-  outer.fini(); in64.fini(); in32.fini(); blas.fini();
-  for (void* q : {(void*)b32, (void*)x32, (void*)r32, (void*)tmp, (void*)tmp2})
-    if (q) (void)hipFree(q);
   return res;
 }
 
@@ -777,6 +786,7 @@ PairResult nested_solve_pair(Operator& op, const zd* const b[2], const zd* const
       G2G_HIP(hipStreamCreate(&s));
       SolveEnv env;
       env.stream = s;
+      env.ws = &op.ws[me];
       env.mv64 = &out.mv64[me];
       env.mv32 = &out.mv32[me];
       env.A64 = [&, me, s](const zd* in, zd* o) {

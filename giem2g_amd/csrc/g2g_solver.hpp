@@ -7,11 +7,27 @@
 
 namespace g2g {
 
+// Arena slots of one solve's device buffers (Operator::ws): the outer and
+// inner solvers' vectors and bases, the mixed-precision and learned-step
+// vectors, and the BLAS reduction buffers
+enum WsSlot {
+  WS_OUTER = 0,   // 5 slots: x, r0, w, V, Z
+  WS_IN64 = 5,    // 4 slots: x, r0, w, V
+  WS_IN32 = 9,    // 4 slots
+  WS_B32 = 13, WS_X32, WS_R32, WS_TMP, WS_TMP2,
+  WS_PART, WS_RES, WS_HBUF,
+  WS_RHS, WS_GUESS, WS_PAIR  // the C ABI's solve entry points
+};
+
+
 struct SolveParams {
   int outer_m = 30, inner_m = 30, max_restart = 100;
   double tol = 1e-8;
   int inner_precision = 0;  // 0 FP64, 1 complex64
   int precond = 0;          // 0 inner GMRES, 1 learned + inner GMRES fallback
+  // learned modes: after this many rejected learned steps in a row the
+  // network is not called again in the solve (0: never stop)
+  int learned_patience = 2;
 };
 
 struct SolveResult {
@@ -39,6 +55,7 @@ struct SolveEnv {
   std::function<void(const zf*, zf*)> A32;
   int64_t* mv64 = nullptr;
   int64_t* mv32 = nullptr;
+  Arena* ws = nullptr;  // the solve's device buffers
 };
 
 SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,

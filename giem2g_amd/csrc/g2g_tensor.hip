@@ -550,16 +550,18 @@ __device__ inline void entry_signs(int Nz, int64_t e, double* sx, double* sy) {
 template <int L> constexpr int ne_for() { return L <= 64 ? 32 : (L <= 128 ? 16 : (L <= 512 ? 8 : 4)); }
 
 // AXIS 0: transform along y for a fixed lx (blockIdx.x); AXIS 1: along x for fixed ky.
-template <int L, int AXIS>
+// L is the buffer capacity; with RT the transform length is plan.L <= L.
+template <int L, int AXIS, bool RT = false>
 __global__ __launch_bounds__(256) void k_tfft(zd* __restrict__ G, const zd* __restrict__ twg,
-                                              int Nx, int Ny, int Nz, int64_t BS) {
+                                              int Nx, int Ny, int Nz, int64_t BS, FftPlan plan) {
   constexpr int NE = ne_for<L>(), LS = L + 1;
   __shared__ zd buf[NE * LS];
   __shared__ zd tw[L];
-  const int N = L / 2;  // Ny (AXIS 0) or Nx (AXIS 1)
+  const int Lr = RT ? plan.L : L;
+  const int N = Lr / 2;  // Ny (AXIS 0) or Nx (AXIS 1)
   const int fixed = blockIdx.x;
   const int64_t e0 = (int64_t)blockIdx.y * NE;
-  for (int i = threadIdx.x; i < L; i += 256) tw[i] = twg[i];
+  for (int i = threadIdx.x; i < Lr; i += 256) tw[i] = twg[i];
   auto gidx = [&](int l) -> int64_t {
     return AXIS == 0 ? ((int64_t)fixed + (int64_t)Nx * l) * BS : ((int64_t)l + (int64_t)Nx * fixed) * BS;
   };
@@ -574,11 +576,11 @@ __global__ __launch_bounds__(256) void k_tfft(zd* __restrict__ G, const zd* __re
     }
     const double sgn = AXIS == 0 ? sy : sx;
     buf[s * LS + l] = g;
-    if (l > 0) buf[s * LS + L - l] = g * sgn;
+    if (l > 0) buf[s * LS + Lr - l] = g * sgn;
     else buf[s * LS + N] = mkz(0, 0);
   }
   __syncthreads();
-  lds_fft<zd, L, NE, LS, 256, -1>(buf, tw);
+  lds_fft_any<zd, L, NE, LS, 256, -1, RT>(buf, tw, plan);
   for (int i = threadIdx.x; i < NE * N; i += 256) {
     const int k = i / NE, s = i - k * NE;
     const int64_t e = e0 + s;
@@ -606,17 +608,18 @@ void op_tensor_fft(Operator& op, zd* G) {
   zd *twx, *twy;
   make_tw64(Nx2, -1, &twx);
   make_tw64(Ny2, -1, &twy);
-  pow2_dispatch(Ny2, [&](auto LC) {
+  const FftPlan py = fft_plan(Ny2), px = fft_plan(Nx2);
+  len_dispatch(Ny2, [&](auto LC) {
     constexpr int LL = decltype(LC)::value;
     constexpr int NE = ne_for<LL>();
-    hipLaunchKernelGGL((k_tfft<LL, 0>), dim3(op.Nx, (unsigned)((op.BS + NE - 1) / NE)), dim3(256), 0,
-                       op.stream, G, twy, op.Nx, op.Ny, op.Nz, op.BS);
+    hipLaunchKernelGGL((k_tfft<LL, 0, decltype(LC)::rt>), dim3(op.Nx, (unsigned)((op.BS + NE - 1) / NE)),
+                       dim3(256), 0, op.stream, G, twy, op.Nx, op.Ny, op.Nz, op.BS, py);
   });
-  pow2_dispatch(Nx2, [&](auto LC) {
+  len_dispatch(Nx2, [&](auto LC) {
     constexpr int LL = decltype(LC)::value;
     constexpr int NE = ne_for<LL>();
-    hipLaunchKernelGGL((k_tfft<LL, 1>), dim3(op.Ny, (unsigned)((op.BS + NE - 1) / NE)), dim3(256), 0,
-                       op.stream, G, twx, op.Nx, op.Ny, op.Nz, op.BS);
+    hipLaunchKernelGGL((k_tfft<LL, 1, decltype(LC)::rt>), dim3(op.Ny, (unsigned)((op.BS + NE - 1) / NE)),
+                       dim3(256), 0, op.stream, G, twx, op.Nx, op.Ny, op.Nz, op.BS, px);
   });
   G2G_HIP(hipGetLastError());
   G2G_HIP(hipStreamSynchronize(op.stream));
@@ -634,7 +637,7 @@ void op_free_tensor_scratch(Operator& op) {
 
 void op_build_tensor(Operator& op) {
   G2G_CHECK(op.have_bkg, -4, "background not set");
-  G2G_CHECK(op.Nl <= 64 && op.Nz <= 64, -3, "tensor build supports Nl <= 64 and Nz <= 64");
+  G2G_CHECK(op.Nl <= 64 && op.Nz <= MAX_NZ, -3, "tensor build supports Nl <= 64 and Nz <= 256");
   const int64_t M = (int64_t)op.Nx * op.Ny;
   const int Nz = op.Nz, P = Nz * (Nz + 1) / 2;
   const double rmin = std::sqrt(op.dx * op.dx + op.dy * op.dy) / 2.0;  // :102
@@ -700,6 +703,8 @@ void op_build_tensor(Operator& op) {
   G2G_HIP(hipMemcpy(d_z, op.z.data(), sizeof(double) * (Nz + 1), hipMemcpyHostToDevice));
   G2G_HIP(hipMemcpy(d_ln, op.lnum.data(), sizeof(int) * (Nz + 1), hipMemcpyHostToDevice));
   const size_t sm = sizeof(zd) * (size_t)(Nz + 1) * 20;
+  if (sm > 65536)
+    G2G_HIP(hipFuncSetAttribute((const void*)k_lambda, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
   hipLaunchKernelGGL(k_lambda, dim3(NTAPS), dim3(64), sm, op.stream, (zd*)op.tabs, d_lms, d_dz,
                      d_z, d_ln, lp);
   G2G_HIP(hipGetLastError());
@@ -909,7 +914,7 @@ __global__ __launch_bounds__(64) void k_rc_lambda(zd* __restrict__ tabs,
   const int Nz = L.Nz, Nl = L.Nl, rl = L.rl, N = L.cell;
   __shared__ zd eta[65], p[65][2], q[65][2], Arr[65][2], wlay[65][2], Arl[65][2];
   __shared__ zd sh_cs[65], sh_k2[65], sh_td[65];
-  __shared__ zd expz[65], pez[130], qez[130];
+  __shared__ zd expz[MAX_NZ + 1], pez[2 * (MAX_NZ + 1)], qez[2 * (MAX_NZ + 1)];
   for (int i = threadIdx.x; i <= Nl; i += blockDim.x) {
     sh_cs[i] = L.bk[i];
     sh_k2[i] = L.bk[Nl + 1 + i];
@@ -1005,25 +1010,26 @@ __global__ __launch_bounds__(64) void k_rc_lambda(zd* __restrict__ tabs,
 
 // In-place forward FFT of every tensor column along y for a fixed ix
 // (AXIS 0) or along x for a fixed iy (AXIS 1), points p = ix*Ny2 + iy.
-template <int L, int AXIS>
+template <int L, int AXIS, bool RT = false>
 __global__ __launch_bounds__(256) void k_rc_tfft(zd* __restrict__ G, const zd* __restrict__ twg, int Ny2,
-                                                 int64_t BS) {
+                                                 int64_t BS, FftPlan plan) {
   constexpr int NE = ne_for<L>(), LS = L + 1;
   __shared__ zd buf[NE * LS];
   __shared__ zd tw[L];
+  const int Lr = RT ? plan.L : L;
   const int fixed = blockIdx.x;
   const int64_t e0 = (int64_t)blockIdx.y * NE;
-  for (int i = threadIdx.x; i < L; i += 256) tw[i] = twg[i];
+  for (int i = threadIdx.x; i < Lr; i += 256) tw[i] = twg[i];
   auto gidx = [&](int l) -> int64_t {
     return AXIS == 0 ? ((int64_t)fixed * Ny2 + l) * BS : ((int64_t)l * Ny2 + fixed) * BS;
   };
-  for (int i = threadIdx.x; i < NE * L; i += 256) {
+  for (int i = threadIdx.x; i < NE * Lr; i += 256) {
     const int l = i / NE, s = i - l * NE;
     buf[s * LS + l] = (e0 + s < BS) ? G[gidx(l) + e0 + s] : mkz(0, 0);
   }
   __syncthreads();
-  lds_fft<zd, L, NE, LS, 256, -1>(buf, tw);
-  for (int i = threadIdx.x; i < NE * L; i += 256) {
+  lds_fft_any<zd, L, NE, LS, 256, -1, RT>(buf, tw, plan);
+  for (int i = threadIdx.x; i < NE * Lr; i += 256) {
     const int k = i / NE, s = i - k * NE;
     if (e0 + s < BS) G[gidx(k) + e0 + s] = buf[s * LS + k];
   }
@@ -1063,7 +1069,7 @@ void rc_prepare_receivers(RcOperator& rc) {
 void rc_build_tensor(RcOperator& rc) {
   Operator& op = *rc.op;
   G2G_CHECK(op.have_bkg, -4, "background not set");
-  G2G_CHECK(op.Nl <= 64 && op.Nz <= 64, -3, "tensor build supports Nl <= 64 and Nz <= 64");
+  G2G_CHECK(op.Nl <= 64 && op.Nz <= MAX_NZ, -3, "tensor build supports Nl <= 64 and Nz <= 256");
   rc_prepare_receivers(rc);
   const int Nz = op.Nz, Nr = rc.Nr;
   const int64_t NP = rc.NP, BS = rc.BS, WS = (int64_t)6 * NTAPS * NP;  // weights per receiver
@@ -1193,17 +1199,18 @@ void rc_tensor_fft(RcOperator& rc) {
   G2G_HIP(hipEventCreate(&a));
   G2G_HIP(hipEventCreate(&b));
   G2G_HIP(hipEventRecord(a, op.stream));
-  pow2_dispatch(Ny2, [&](auto LC) {
+  const FftPlan py = fft_plan(Ny2), px = fft_plan(Nx2);
+  len_dispatch(Ny2, [&](auto LC) {
     constexpr int LL = decltype(LC)::value;
     constexpr int NE = ne_for<LL>();
-    hipLaunchKernelGGL((k_rc_tfft<LL, 0>), dim3(Nx2, (unsigned)((rc.BS + NE - 1) / NE)), dim3(256), 0,
-                       op.stream, rc.G, twy, Ny2, rc.BS);
+    hipLaunchKernelGGL((k_rc_tfft<LL, 0, decltype(LC)::rt>), dim3(Nx2, (unsigned)((rc.BS + NE - 1) / NE)),
+                       dim3(256), 0, op.stream, rc.G, twy, Ny2, rc.BS, py);
   });
-  pow2_dispatch(Nx2, [&](auto LC) {
+  len_dispatch(Nx2, [&](auto LC) {
     constexpr int LL = decltype(LC)::value;
     constexpr int NE = ne_for<LL>();
-    hipLaunchKernelGGL((k_rc_tfft<LL, 1>), dim3(Ny2, (unsigned)((rc.BS + NE - 1) / NE)), dim3(256), 0,
-                       op.stream, rc.G, twx, Ny2, rc.BS);
+    hipLaunchKernelGGL((k_rc_tfft<LL, 1, decltype(LC)::rt>), dim3(Ny2, (unsigned)((rc.BS + NE - 1) / NE)),
+                       dim3(256), 0, op.stream, rc.G, twx, Ny2, rc.BS, px);
   });
   G2G_HIP(hipGetLastError());
   G2G_HIP(hipEventRecord(b, op.stream));

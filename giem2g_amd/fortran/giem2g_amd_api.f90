@@ -12,7 +12,7 @@ module giem2g_amd_api
   type, bind(c) :: giem2g_solver_params
      integer(c_int) :: outer_m, inner_m, max_restart
      real(c_double) :: tol
-     integer(c_int) :: inner_precision, precond, verbose, reserved
+     integer(c_int) :: inner_precision, precond, verbose, learned_patience
   end type giem2g_solver_params
 
   type, bind(c) :: giem2g_solve_info

@@ -255,8 +255,13 @@ class IEOperator:
         return E
 
     def solve(self, Ebkg, E0=None, outer_m=30, inner_m=30, max_restart=100, tol=1e-8,
-              inner_precision=0, precond=0, verbose=0):
-        p = L.SolverParams(outer_m, inner_m, max_restart, tol, inner_precision, precond, verbose, 0)
+              inner_precision=0, precond=0, verbose=0, learned_patience=0):
+        """Nested FGMRES solve of the scaled system for the background field
+        Ebkg (numpy: on the host; torch: on the GPU).  learned_patience:
+        rejected learned steps in a row after which the network is not
+        called again in this solve (0: 2, negative: never stop)."""
+        p = L.SolverParams(outer_m, inner_m, max_restart, tol, inner_precision, precond, verbose,
+                           learned_patience)
         info = L.SolveInfo()
         if isinstance(Ebkg, np.ndarray):
             Ebkg = np.ascontiguousarray(Ebkg, np.complex128)

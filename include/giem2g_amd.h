@@ -240,7 +240,9 @@ typedef struct {
   int inner_precision;
   int precond;
   int verbose;
-  int reserved;
+  int learned_patience; /* learned modes: after this many rejected learned
+                           steps in a row the network is not called again in
+                           the solve; 0 = 2 (default), < 0 = never stop */
 } giem2g_solver_params;
 
 typedef struct {

@@ -273,13 +273,17 @@ def test_component_order_yxz(oracle):
     b.close()
 
 
-def test_reference_abi_lifecycle(oracle):
+@pytest.mark.parametrize("Nx,Ny,Nz", [(16, 16, 4), (24, 40, 72)])
+def test_reference_abi_lifecycle(oracle, Nx, Ny, Nz):
     """giem2g_calc_data_sizes -> prepare -> calc_ie_kernel -> calc_fft ->
     set_anomaly_conductivity -> apply -> print_stats -> delete
-    (giem2g_interfaces.F90:76-254) gives the oracle's operator."""
+    (giem2g_interfaces.F90:76-254) gives the oracle's operator, also on a
+    grid the reference accepts beyond powers of two and 64 layers
+    (24x40x72: 48- and 80-point transforms, the tiled contraction)."""
     from giem2g_amd import lib as L, models
     lib = L.load()
-    m = models.small(16, 16, 4, seed=8, freq=5.0)
+    print("ABI lifecycle", Nx, Ny, Nz, flush=True)
+    m = models.small(Nx, Ny, Nz, seed=8, freq=5.0, dz=25.0)
     z = np.ascontiguousarray(m.z)
     dz = np.ascontiguousarray(np.diff(m.z))
     an = L.Anomaly(m.Nx, m.Ny, m.Nz, m.dx, m.dy, z.ctypes.data, dz.ctypes.data)
@@ -313,10 +317,12 @@ def test_reference_abi_lifecycle(oracle):
 
 def test_error_paths():
     from giem2g_amd import lib as L, operator as gop
-    with pytest.raises(L.G2GError, match="powers of two"):
+    with pytest.raises(L.G2GError, match="even and in"):
         gop.IEOperator(6, 8, 2, 100.0, 100.0, [0, 1, 2])
+    with pytest.raises(L.G2GError, match="even and in"):
+        gop.IEOperator(9, 8, 2, 100.0, 100.0, [0, 1, 2])
     with pytest.raises(L.G2GError, match="Nz"):
-        gop.IEOperator(8, 8, 70, 100.0, 100.0, np.arange(71.0))
+        gop.IEOperator(8, 8, 300, 100.0, 100.0, np.arange(301.0))
     op = gop.IEOperator(8, 8, 2, 100.0, 100.0, [0, 10, 20])
     with pytest.raises(L.G2GError, match="not ready"):
         op.apply_host(np.zeros(3 * 128, np.complex128))

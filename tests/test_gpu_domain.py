@@ -1,0 +1,145 @@
+"""Grids beyond powers of two and beyond 64 layers, as the reference accepts
+them (any even Nx, Ny: CalcLocalKernelSize, Integral_Equation_Module.F90:
+105-129; any Nz), against the oracle.
+
+  - 2Nx- and 2Ny-point transforms that are not powers of two run factor-plan
+    Stockham passes (radices 4, 2 and any odd prime: 20 = 4*5, 28 = 4*7,
+    44 = 4*11, 52 = 4*13, 68 = 4*17, 48 = 4*4*3, 80 = 4*4*5);
+  - more than 64 layers run the tiled vector-ALU contraction (72, 100, 256);
+  - the operator apply (FP64 within 1e-12, complex64 within 2e-6) on the
+    oracle's spectrum, the GPU tensor and its spectrum, and FP64 and
+    mixed-precision solves;
+  - the paper's COMMEMI3D-3 lateral grid, 176x224 (352- and 448-point
+    transforms), with sampled real-space blocks and an apply on 4 layers.
+"""
+import numpy as np
+import pytest
+
+from tests.conftest import rel, tensor_block_tol
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available()
+    return torch
+
+
+def oracle_model(po, m):
+    return po.Model(m.csigb(), m.thick, m.omega, m.Nx, m.Ny, m.Nz, m.dx, m.dy, m.z)
+
+
+def rand_vec(n, seed):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(n) + 1j * rng.standard_normal(n)
+
+
+@pytest.mark.parametrize("Nx,Ny,Nz", [(10, 14, 5), (24, 40, 8), (22, 12, 3), (12, 26, 6), (34, 16, 4),
+                                      (8, 8, 72), (12, 10, 100), (16, 8, 256)])
+def test_apply_on_oracle_spectrum(oracle, Nx, Ny, Nz):
+    torch = _torch()
+    from giem2g_amd import models, operator as gop
+    print("apply", Nx, Ny, Nz, flush=True)
+    m = models.small(Nx, Ny, Nz, seed=Nx + 3 * Ny + Nz, freq=0.8, dz=20.0)
+    om = oracle_model(oracle, m)
+    Gs = om.tensor_fft(om.tensor_realspace())
+    op = gop.IEOperator.from_model(m)
+    op.set_background(m.csigb(), m.thick, m.omega)
+    op.put_tensor(Gs, spectral=True)
+    op.set_sigma(m.csiga())
+    v = rand_vec(3 * m.ncell, Nz)
+    ref = om.apply(Gs, m.csiga(), v)
+    e64 = rel(op.apply_host(v), ref)
+    vt = torch.from_numpy(v).cuda().to(torch.complex64)
+    o32 = torch.empty_like(vt)
+    op.apply_dev_f32(vt, o32)
+    op.sync()
+    e32 = rel(o32.cpu().numpy().astype(np.complex128), ref)
+    print("  FP64 %.2e complex64 %.2e" % (e64, e32), flush=True)
+    assert e64 < 1e-12 and e32 < 2e-6
+    op.close()
+
+
+@pytest.mark.parametrize("Nx,Ny,Nz", [(24, 40, 8), (10, 14, 72)])
+def test_tensor_and_spectrum(oracle, Nx, Ny, Nz):
+    """Layers of 20 m, so that the layer borders meet the background's 1 km
+    interface.  A cell that straddles an interface (15 m layers: 990-1005 m)
+    is outside what the layered kernel assumes (one background layer per
+    cell, the layer of the cell's middle): there the reference's own Fortran
+    blocks and the oracle's differ by 2e-2, and so would the GPU's."""
+    from giem2g_amd import models, operator as gop
+    print("tensor", Nx, Ny, Nz, flush=True)
+    m = models.small(Nx, Ny, Nz, seed=5 * Nx + Nz, freq=2.0, dz=20.0)
+    om = oracle_model(oracle, m)
+    Gr = om.tensor_realspace()
+    op = gop.IEOperator.from_model(m)
+    op.set_background(m.csigb(), m.thick, m.omega)
+    op.tensor_stage(0)
+    Gg = op.get_tensor()
+    BS = op.block_size
+    for lx, ly in [(0, 0), (1, 0), (0, 1), (Nx - 1, Ny - 1), (Nx // 2, 3)]:
+        b = lx + Nx * ly
+        assert np.linalg.norm(Gg[b * BS:(b + 1) * BS] - Gr[b * BS:(b + 1) * BS]) < tensor_block_tol(om, lx, ly)
+    assert rel(Gg, Gr) < 1e-10
+    op.tensor_stage(1)
+    e = rel(op.get_tensor(), om.tensor_fft(Gg))
+    print("  spectrum %.2e" % e, flush=True)
+    assert e < 1e-12
+    op.close()
+
+
+@pytest.mark.parametrize("Nx,Ny,Nz", [(20, 12, 6), (12, 10, 72)])
+def test_solves(oracle, Nx, Ny, Nz):
+    torch = _torch()
+    from giem2g_amd import models, operator as gop
+    print("solve", Nx, Ny, Nz, flush=True)
+    m = models.small(Nx, Ny, Nz, seed=Nx * Ny + Nz, freq=1.0, dz=20.0)
+    om = oracle_model(oracle, m)
+    Gs = om.tensor_fft(om.tensor_realspace())
+    op = gop.IEOperator.from_model(m)
+    op.set_background(m.csigb(), m.thick, m.omega)
+    op.put_tensor(Gs, spectral=True)
+    op.set_sigma(m.csiga())
+    Eb = op.plane_wave(1)
+    kw = dict(outer_m=15, inner_m=15, max_restart=50, tol=1e-8)
+    Eo, oi = om.solve(Gs, m.csiga(), Eb, m_out=15, m_in=15, maxit=50, tol=1e-8)
+    E64, i64 = op.solve(Eb, **kw)
+    Emx, imx = op.solve(torch.from_numpy(Eb).cuda(), inner_precision=1, **kw)
+    e64, emx = rel(E64, Eo), rel(Emx.cpu().numpy(), Eo)
+    print("  FP64 %.2e (outer %d / %d), mixed %.2e" % (e64, i64["outer_iterations"], oi.iterations, emx),
+          flush=True)
+    assert i64["status"] == 1 and imx["status"] == 1 and oi.stat == 1
+    assert i64["outer_iterations"] == oi.iterations
+    assert e64 < 1e-6 and emx < 1e-6
+    op.close()
+
+
+def test_commemi3d3_lateral_grid(oracle):
+    """176 x 224 cells (the paper's scaling model, giem2g.pdf p.10) on 4
+    layers: sampled real-space blocks against the oracle's elements, and one
+    FP64 apply against the oracle's operator on the GPU's spectrum (the
+    352- and 448-point passes)."""
+    from giem2g_amd import models, operator as gop
+    print("176x224x4", flush=True)
+    m = models.small(176, 224, 4, seed=3, freq=1.0, dz=50.0)
+    om = oracle_model(oracle, m)
+    op = gop.IEOperator.from_model(m)
+    op.set_background(m.csigb(), m.thick, m.omega)
+    op.tensor_stage(0)
+    Gg = op.get_tensor()
+    BS = op.block_size
+    for lx, ly in [(0, 0), (1, 2), (175, 0), (0, 223), (100, 150), (175, 223)]:
+        b = lx + 176 * ly
+        blk = om.tensor_element(lx, ly)
+        assert np.linalg.norm(Gg[b * BS:(b + 1) * BS] - blk) < tensor_block_tol(om, lx, ly), (lx, ly)
+    del Gg
+    op.tensor_stage(1)
+    Gs = op.get_tensor()
+    op.set_sigma(m.csiga())
+    v = rand_vec(3 * m.ncell, 17)
+    print("  oracle apply", flush=True)
+    e = rel(op.apply_host(v), om.apply(Gs, m.csiga(), v))
+    print("  apply %.2e" % e, flush=True)
+    assert e < 1e-12
+    op.close()

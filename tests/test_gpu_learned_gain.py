@@ -7,9 +7,9 @@ training range 10000..11999).
 Measured with these weights: on the residuals of the outer iterates the
 network's step reduces the residual by less than 1% (least-squares
 rho = ||r - alpha A M(r)|| / ||r|| > 0.99), so the acceptance test rejects
-it and the fallback runs the plain inner GMRES: the gain is nil and the cost
-is one extra operator application and one network application per outer
-step.  This test asserts what the fallback guarantees -- the same
+it and the fallback runs the plain inner GMRES: the gain is nil, and after
+two rejected steps in a row the solve stops calling the network, so the cost
+is at most two extra operator applications and two network applications.  This test asserts what the fallback guarantees -- the same
 convergence, the same field within 1e-6, at most that overhead -- and
 prints the measured gain and rho.
 """
@@ -62,7 +62,12 @@ def test_learned_preconditioner_gain(seed, mode):
     for info in (i0, i1):
         assert info["status"] == 1 and info["true_residual"] < 1e-8
     assert rel(E1, E0) < 1e-6
-    assert i1["precond_calls"] == i1["outer_iterations"]
+    # after two rejected learned steps in a row the solve stops calling the
+    # network (learned_patience, default 2)
+    rejected = i1["precond_fallbacks"]
+    assert 1 <= i1["precond_calls"] <= i1["outer_iterations"]
+    if rejected == i1["precond_calls"]:
+        assert i1["precond_calls"] == min(2, i1["outer_iterations"])
     assert i1["outer_iterations"] <= i0["outer_iterations"] + 1
     assert cost1 <= cost0 + 2 * i1["precond_calls"] + 2
     op.set_learned(None)

@@ -60,7 +60,8 @@ def test_receiver_preparation_and_plane_wave(oracle):
     op.close()
 
 
-@pytest.mark.parametrize("Nx,Ny,Nz,Nr", [(8, 8, 4, 3), (16, 8, 3, 4), (8, 16, 5, 1)])
+@pytest.mark.parametrize("Nx,Ny,Nz,Nr", [(8, 8, 4, 3), (16, 8, 3, 4), (8, 16, 5, 1),
+                                          (10, 14, 5, 3), (8, 8, 72, 1)])
 def test_receiver_tensor_matches_oracle(oracle, Nx, Ny, Nz, Nr):
     from giem2g_amd import models
     print("case", Nx, Ny, Nz, Nr, flush=True)
@@ -81,7 +82,7 @@ def test_receiver_tensor_matches_oracle(oracle, Nx, Ny, Nz, Nr):
 
 
 @pytest.mark.parametrize("Nx,Ny,Nz,Nr", [(8, 8, 4, 3), (16, 8, 3, 4), (8, 16, 5, 1), (8, 8, 1, 3),
-                                          (16, 16, 32, 4)])
+                                          (16, 16, 32, 4), (12, 10, 6, 4), (8, 8, 70, 3)])
 def test_continuation_matches_oracle_on_oracle_tensor(oracle, Nx, Ny, Nz, Nr):
     """The FFT continuation alone (forward FFTs, 17 block products, inverse
     FFTs) on the oracle's spectrum; host and HBM entry points agree exactly."""
