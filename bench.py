@@ -483,6 +483,9 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the 128x128x64 multi-frequency run")
     ap.add_argument("--c3-freqs", type=int, default=16, help="number of C3 frequencies")
     ap.add_argument("--headline-only", action="store_true", help="only the C2 solves (profiling runs)")
+    ap.add_argument("--no-solve", action="store_true",
+                    help="skip the C2 solves: only the C1 apply, continuation and batched records "
+                         "(profiling runs of those kernels)")
     ap.add_argument("--clock-warmup", type=float, default=CLOCK_WARMUP_S,
                     help="seconds of FP64 applies before the timed C2 solves (0 for counter runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -502,6 +505,15 @@ def main():
         dst.copy_(src)  # reads and writes exactly 1 GiB
         torch.cuda.synchronize()
         del src, dst
+    if a.no_solve:
+        line = {"metric": "IE operator apply (profiling run, no solves)"}
+        m1, op1, v1, st64, st32, c1rec = c1_apply(torch, gop, models, 200, not a.no_cpu)
+        line["c1_apply"] = c1rec
+        line["recalc"] = recalc_bench(torch, gop, op1, m1, v1, cpu=not a.no_cpu)
+        line["batched"] = batch_bench(torch, op1, m1, st64, st32)
+        op1.close()
+        print(json.dumps(line), flush=True)
+        return 0
     m2, op2, E2, head = c2_headline(torch, gop, models, a.steps, a.warmup, a.clock_warmup)
     top = head.get("mixed+learned", head["mixed"])
     top_name = "mixed+learned" if "mixed+learned" in head else "mixed"
