@@ -535,8 +535,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS if ach else None,
                      "traffic": read_traffic(os.path.join(ROOT, "profiles", "contraction_c2_c64_traffic.txt")),
-                     "kernel": "k_contract3<float,32,1,...> (complex64 tensor contraction of the inner solver, "
-                               "matrix cores)",
+                     "kernel": "k_contract1<float,32,false,4> (complex64 tensor contraction of the inner "
+                               "solver, small-block matrix cores)",
                      "algorithmic_bytes_per_launch": b32["contraction"],
                      "avg_launch_ms": cc.get("avg_launch_ms"), "launches": cc.get("launches")},
         "solve": head,

@@ -20,6 +20,7 @@
 
 #include <cmath>
 #include <type_traits>
+#include <utility>
 
 namespace g2g {
 
@@ -1060,7 +1061,7 @@ __device__ __forceinline__ void contract3_body(cx_t<R>* __restrict__ S, int64_t 
   static_assert(NZ % 16 == 0 && PF <= 16, "Nz must be a multiple of 16, at most 64");
   static_assert((NW == 4 || NW == 8) && KW % 4 == 0, "four or eight waves, whole MFMA steps each");
   static_assert(NRHS == 1 || NRHS == 2 || NRHS == 4 || NRHS == 8, "1, 2, 4 or 8 right-hand sides");
-  static_assert(DEEP == 1 || DEEP == 2, "one or two components in flight");
+  static_assert(DEEP >= 1 && DEEP <= 6, "one to six components in flight");
   extern __shared__ unsigned char smem_raw[];
   T2* vin = reinterpret_cast<T2*>(smem_raw);  // [NRHS][4][3NZ], point-signed inputs; rhs r at S + r*sst
   T2* gbuf = vin + NRHS * 4 * NZ3;            // [2][GBS]
@@ -1084,15 +1085,20 @@ __device__ __forceinline__ void contract3_body(cx_t<R>* __restrict__ S, int64_t 
 #define VAL(p) ((!((p) & 1) || kx > 0) && (!((p) & 2) || ky > 0))
   const T2* blk = G + (int64_t)b * BS;
   const T2 zero = mk<T2>(0, 0);
-  // two register sets F and G (named registers: arrays of FP64 pairs were
-  // placed in scratch memory by the compiler)
-#define C3_DECL(q) T2 rf##q = zero, rg##q = zero;
+  // DEEP register sets F, G, H, I, J, K; component c travels in set c % DEEP
+  // (named registers: arrays of FP64 pairs were placed in scratch memory by
+  // the compiler)
+#define C3_DECL(q) T2 rf##q = zero, rg##q = zero, rh##q = zero, ri##q = zero, rj##q = zero, rk##q = zero;
   G2G_FOR16(C3_DECL)
 #undef C3_DECL
   // loads are unconditional (index clamped into the component) so that the
   // compiler counts outstanding loads exactly and keeps later sets in flight
 #define C3_LDF(q) if ((q) < PF) rf##q = ld_stream(src_ + min(tid + (q) * NTH, n_ - 1));
 #define C3_LDG(q) if ((q) < PF) rg##q = ld_stream(src_ + min(tid + (q) * NTH, n_ - 1));
+#define C3_LDH(q) if ((q) < PF) rh##q = ld_stream(src_ + min(tid + (q) * NTH, n_ - 1));
+#define C3_LDI(q) if ((q) < PF) ri##q = ld_stream(src_ + min(tid + (q) * NTH, n_ - 1));
+#define C3_LDJ(q) if ((q) < PF) rj##q = ld_stream(src_ + min(tid + (q) * NTH, n_ - 1));
+#define C3_LDK(q) if ((q) < PF) rk##q = ld_stream(src_ + min(tid + (q) * NTH, n_ - 1));
 #define C3_PREFETCH(c, SET)                                                           \
   {                                                                                   \
     const T2* src_ = (c) < 4 ? blk + (c) * Pn : blk + 4 * Pn + ((c) - 4) * Fn;        \
@@ -1107,12 +1113,26 @@ __device__ __forceinline__ void contract3_body(cx_t<R>* __restrict__ S, int64_t 
   }
 #define C3_STF(q) C3_STX(q, rf)
 #define C3_STG(q) C3_STX(q, rg)
+#define C3_STH(q) C3_STX(q, rh)
+#define C3_STI(q) C3_STX(q, ri)
+#define C3_STJ(q) C3_STX(q, rj)
+#define C3_STK(q) C3_STX(q, rk)
 #define C3_STASH(c, SET, dst)          \
   {                                    \
     const int c_ = (c);                \
     T2* dst_ = (dst);                  \
     const int n_ = c_ < 4 ? Pn : Fn;   \
     G2G_FOR16(C3_ST##SET)              \
+  }
+  // the same with the set given by its compile-time number
+#define C3_SETS(OP, c, k, ...)                          \
+  {                                                     \
+    if constexpr ((k) == 0) OP(c, F, ##__VA_ARGS__)     \
+    else if constexpr ((k) == 1) OP(c, G, ##__VA_ARGS__) \
+    else if constexpr ((k) == 2) OP(c, H, ##__VA_ARGS__) \
+    else if constexpr ((k) == 3) OP(c, I, ##__VA_ARGS__) \
+    else if constexpr ((k) == 4) OP(c, J, ##__VA_ARGS__) \
+    else OP(c, K, ##__VA_ARGS__)                        \
   }
   // the point rows: complex64 issues them first as unrolled clamped loads
   // (paired Nz = 64: 6% faster); FP64 keeps the plain loop (the early
@@ -1144,7 +1164,11 @@ __device__ __forceinline__ void contract3_body(cx_t<R>* __restrict__ S, int64_t 
     G2G_FOR16(C3_VQL)
   }
   C3_PREFETCH(0, F)
-  if (DEEP == 2) C3_PREFETCH(1, G)
+  if constexpr (DEEP > 1) C3_PREFETCH(1, G)
+  if constexpr (DEEP > 2) C3_PREFETCH(2, H)
+  if constexpr (DEEP > 3) C3_PREFETCH(3, I)
+  if constexpr (DEEP > 4) C3_PREFETCH(4, J)
+  if constexpr (DEEP > 5) C3_PREFETCH(5, K)
   if constexpr (EARLY) {
     G2G_FOR16(C3_VQP)
   } else {
@@ -1193,14 +1217,8 @@ __device__ __forceinline__ void contract3_body(cx_t<R>* __restrict__ S, int64_t 
   // that accumulator and register-set indices stay static
   auto step = [&](auto CC) {
     constexpr int c = decltype(CC)::value;
-    if constexpr (DEEP == 2) {
-      if constexpr (c + 2 < 6) {
-        if constexpr ((c & 1) != 0) C3_PREFETCH(c + 2, G)
-        else C3_PREFETCH(c + 2, F)
-      }
-    } else if constexpr (c + 1 < 6) {
-      C3_PREFETCH(c + 1, F)
-    }
+    // the set of component c was stashed at the end of the previous step
+    if constexpr (c + DEEP < 6) C3_SETS(C3_PREFETCH, c + DEEP, c % DEEP)
     const T2* cur = gbuf + (c & 1) * GBS;
     // input components of the step's first and second use
     constexpr int ciA = c == 0 ? 0 : (c == 3 || c >= 4 ? 2 : 1);
@@ -1243,8 +1261,7 @@ __device__ __forceinline__ void contract3_body(cx_t<R>* __restrict__ S, int64_t 
       }
     }
     if constexpr (c < 5) {
-      if constexpr (DEEP == 2 && (c & 1) == 0) C3_STASH(c + 1, G, gbuf + ((c + 1) & 1) * GBS)
-      else C3_STASH(c + 1, F, gbuf + ((c + 1) & 1) * GBS)
+      C3_SETS(C3_STASH, c + 1, (c + 1) % DEEP, gbuf + ((c + 1) & 1) * GBS)
     }
     __syncthreads();
   };
@@ -1314,10 +1331,19 @@ __device__ __forceinline__ void contract3_body(cx_t<R>* __restrict__ S, int64_t 
 #undef C3_DIVNZ3
 #undef C3_LDF
 #undef C3_LDG
+#undef C3_LDH
+#undef C3_LDI
+#undef C3_LDJ
+#undef C3_LDK
 #undef C3_PREFETCH
 #undef C3_STX
 #undef C3_STF
 #undef C3_STG
+#undef C3_STH
+#undef C3_STI
+#undef C3_STJ
+#undef C3_STK
+#undef C3_SETS
 #undef C3_STASH
 #undef PX
 #undef PY
@@ -1390,6 +1416,292 @@ static bool launch_contract3(const Operator& op, const cx_t<R>* G, cx_t<R>* S, i
   }
   return true;
 }
+// ------------------------------------------- one vector, small-block MFMA ---
+// The product of one vector with every output column a point.  The 16x16x4
+// tiles above fill eight of their sixteen operand columns when one vector is
+// contracted (four points x Re/Im), so half their matrix-core work repeats.
+// Here every block of v_mfma_f32_4x4x1_16b (sixteen 4x4 blocks, each with its
+// own operands) and of v_mfma_f64_4x4x4_4b (four 4x4x4 blocks) has the four
+// points as its columns and one (FP32) or four (FP64) source layers as its
+// depth, so no column repeats:
+//   Re out += Gr Re v - Gi Im v,  Im out += Gr Im v + Gi Re v
+// (four real products per tensor element).  Lane layouts (measured):
+//   f32 4x4x1_16b: block b = lane/4; lane 4b+i supplies A_b[row i] and
+//     B_b[col i]; lane 4b+c receives D_b[row q][col c] in register q.
+//   f64 4x4x4_4b: lane 16k+4b+m supplies A_b[row m][k], B_b[k][col m] and
+//     receives D_b[row k][col m].
+// The whole tensor block of the (kx,ky) point (six components) is copied into
+// LDS by direct global-to-LDS loads issued at once, so a block waits for one
+// memory latency instead of one per component: the packed symmetric
+// components as stored (16-byte lanes), the full xz and yz components
+// gathered into an XOR-swizzled column layout, element (row, col) at
+// col*NZ + (row ^ (col & SWM)), whose columns (A v) and rows (A^T v) are both
+// read without bank conflicts.  Waves split the source layers; partial sums
+// are added through LDS in a fixed order (results do not depend on
+// scheduling).
+template <typename R, int NZ, int NW_ = 4> struct C1Cfg {
+  static constexpr bool F32 = sizeof(R) == 4;
+  static constexpr int NW = NW_, NTH = 64 * NW, JW = NZ / NW;
+  static constexpr int P = NZ * (NZ + 1) / 2;
+  static constexpr int ES = (int)sizeof(cx_t<R>);
+  static constexpr int SWM = (F32 && NZ >= 32) ? 31 : 15;
+  // FP32: one instruction = one output component, G source layers x NZ rows
+  static constexpr int G = 64 / NZ;
+  // FP64: one instruction = sixteen output rows (four blocks of four)
+  static constexpr int NI = F32 ? 3 : 3 * NZ / 16;
+  static constexpr int VIN_B = 3 * NZ * 4 * ES;                  // [ci][z][point]
+  static constexpr int SYM_B = (4 * P * ES + 1023) / 1024 * 1024;  // whole 1 KB copies
+  static constexpr int FULL_B = 2 * NZ * NZ * ES;
+  static constexpr int RED_B = NW * NI * 2 * (F32 ? 256 : 64) * (int)sizeof(R);
+  static constexpr int smem = VIN_B + (SYM_B + FULL_B > RED_B ? SYM_B + FULL_B : RED_B);
+};
+
+#define G2G_LDS(p) ((__attribute__((address_space(3))) void*)(p))
+// cache policy of the tensor copies: non-temporal (each apply reads the tensor
+// once; measured 6% faster for complex64 at 256x256x32, 8% for FP64 at
+// 128x128x32, than the default policy)
+constexpr int C1_NT = 2;
+// f(integral_constant<int, i>) for i = 0..N-1
+template <typename F, int... Is> __device__ __forceinline__ void static_for_seq(F& f, std::integer_sequence<int, Is...>) {
+  (f(std::integral_constant<int, Is>{}), ...);
+}
+template <int N, typename F> __device__ __forceinline__ void static_for(F& f) {
+  static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
+
+template <typename R, int NZ, bool PAD, int NW_>
+__global__ __launch_bounds__(64 * NW_) void k_contract1(cx_t<R>* __restrict__ S, const cx_t<R>* __restrict__ G, Geo g,
+                                                        int64_t BS) {
+  using T2 = cx_t<R>;
+  using C = C1Cfg<R, NZ, NW_>;
+  constexpr bool F32 = C::F32;
+  constexpr int NW = C::NW, NTH = C::NTH, JW = C::JW, ES = C::ES, SWM = C::SWM, NI = C::NI;
+  static_assert(NZ == 16 || NZ == 32, "16 or 32 layers (fewer: zero-padded)");
+  static_assert(JW % (F32 ? C::G : 4) == 0, "whole source-layer groups per wave");
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T2* vin = reinterpret_cast<T2*>(smem_raw);
+  const T2* sym = reinterpret_cast<const T2*>(smem_raw + C::VIN_B);
+  const T2* full = reinterpret_cast<const T2*>(smem_raw + C::VIN_B + C::SYM_B);
+  R* red = reinterpret_cast<R*>(smem_raw + C::VIN_B);  // after the products, over the tensor
+  const int nz = PAD ? g.Nz : NZ, nz3 = 3 * nz;
+  const int Pn = PAD ? nz * (nz + 1) / 2 : C::P;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int kx = blockIdx.x / g.Ny, ky = blockIdx.x % g.Ny, b = kx + g.Nx * ky;
+  const int Nx2 = 2 * g.Nx, Ny2 = 2 * g.Ny, gNx = g.Nx, gNy = g.Ny;
+#define PX(p) (((p) & 1) ? (kx == 0 ? gNx : Nx2 - kx) : kx)
+#define PY(p) (((p) & 2) ? (ky == 0 ? gNy : Ny2 - ky) : ky)
+#define VAL(p) ((!((p) & 1) || kx > 0) && (!((p) & 2) || ky > 0))
+  const T2* blk = G + (int64_t)b * BS;
+  const T2 zero = mk<T2>(0, 0);
+  // 1. the four point rows into registers (issued first: waiting for them
+  //    does not wait for the tensor copies issued after them)
+  constexpr int NV = 4 * 3 * NZ, NVQ = (NV + NTH - 1) / NTH;
+  T2 vq[NVQ];
+#pragma unroll
+  for (int k = 0; k < NVQ; k++) {
+    const int i = tid + k * NTH;
+    const int p = i / (3 * NZ), e = i - p * (3 * NZ), ci = e / NZ, z = e - ci * NZ;
+    vq[k] = zero;
+    if (i < NV && VAL(p) && z < nz) vq[k] = S[((int64_t)PX(p) * Ny2 + PY(p)) * nz3 + ci * nz + z];
+  }
+  // 2. the tensor block into LDS
+  {
+    const unsigned char* gb = reinterpret_cast<const unsigned char*>(blk);
+    const int ngr = 4 * Pn * ES / 16;  // 16-byte pieces of the packed components (4 Pn is even)
+    const int nts = (ngr + 63) / 64;
+    for (int t = wave; t < nts; t += NW) {
+      const int gi = min(t * 64 + lane, ngr - 1);
+      __builtin_amdgcn_global_load_lds((const void*)(gb + 16 * (int64_t)gi), G2G_LDS(smem_raw + C::VIN_B + t * 1024),
+                                       16, 0, C1_NT);
+    }
+    const T2* fsrc = blk + 4 * Pn;
+    constexpr int PER = F32 ? 32 : 64;  // complex values per copy instruction
+    constexpr int NTF = 2 * NZ * NZ / PER;
+    for (int t = wave; t < NTF; t += NW) {
+      const int pos = t * PER + (F32 ? lane >> 1 : lane);
+      const int comp = pos / (NZ * NZ), loc = pos - comp * (NZ * NZ);
+      const int col = loc / NZ, row = (loc - col * NZ) ^ (col & SWM);
+      // padded rows and columns copy a real element (finite; its products
+      // meet zero operands or land in rows that are not stored)
+      const int rc = PAD ? min(row, nz - 1) : row, cc = PAD ? min(col, nz - 1) : col;
+      const T2* src = fsrc + comp * nz * nz + cc * nz + rc;
+      unsigned char* dst = smem_raw + C::VIN_B + C::SYM_B + t * (PER * ES);
+      if constexpr (F32)
+        __builtin_amdgcn_global_load_lds((const void*)(reinterpret_cast<const float*>(src) + (lane & 1)), G2G_LDS(dst),
+                                         4, 0, C1_NT);
+      else
+        __builtin_amdgcn_global_load_lds((const void*)src, G2G_LDS(dst), 16, 0, C1_NT);
+    }
+  }
+  // 3. point rows with the point signs (x times -1 for -kx, y for -ky),
+  //    layer-major with the four points innermost
+#pragma unroll
+  for (int k = 0; k < NVQ; k++) {
+    const int i = tid + k * NTH;
+    const int p = i / (3 * NZ), e = i - p * (3 * NZ), ci = e / NZ, z = e - ci * NZ;
+    T2 v = vq[k];
+    if ((ci == 0 && (p & 1)) || (ci == 1 && (p & 2))) v = mk<T2>(-v.x, -v.y);
+    if (i < NV) vin[(ci * NZ + z) * 4 + p] = v;
+  }
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  // 4. products.  Tensor element (r, j) of output component co and input ci:
+  //    ci = x: Sxx, Sxy, -Axz^T;  ci = y: Sxy, Syy, -Ayz^T;  ci = z: Axz, Ayz, Szz
+  auto fetch = [&](auto CI, auto CO, int r, int j, int tr, int tj) -> T2 {
+    constexpr int ci = decltype(CI)::value, co = decltype(CO)::value;
+    if constexpr (ci == 2 && co < 2) {
+      return full[co * NZ * NZ + j * NZ + (r ^ (j & SWM))];
+    } else if constexpr (ci < 2 && co == 2) {
+      const T2 a = full[ci * NZ * NZ + r * NZ + (j ^ (r & SWM))];
+      return mk<T2>(-a.x, -a.y);
+    } else {
+      constexpr int comp = ci == 2 ? 3 : (ci != co ? 1 : (ci == 0 ? 0 : 2));
+      T2 a = sym[comp * Pn + (j <= r ? j + tr : r + tj)];
+      if (PAD && j >= nz) a = zero;  // beyond the packed rows: not stored
+      return a;
+    }
+  };
+  using std::integral_constant;
+  if constexpr (F32) {
+    using V4 = v4t<float>::t;
+    V4 are[3], aim[3];
+#pragma unroll
+    for (int co = 0; co < 3; co++) are[co] = aim[co] = V4{0, 0, 0, 0};
+    // lane: A row r, source-layer offset gl, point column pc
+    const int r = lane % NZ, gl = lane / NZ, pc = lane & 3, tr = r * (r + 1) / 2;
+    auto ci_pass = [&](auto CI) {
+      constexpr int ci = decltype(CI)::value;
+#pragma unroll
+      for (int jg = 0; jg < JW / C::G; jg++) {
+        const int j = wave * JW + jg * C::G + gl, tj = j * (j + 1) / 2;
+        const T2 bv = vin[(ci * NZ + j) * 4 + pc];
+        const float nbi = -bv.y;
+        const T2 a0 = fetch(CI, integral_constant<int, 0>{}, r, j, tr, tj);
+        const T2 a1 = fetch(CI, integral_constant<int, 1>{}, r, j, tr, tj);
+        const T2 a2 = fetch(CI, integral_constant<int, 2>{}, r, j, tr, tj);
+        are[0] = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.x, bv.x, are[0], 0, 0, 0);
+        are[1] = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.x, bv.x, are[1], 0, 0, 0);
+        are[2] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.x, bv.x, are[2], 0, 0, 0);
+        aim[0] = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.x, bv.y, aim[0], 0, 0, 0);
+        aim[1] = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.x, bv.y, aim[1], 0, 0, 0);
+        aim[2] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.x, bv.y, aim[2], 0, 0, 0);
+        are[0] = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.y, nbi, are[0], 0, 0, 0);
+        are[1] = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.y, nbi, are[1], 0, 0, 0);
+        are[2] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.y, nbi, are[2], 0, 0, 0);
+        aim[0] = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.y, bv.x, aim[0], 0, 0, 0);
+        aim[1] = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.y, bv.x, aim[1], 0, 0, 0);
+        aim[2] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.y, bv.x, aim[2], 0, 0, 0);
+      }
+    };
+    ci_pass(integral_constant<int, 0>{});
+    ci_pass(integral_constant<int, 1>{});
+    ci_pass(integral_constant<int, 2>{});
+    __syncthreads();
+#pragma unroll
+    for (int co = 0; co < 3; co++) {
+      *reinterpret_cast<V4*>(red + ((wave * 3 + co) * 2 + 0) * 256 + lane * 4) = are[co];
+      *reinterpret_cast<V4*>(red + ((wave * 3 + co) * 2 + 1) * 256 + lane * 4) = aim[co];
+    }
+  } else {
+    double are[NI], aim[NI];
+#pragma unroll
+    for (int I = 0; I < NI; I++) are[I] = aim[I] = 0.0;
+    // lane 16k + 4b + m: source layer k of the quad, block b, A row / B column m
+    const int kq = lane >> 4, bq = (lane >> 2) & 3, m = lane & 3;
+    auto ci_pass = [&](auto CI) {
+      constexpr int ci = decltype(CI)::value;
+#pragma unroll
+      for (int jq = 0; jq < JW / 4; jq++) {
+        const int j = wave * JW + 4 * jq + kq, tj = j * (j + 1) / 2;
+        const T2 bv = vin[(ci * NZ + j) * 4 + m];
+        const double nbi = -bv.y;
+        auto rows = [&](auto II) {
+          constexpr int I = decltype(II)::value, co = (4 * I) / (NZ / 4);
+          const int r = 4 * ((4 * I + bq) % (NZ / 4)) + m;
+          const T2 a = fetch(CI, integral_constant<int, co>{}, r, j, r * (r + 1) / 2, tj);
+          are[I] = __builtin_amdgcn_mfma_f64_4x4x4f64(a.x, bv.x, are[I], 0, 0, 0);
+          aim[I] = __builtin_amdgcn_mfma_f64_4x4x4f64(a.x, bv.y, aim[I], 0, 0, 0);
+          are[I] = __builtin_amdgcn_mfma_f64_4x4x4f64(a.y, nbi, are[I], 0, 0, 0);
+          aim[I] = __builtin_amdgcn_mfma_f64_4x4x4f64(a.y, bv.x, aim[I], 0, 0, 0);
+        };
+        static_for<NI>(rows);
+      }
+    };
+    ci_pass(integral_constant<int, 0>{});
+    ci_pass(integral_constant<int, 1>{});
+    ci_pass(integral_constant<int, 2>{});
+    __syncthreads();
+#pragma unroll
+    for (int I = 0; I < NI; I++) {
+      red[((wave * NI + I) * 2 + 0) * 64 + lane] = are[I];
+      red[((wave * NI + I) * 2 + 1) * 64 + lane] = aim[I];
+    }
+  }
+  __syncthreads();
+  // 5. sums over waves (and FP32 layer offsets) in a fixed order, output signs, store
+  for (int e = tid; e < 4 * nz3; e += NTH) {
+    const int p = e / nz3, f = e - p * nz3, co = f / nz, z = f - co * nz;
+    R sr = 0, si = 0;
+    if constexpr (F32) {
+#pragma unroll
+      for (int w = 0; w < NW; w++)
+#pragma unroll
+        for (int gg = 0; gg < C::G; gg++) {
+          const int at = ((w * 3 + co) * 2) * 256 + (4 * (gg * (NZ / 4) + (z >> 2)) + p) * 4 + (z & 3);
+          sr += red[at];
+          si += red[at + 256];
+        }
+    } else {
+      const int rb = co * (NZ / 4) + (z >> 2), I = rb >> 2, bq = rb & 3;
+#pragma unroll
+      for (int w = 0; w < NW; w++) {
+        const int at = ((w * NI + I) * 2) * 64 + 16 * (z & 3) + 4 * bq + p;
+        sr += red[at];
+        si += red[at + 64];
+      }
+    }
+    // output sign as arithmetic on 0/1 flags: the short-circuit form
+    // (co == 0 && p & 1) || (co == 1 && p & 2) selecting -sum compiled, for
+    // FP32 with this toolchain, to code that kept +sum for co = 1, p & 2
+    const int flip = ((co == 0) & (p & 1)) | ((co == 1) & ((p >> 1) & 1));
+    const R so = R(1 - 2 * flip);
+    const T2 o = VAL(p) ? mk<T2>(so * sr, so * si) : zero;
+    S[((int64_t)PX(p) * Ny2 + PY(p)) * nz3 + f] = o;
+  }
+#undef PX
+#undef PY
+#undef VAL
+}
+
+// waves per block at 32 layers (eight measured 1% slower in FP64 and 6%
+// slower in complex64 at 256x256x32)
+template <typename R> constexpr int C1_W32 = 4;
+// Automatic use of the small-block kernel for one vector (contraction alone
+// at 128x128, Nz = 8 to 32): FP64 at every Nz (3-30% faster than the 16x16x4
+// tiles, except 1% slower at Nz = 28); complex64 from Nz = 24 (3-9% faster;
+// below 17 layers the tensor stays in the 256 MB last-level cache and the
+// 16x16x4 tiles were 10-20% faster, and at Nz = 17-23 the vector-ALU kernel
+// stays faster)
+template <typename R> bool c1_auto(int nz) { return nz >= (sizeof(R) == 8 ? 8 : 24) && nz <= 32; }
+// The small-block kernel for Nz from 8 to 32 (padded to 16 or 32 layers).
+// False for other Nz.
+template <typename R>
+static bool launch_contract1(const Operator& op, const cx_t<R>* G, cx_t<R>* S, const Geo& g, hipStream_t s) {
+  if (op.Nz < C3_MIN_NZ || op.Nz > 32) return false;
+  auto go = [&](auto kern, int sm, int nw) {
+    if (sm > 65536)
+      G2G_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    hipLaunchKernelGGL(kern, dim3(op.Nx * op.Ny), dim3(64 * nw), sm, s, S, G, g, op.BS);
+  };
+  constexpr int W32 = C1_W32<R>;
+  if (op.Nz == 16) go(k_contract1<R, 16, false, 4>, C1Cfg<R, 16, 4>::smem, 4);
+  else if (op.Nz == 32) go(k_contract1<R, 32, false, W32>, C1Cfg<R, 32, W32>::smem, W32);
+  else if (op.Nz < 16) go(k_contract1<R, 16, true, 4>, C1Cfg<R, 16, 4>::smem, 4);
+  else go(k_contract1<R, 32, true, W32>, C1Cfg<R, 32, W32>::smem, W32);
+  return true;
+}
+
 // Automatic kernel for one vector: the matrix cores, except complex64 at
 // padded layer counts where the vector-ALU kernel measured faster (128x128,
 // contraction alone: Nz = 20 by 19%, Nz = 50 and 56 by 5-8%; the padded tiles
@@ -1559,8 +1871,10 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
     // matrix cores for Nz from 8 to 64 (complex64: where c3_auto says so)
     // unless a vector-ALU kernel is requested (FP64 at 128x128x32:
     // 0.28-0.30 ms against 0.33 ms)
-    if ((op.contraction == 2 || (op.contraction == 0 && c3_auto<R>(op.Nz))) &&
-        launch_contract3<R, 1>(op, b.G, b.S, 0, g, s)) {
+    if ((op.contraction == 4 || (op.contraction == 0 && c1_auto<R>(op.Nz))) &&
+        launch_contract1<R>(op, b.G, b.S, g, s)) {
+    } else if ((op.contraction == 2 || (op.contraction == 0 && c3_auto<R>(op.Nz))) &&
+               launch_contract3<R, 1>(op, b.G, b.S, 0, g, s)) {
     } else if (op.contraction == 3) {
       if (op.Nz * op.Nz <= 4 * NT) launch(k_contract<R, 4>);
       else launch(k_contract<R, 16>);
@@ -1644,7 +1958,10 @@ static void op_apply_k(Operator& op, OpBuffers<R>& b, int k, const cx_t<R>* cons
   if (prof) pe[1] = op.prof.record(s);
   bool ok = false;
   switch (kk) {
-    case 1: ok = launch_contract3<R, 1>(op, b.G, b.SB, nS, g, s); break;
+    case 1:
+      ok = (op.contraction == 0 && c1_auto<R>(op.Nz)) ? launch_contract1<R>(op, b.G, b.SB, g, s)
+                                                        : launch_contract3<R, 1>(op, b.G, b.SB, nS, g, s);
+      break;
     case 2: ok = launch_contract3<R, 2>(op, b.G, b.SB, nS, g, s); break;
     case 4: ok = launch_contract3<R, 4>(op, b.G, b.SB, nS, g, s); break;
     default:

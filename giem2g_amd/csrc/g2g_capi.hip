@@ -804,9 +804,11 @@ int giem2g_last_history(void* p, double* out, int maxlen, int* len) {
 int giem2g_op_set_contraction(void* p, int mode) {
   return guard([&] {
     Handle* h = H(p);
-    G2G_CHECK(mode >= 0 && mode <= 3, -3, "contraction mode must be 0, 1, 2 or 3");
+    G2G_CHECK(mode >= 0 && mode <= 4, -3, "contraction mode must be 0, 1, 2, 3 or 4");
     G2G_CHECK(mode != 2 || op_pair_supported(h->op), -3,
               "the matrix-core contraction needs Nz from 8 to 64");
+    G2G_CHECK(mode != 4 || (h->op.Nz >= 8 && h->op.Nz <= 32), -3,
+              "the small-block matrix-core contraction needs Nz from 8 to 32");
     h->op.contraction = mode;
   });
 }

@@ -179,7 +179,7 @@ struct Operator {
   OpBuffers<double> d64;
   OpBuffers<float> d32;
   hipStream_t stream = nullptr;
-  int contraction = 0;  // 0 automatic (2 where it exists, else 1), 1 vector ALU two-deep, 2 matrix cores, 3 vector ALU one-deep
+  int contraction = 0;  // 0 automatic (2 where it exists, else 1), 1 vector ALU two-deep, 2 matrix cores, 3 vector ALU one-deep, 4 small-block matrix cores
   Counters cnt;
   // tensor-build scratch (frequency independent weights are cached)
   double* wts = nullptr;  // 6 * 501 * Nx*Ny masked, lambda-scaled weights

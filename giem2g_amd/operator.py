@@ -286,10 +286,12 @@ class IEOperator:
         L.check(self.lib.giem2g_op_scale(self.h, mode, a, b))
 
     def set_contraction(self, mode):
-        """0 automatic (2 for Nz from 8 to 64, otherwise 1; complex64 at
+        """0 automatic (4 for FP64 at Nz from 8 to 32 and complex64 at Nz
+        from 24 to 32, else 2 for Nz from 8 to 64, otherwise 1; complex64 at
         Nz = 17-23, 33-35, 49-63 uses 1), 1 vector ALU
         (two components in flight), 2 matrix cores (MFMA), 3 vector ALU (one
-        component in flight)."""
+        component in flight), 4 small-block matrix cores (Nz from 8 to 32:
+        every operand column a point)."""
         L.check(self.lib.giem2g_op_set_contraction(self.h, mode))
 
     def set_learned(self, net, accept=0.9):

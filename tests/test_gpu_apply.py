@@ -81,6 +81,44 @@ def test_matrix_core_contraction_matches_oracle(oracle, Nx, Ny, Nz):
     op.close()
 
 
+@pytest.mark.parametrize("Nx,Ny,Nz", [(16, 16, 16), (8, 16, 32), (8, 8, 8), (8, 8, 12), (16, 8, 24),
+                                      (8, 8, 31), (10, 14, 20), (12, 8, 9), (8, 12, 17)])
+def test_small_block_contraction_matches_oracle(oracle, Nx, Ny, Nz):
+    """The one-vector small-block matrix-core contraction (mode 4: Nz 16 and
+    32, and 8 to 31 zero-padded to 16 or 32; 10x14 has a side that is not a
+    power of two) against the oracle, FP64 at 1e-12 and complex64 within
+    2e-6, and against the 16x16x4 tiles (mode 2) in FP64 at 1e-14.  Nz
+    beyond 32 is refused."""
+    torch = _torch()
+    from giem2g_amd import lib as L, models, operator as gop
+    print("case", Nx, Ny, Nz, flush=True)
+    m = models.small(Nx, Ny, Nz, seed=Nx + 5 * Nz, freq=1.5, dz=20.0)
+    om = oracle_model(oracle, m)
+    Gs = om.tensor_fft(om.tensor_realspace())
+    v = rand_vec(3 * m.ncell, 14)
+    ref = om.apply(Gs, m.csiga(), v)
+    op = gop.IEOperator.from_model(m)
+    op.set_background(m.csigb(), m.thick, m.omega)
+    op.put_tensor(Gs, spectral=True)
+    op.set_sigma(m.csiga())
+    op.set_contraction(2)
+    r2 = op.apply_host(v)
+    op.set_contraction(4)
+    r4 = op.apply_host(v)
+    assert rel(r4, ref) < 1e-12
+    assert rel(r4, r2) < 1e-14
+    vt = torch.from_numpy(v).cuda().to(torch.complex64)
+    o32 = torch.empty_like(vt)
+    op.apply_dev_f32(vt, o32)
+    torch.cuda.synchronize()
+    assert rel(o32.to(torch.complex128).cpu().numpy(), ref) < 2e-6
+    op.close()
+    big = gop.IEOperator.from_model(models.small(8, 8, 33, seed=1, freq=1.5, dz=20.0))
+    with pytest.raises(L.G2GError, match="8 to 32"):
+        big.set_contraction(4)
+    big.close()
+
+
 @pytest.mark.parametrize("Nx,Ny,Nz", [(16, 8, 16), (8, 16, 32), (8, 8, 48), (8, 8, 64),
                                       (8, 8, 12), (8, 16, 50), (16, 8, 9)])
 def test_paired_apply_matches_oracle(oracle, Nx, Ny, Nz):
