@@ -1439,9 +1439,10 @@ static bool launch_contract3(const Operator& op, const cx_t<R>* G, cx_t<R>* S, i
 // read without bank conflicts.  Waves split the source layers; partial sums
 // are added through LDS in a fixed order (results do not depend on
 // scheduling).
-template <typename R, int NZ, int NW_ = 4> struct C1Cfg {
+template <typename R, int NZ, int NW_, bool MIR> struct C1Cfg {
   static constexpr bool F32 = sizeof(R) == 4;
   static constexpr int NW = NW_, NTH = 64 * NW, JW = NZ / NW;
+  static constexpr int NSET = MIR ? 2 : 1;  // point sets per block: the quad and its mirror
   static constexpr int P = NZ * (NZ + 1) / 2;
   static constexpr int ES = (int)sizeof(cx_t<R>);
   static constexpr int SWM = (F32 && NZ >= 32) ? 31 : 15;
@@ -1449,10 +1450,13 @@ template <typename R, int NZ, int NW_ = 4> struct C1Cfg {
   static constexpr int G = 64 / NZ;
   // FP64: one instruction = sixteen output rows (four blocks of four)
   static constexpr int NI = F32 ? 3 : 3 * NZ / 16;
-  static constexpr int VIN_B = 3 * NZ * 4 * ES;                  // [ci][z][point]
+  static constexpr int VS = 3 * NZ * 4;                           // point-row values per set
+  static constexpr int VIN_B = NSET * VS * ES;                    // [set][ci][z][point]
   static constexpr int SYM_B = (4 * P * ES + 1023) / 1024 * 1024;  // whole 1 KB copies
   static constexpr int FULL_B = 2 * NZ * NZ * ES;
-  static constexpr int RED_B = NW * NI * 2 * (F32 ? 256 : 64) * (int)sizeof(R);
+  // partial sums per wave: FP32 after the G layer offsets are added across
+  // lanes (4 NZ values per output component and part), FP64 64 per tile row set
+  static constexpr int RED_B = NSET * NW * NI * 2 * (F32 ? 4 * NZ : 64) * (int)sizeof(R);
   static constexpr int smem = VIN_B + (SYM_B + FULL_B > RED_B ? SYM_B + FULL_B : RED_B);
 };
 
@@ -1469,13 +1473,29 @@ template <int N, typename F> __device__ __forceinline__ void static_for(F& f) {
   static_for_seq(f, std::make_integer_sequence<int, N>{});
 }
 
-template <typename R, int NZ, bool PAD, int NW_>
-__global__ __launch_bounds__(64 * NW_) void k_contract1(cx_t<R>* __restrict__ S, const cx_t<R>* __restrict__ G, Geo g,
+// MIR: the x-y mirror of a square lateral grid with dx = dy.  Swapping the x
+// and y axes maps the layered-earth tensor onto itself, so the block of
+// (ky, kx) is the block of (kx, ky) with x and y exchanged:
+//   G0(ky, kx) = P G0(kx, ky) P,  P = the x <-> y component swap.
+// A block then serves the quad (kx, ky) and the mirrored quad (ky, kx)
+// (kx < ky; blocks cover kx <= ky) from one tensor read.  With
+// P D_p P = D_q (q: p with its two sign bits exchanged), point q of the
+// mirrored set is the point whose row sits at the transposed coordinates
+// (PY(q), PX(q)); its input is read x <-> y exchanged and its output stored
+// the same way, and both take the signs of point q:
+//   out'(q) = P D_q G0 D_q P v'(q).
+// waves per SIMD asked of the compiler: four for the mirrored FP32 kernels,
+// whose 39 KB of LDS at Nz = 32 fit four blocks per CU (their registers
+// otherwise allowed three)
+template <typename R, bool MIR> constexpr int c1_waves() { return (sizeof(R) == 4 && MIR) ? 4 : 1; }
+template <typename R, int NZ, bool PAD, int NW_, bool MIR>
+__global__ __launch_bounds__(64 * NW_) __attribute__((amdgpu_waves_per_eu(c1_waves<R, MIR>()))) void k_contract1(cx_t<R>* __restrict__ S, const cx_t<R>* __restrict__ G, Geo g,
                                                         int64_t BS) {
   using T2 = cx_t<R>;
-  using C = C1Cfg<R, NZ, NW_>;
+  using C = C1Cfg<R, NZ, NW_, MIR>;
   constexpr bool F32 = C::F32;
-  constexpr int NW = C::NW, NTH = C::NTH, JW = C::JW, ES = C::ES, SWM = C::SWM, NI = C::NI;
+  constexpr int NW = C::NW, NTH = C::NTH, JW = C::JW, ES = C::ES, SWM = C::SWM, NI = C::NI, NSET = C::NSET;
+  constexpr int VS = C::VS;
   static_assert(NZ == 16 || NZ == 32, "16 or 32 layers (fewer: zero-padded)");
   static_assert(JW % (F32 ? C::G : 4) == 0, "whole source-layer groups per wave");
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1487,143 +1507,231 @@ __global__ __launch_bounds__(64 * NW_) void k_contract1(cx_t<R>* __restrict__ S,
   const int Pn = PAD ? nz * (nz + 1) / 2 : C::P;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int kx = blockIdx.x / g.Ny, ky = blockIdx.x % g.Ny, b = kx + g.Nx * ky;
+  int kx, ky;
+  if constexpr (MIR) {
+    // block t -> (kx, ky) with kx <= ky, t = ky (ky + 1) / 2 + kx
+    const int t = blockIdx.x;
+    int y = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+    while ((y + 1) * (y + 2) / 2 <= t) y++;
+    while (y * (y + 1) / 2 > t) y--;
+    ky = y;
+    kx = t - y * (y + 1) / 2;
+  } else {
+    // blocks walk ky fastest: concurrent blocks touch neighbouring spectrum rows
+    kx = blockIdx.x / g.Ny;
+    ky = blockIdx.x % g.Ny;
+  }
+  const int b = kx + g.Nx * ky;
+  const bool two = MIR && kx != ky;  // the diagonal maps onto itself: one set
   const int Nx2 = 2 * g.Nx, Ny2 = 2 * g.Ny, gNx = g.Nx, gNy = g.Ny;
 #define PX(p) (((p) & 1) ? (kx == 0 ? gNx : Nx2 - kx) : kx)
 #define PY(p) (((p) & 2) ? (ky == 0 ? gNy : Ny2 - ky) : ky)
 #define VAL(p) ((!((p) & 1) || kx > 0) && (!((p) & 2) || ky > 0))
+  // spectrum row of point q of set st (the mirrored set at transposed coordinates)
+#define ROW(st, q) ((st) == 0 ? (int64_t)PX(q) * Ny2 + PY(q) : (int64_t)PY(q) * Ny2 + PX(q))
   const T2* blk = G + (int64_t)b * BS;
   const T2 zero = mk<T2>(0, 0);
-  // 1. the four point rows into registers (issued first: waiting for them
-  //    does not wait for the tensor copies issued after them)
-  constexpr int NV = 4 * 3 * NZ, NVQ = (NV + NTH - 1) / NTH;
-  T2 vq[NVQ];
+  // 1. the point rows into registers (issued first: waiting for them does
+  //    not wait for the tensor copies issued after them).  Wave w takes the
+  //    rows w*RPW .. w*RPW + RPW - 1 (row = 4 set + point), so a row's base
+  //    and validity are wave-uniform; lane l takes elements l and l + 64 of
+  //    its 3 NZ (component-major)
+  constexpr int NROW = NSET * 4, RPW = NROW / NW, EPL = (3 * NZ + 63) / 64;
+  static_assert(NROW % NW == 0, "whole point rows per wave");
+  T2 vq[RPW][EPL];
 #pragma unroll
-  for (int k = 0; k < NVQ; k++) {
-    const int i = tid + k * NTH;
-    const int p = i / (3 * NZ), e = i - p * (3 * NZ), ci = e / NZ, z = e - ci * NZ;
-    vq[k] = zero;
-    if (i < NV && VAL(p) && z < nz) vq[k] = S[((int64_t)PX(p) * Ny2 + PY(p)) * nz3 + ci * nz + z];
+  for (int rr = 0; rr < RPW; rr++) {
+    const int rho = wave * RPW + rr, st = rho >> 2, q = rho & 3;
+    const bool live = (st == 0 || two) && VAL(q);
+    const T2* row = S + ROW(st, q) * nz3;
+#pragma unroll
+    for (int k = 0; k < EPL; k++) {
+      const int e = lane + 64 * k, ci = e / NZ, z = e - ci * NZ;
+      const int cs = (st == 0 || ci == 2) ? ci : 1 - ci;  // mirrored set: x <-> y
+      vq[rr][k] = zero;
+      if (live && e < 3 * NZ && z < nz) vq[rr][k] = row[cs * nz + z];
+    }
   }
   // 2. the tensor block into LDS
   {
     const unsigned char* gb = reinterpret_cast<const unsigned char*>(blk);
-    const int ngr = 4 * Pn * ES / 16;  // 16-byte pieces of the packed components (4 Pn is even)
-    const int nts = (ngr + 63) / 64;
-    for (int t = wave; t < nts; t += NW) {
-      const int gi = min(t * 64 + lane, ngr - 1);
-      __builtin_amdgcn_global_load_lds((const void*)(gb + 16 * (int64_t)gi), G2G_LDS(smem_raw + C::VIN_B + t * 1024),
-                                       16, 0, C1_NT);
+    const unsigned ngr = 4 * Pn * ES / 16;  // 16-byte pieces of the packed components (4 Pn is even)
+    const unsigned nts = (ngr + 63) / 64;
+    for (unsigned t = wave; t < nts; t += NW) {
+      const unsigned gi = min(t * 64 + lane, ngr - 1);
+      __builtin_amdgcn_global_load_lds((const void*)(gb + 16 * gi), G2G_LDS(smem_raw + C::VIN_B + t * 1024), 16, 0,
+                                       C1_NT);
     }
-    const T2* fsrc = blk + 4 * Pn;
-    constexpr int PER = F32 ? 32 : 64;  // complex values per copy instruction
-    constexpr int NTF = 2 * NZ * NZ / PER;
-    for (int t = wave; t < NTF; t += NW) {
-      const int pos = t * PER + (F32 ? lane >> 1 : lane);
-      const int comp = pos / (NZ * NZ), loc = pos - comp * (NZ * NZ);
-      const int col = loc / NZ, row = (loc - col * NZ) ^ (col & SWM);
+    // full components, swizzled: copy t fills positions t PER .. t PER + PER - 1
+    // (component-major, then column-major); unsigned offsets from the
+    // component base keep the per-lane work to a few operations
+    const unsigned char* fb = reinterpret_cast<const unsigned char*>(blk + 4 * Pn);
+    constexpr unsigned PER = F32 ? 32 : 64;  // complex values per copy instruction
+    constexpr unsigned NTF = 2 * NZ * NZ / PER;
+    const unsigned lp = F32 ? (unsigned)lane >> 1 : (unsigned)lane;
+    const unsigned part = F32 ? 4u * (lane & 1) : 0u;
+    for (unsigned t = wave; t < NTF; t += NW) {
+      const unsigned pos = t * PER + lp;
+      const unsigned comp = pos / (NZ * NZ), loc = pos % (NZ * NZ);
+      const unsigned col = loc / NZ, row = (loc % NZ) ^ (col & SWM);
       // padded rows and columns copy a real element (finite; its products
       // meet zero operands or land in rows that are not stored)
-      const int rc = PAD ? min(row, nz - 1) : row, cc = PAD ? min(col, nz - 1) : col;
-      const T2* src = fsrc + comp * nz * nz + cc * nz + rc;
+      const unsigned rc = PAD ? min(row, (unsigned)nz - 1) : row, cc = PAD ? min(col, (unsigned)nz - 1) : col;
+      const unsigned off = ((comp * nz + cc) * nz + rc) * ES + part;
       unsigned char* dst = smem_raw + C::VIN_B + C::SYM_B + t * (PER * ES);
       if constexpr (F32)
-        __builtin_amdgcn_global_load_lds((const void*)(reinterpret_cast<const float*>(src) + (lane & 1)), G2G_LDS(dst),
-                                         4, 0, C1_NT);
+        __builtin_amdgcn_global_load_lds((const void*)(fb + off), G2G_LDS(dst), 4, 0, C1_NT);
       else
-        __builtin_amdgcn_global_load_lds((const void*)src, G2G_LDS(dst), 16, 0, C1_NT);
+        __builtin_amdgcn_global_load_lds((const void*)(fb + off), G2G_LDS(dst), 16, 0, C1_NT);
     }
   }
-  // 3. point rows with the point signs (x times -1 for -kx, y for -ky),
-  //    layer-major with the four points innermost
+  // 3. point rows with the point signs (x times -1 for -kx, y for -ky; 0/1
+  //    flags, no branch), layer-major with the four points innermost
 #pragma unroll
-  for (int k = 0; k < NVQ; k++) {
-    const int i = tid + k * NTH;
-    const int p = i / (3 * NZ), e = i - p * (3 * NZ), ci = e / NZ, z = e - ci * NZ;
-    T2 v = vq[k];
-    if ((ci == 0 && (p & 1)) || (ci == 1 && (p & 2))) v = mk<T2>(-v.x, -v.y);
-    if (i < NV) vin[(ci * NZ + z) * 4 + p] = v;
+  for (int rr = 0; rr < RPW; rr++) {
+    const int rho = wave * RPW + rr, st = rho >> 2, q = rho & 3;
+#pragma unroll
+    for (int k = 0; k < EPL; k++) {
+      const int e = lane + 64 * k, ci = e / NZ, z = e - ci * NZ;
+      const int flip = ((ci == 0) & (q & 1)) | ((ci == 1) & ((q >> 1) & 1));
+      const R sg = R(1 - 2 * flip);
+      if (e < 3 * NZ) vin[st * VS + (ci * NZ + z) * 4 + q] = mk<T2>(sg * vq[rr][k].x, sg * vq[rr][k].y);
+    }
   }
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   // 4. products.  Tensor element (r, j) of output component co and input ci:
   //    ci = x: Sxx, Sxy, -Axz^T;  ci = y: Sxy, Syy, -Ayz^T;  ci = z: Axz, Ayz, Szz
-  auto fetch = [&](auto CI, auto CO, int r, int j, int tr, int tj) -> T2 {
+  //    A lane's offsets of element (r, j) in a packed component (ps), in a
+  //    full one read down a column (pd) and along a row (pt) do not depend
+  //    on the component: computed once per source layer, the fetches then
+  //    add compile-time component offsets
+  struct Off { int ps, pd, pt; bool in; };
+  auto offsets = [&](int r, int j) -> Off {
+    Off o;
+    o.ps = j <= r ? j + r * (r + 1) / 2 : r + j * (j + 1) / 2;
+    o.pd = j * NZ + (r ^ (j & SWM));
+    o.pt = r * NZ + (j ^ (r & SWM));
+    o.in = !PAD || j < nz;
+    return o;
+  };
+  auto fetch = [&](auto CI, auto CO, const Off& o) -> T2 {
     constexpr int ci = decltype(CI)::value, co = decltype(CO)::value;
     if constexpr (ci == 2 && co < 2) {
-      return full[co * NZ * NZ + j * NZ + (r ^ (j & SWM))];
+      return full[co * NZ * NZ + o.pd];
     } else if constexpr (ci < 2 && co == 2) {
-      const T2 a = full[ci * NZ * NZ + r * NZ + (j ^ (r & SWM))];
+      const T2 a = full[ci * NZ * NZ + o.pt];
       return mk<T2>(-a.x, -a.y);
     } else {
       constexpr int comp = ci == 2 ? 3 : (ci != co ? 1 : (ci == 0 ? 0 : 2));
-      T2 a = sym[comp * Pn + (j <= r ? j + tr : r + tj)];
-      if (PAD && j >= nz) a = zero;  // beyond the packed rows: not stored
+      T2 a = sym[comp * Pn + o.ps];
+      if (PAD && !o.in) a = zero;  // beyond the packed rows: not stored
       return a;
     }
   };
   using std::integral_constant;
   if constexpr (F32) {
     using V4 = v4t<float>::t;
-    V4 are[3], aim[3];
+    V4 are[NSET][3], aim[NSET][3];
 #pragma unroll
-    for (int co = 0; co < 3; co++) are[co] = aim[co] = V4{0, 0, 0, 0};
+    for (int st = 0; st < NSET; st++)
+#pragma unroll
+      for (int co = 0; co < 3; co++) are[st][co] = aim[st][co] = V4{0, 0, 0, 0};
     // lane: A row r, source-layer offset gl, point column pc
-    const int r = lane % NZ, gl = lane / NZ, pc = lane & 3, tr = r * (r + 1) / 2;
+    const int r = lane % NZ, gl = lane / NZ, pc = lane & 3;
+    constexpr int NJG = JW / C::G;
+    Off off[NJG];
+#pragma unroll
+    for (int jg = 0; jg < NJG; jg++) off[jg] = offsets(r, wave * JW + jg * C::G + gl);
     auto ci_pass = [&](auto CI) {
       constexpr int ci = decltype(CI)::value;
 #pragma unroll
-      for (int jg = 0; jg < JW / C::G; jg++) {
-        const int j = wave * JW + jg * C::G + gl, tj = j * (j + 1) / 2;
-        const T2 bv = vin[(ci * NZ + j) * 4 + pc];
-        const float nbi = -bv.y;
-        const T2 a0 = fetch(CI, integral_constant<int, 0>{}, r, j, tr, tj);
-        const T2 a1 = fetch(CI, integral_constant<int, 1>{}, r, j, tr, tj);
-        const T2 a2 = fetch(CI, integral_constant<int, 2>{}, r, j, tr, tj);
-        are[0] = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.x, bv.x, are[0], 0, 0, 0);
-        are[1] = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.x, bv.x, are[1], 0, 0, 0);
-        are[2] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.x, bv.x, are[2], 0, 0, 0);
-        aim[0] = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.x, bv.y, aim[0], 0, 0, 0);
-        aim[1] = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.x, bv.y, aim[1], 0, 0, 0);
-        aim[2] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.x, bv.y, aim[2], 0, 0, 0);
-        are[0] = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.y, nbi, are[0], 0, 0, 0);
-        are[1] = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.y, nbi, are[1], 0, 0, 0);
-        are[2] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.y, nbi, are[2], 0, 0, 0);
-        aim[0] = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.y, bv.x, aim[0], 0, 0, 0);
-        aim[1] = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.y, bv.x, aim[1], 0, 0, 0);
-        aim[2] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.y, bv.x, aim[2], 0, 0, 0);
+      for (int jg = 0; jg < NJG; jg++) {
+        const int j = wave * JW + jg * C::G + gl;
+        const T2 a0 = fetch(CI, integral_constant<int, 0>{}, off[jg]);
+        const T2 a1 = fetch(CI, integral_constant<int, 1>{}, off[jg]);
+        const T2 a2 = fetch(CI, integral_constant<int, 2>{}, off[jg]);
+#pragma unroll
+        for (int st = 0; st < NSET; st++) {
+          const T2 bv = vin[st * VS + (ci * NZ + j) * 4 + pc];
+          const float nbi = -bv.y;
+          are[st][0] = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.x, bv.x, are[st][0], 0, 0, 0);
+          are[st][1] = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.x, bv.x, are[st][1], 0, 0, 0);
+          are[st][2] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.x, bv.x, are[st][2], 0, 0, 0);
+          aim[st][0] = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.x, bv.y, aim[st][0], 0, 0, 0);
+          aim[st][1] = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.x, bv.y, aim[st][1], 0, 0, 0);
+          aim[st][2] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.x, bv.y, aim[st][2], 0, 0, 0);
+          are[st][0] = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.y, nbi, are[st][0], 0, 0, 0);
+          are[st][1] = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.y, nbi, are[st][1], 0, 0, 0);
+          are[st][2] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.y, nbi, are[st][2], 0, 0, 0);
+          aim[st][0] = __builtin_amdgcn_mfma_f32_4x4x1f32(a0.y, bv.x, aim[st][0], 0, 0, 0);
+          aim[st][1] = __builtin_amdgcn_mfma_f32_4x4x1f32(a1.y, bv.x, aim[st][1], 0, 0, 0);
+          aim[st][2] = __builtin_amdgcn_mfma_f32_4x4x1f32(a2.y, bv.x, aim[st][2], 0, 0, 0);
+        }
       }
     };
     ci_pass(integral_constant<int, 0>{});
     ci_pass(integral_constant<int, 1>{});
     ci_pass(integral_constant<int, 2>{});
-    __syncthreads();
+    // the G layer offsets of a row block sit in lanes NZ apart: add them
+    // across lanes (offsets 0+1, then +2+3 at NZ = 16), lanes below NZ keep
+    // the sums
 #pragma unroll
-    for (int co = 0; co < 3; co++) {
-      *reinterpret_cast<V4*>(red + ((wave * 3 + co) * 2 + 0) * 256 + lane * 4) = are[co];
-      *reinterpret_cast<V4*>(red + ((wave * 3 + co) * 2 + 1) * 256 + lane * 4) = aim[co];
+    for (int st = 0; st < NSET; st++)
+#pragma unroll
+      for (int co = 0; co < 3; co++)
+#pragma unroll
+        for (int o = NZ; o < 64; o <<= 1)
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            are[st][co][q] += __shfl_xor(are[st][co][q], o);
+            aim[st][co][q] += __shfl_xor(aim[st][co][q], o);
+          }
+    __syncthreads();
+    if (lane < NZ) {
+#pragma unroll
+      for (int st = 0; st < NSET; st++)
+#pragma unroll
+        for (int co = 0; co < 3; co++) {
+          R* dst = red + (((st * NW + wave) * 3 + co) * 2) * (4 * NZ) + lane * 4;
+          *reinterpret_cast<V4*>(dst) = are[st][co];
+          *reinterpret_cast<V4*>(dst + 4 * NZ) = aim[st][co];
+        }
     }
   } else {
-    double are[NI], aim[NI];
+    double are[NSET][NI], aim[NSET][NI];
 #pragma unroll
-    for (int I = 0; I < NI; I++) are[I] = aim[I] = 0.0;
+    for (int st = 0; st < NSET; st++)
+#pragma unroll
+      for (int I = 0; I < NI; I++) are[st][I] = aim[st][I] = 0.0;
     // lane 16k + 4b + m: source layer k of the quad, block b, A row / B column m
     const int kq = lane >> 4, bq = (lane >> 2) & 3, m = lane & 3;
+    // element offsets per (source-layer quad, row set): the row set of
+    // instruction I depends on I only through (4 I) mod (NZ / 4)
+    constexpr int NJQ = JW / 4, NRS = (NZ / 4) / 4;
+    Off off[NJQ][NRS];
+#pragma unroll
+    for (int jq = 0; jq < NJQ; jq++)
+#pragma unroll
+      for (int rs = 0; rs < NRS; rs++) off[jq][rs] = offsets(4 * (4 * rs + bq) + m, wave * JW + 4 * jq + kq);
     auto ci_pass = [&](auto CI) {
       constexpr int ci = decltype(CI)::value;
 #pragma unroll
-      for (int jq = 0; jq < JW / 4; jq++) {
-        const int j = wave * JW + 4 * jq + kq, tj = j * (j + 1) / 2;
-        const T2 bv = vin[(ci * NZ + j) * 4 + m];
-        const double nbi = -bv.y;
+      for (int jq = 0; jq < NJQ; jq++) {
+        const int j = wave * JW + 4 * jq + kq;
+        T2 bv[NSET];
+#pragma unroll
+        for (int st = 0; st < NSET; st++) bv[st] = vin[st * VS + (ci * NZ + j) * 4 + m];
         auto rows = [&](auto II) {
-          constexpr int I = decltype(II)::value, co = (4 * I) / (NZ / 4);
-          const int r = 4 * ((4 * I + bq) % (NZ / 4)) + m;
-          const T2 a = fetch(CI, integral_constant<int, co>{}, r, j, r * (r + 1) / 2, tj);
-          are[I] = __builtin_amdgcn_mfma_f64_4x4x4f64(a.x, bv.x, are[I], 0, 0, 0);
-          aim[I] = __builtin_amdgcn_mfma_f64_4x4x4f64(a.x, bv.y, aim[I], 0, 0, 0);
-          are[I] = __builtin_amdgcn_mfma_f64_4x4x4f64(a.y, nbi, are[I], 0, 0, 0);
-          aim[I] = __builtin_amdgcn_mfma_f64_4x4x4f64(a.y, bv.x, aim[I], 0, 0, 0);
+          constexpr int I = decltype(II)::value, co = (4 * I) / (NZ / 4), rs = I % NRS;
+          const T2 a = fetch(CI, integral_constant<int, co>{}, off[jq][rs]);
+#pragma unroll
+          for (int st = 0; st < NSET; st++) {
+            are[st][I] = __builtin_amdgcn_mfma_f64_4x4x4f64(a.x, bv[st].x, are[st][I], 0, 0, 0);
+            aim[st][I] = __builtin_amdgcn_mfma_f64_4x4x4f64(a.x, bv[st].y, aim[st][I], 0, 0, 0);
+            are[st][I] = __builtin_amdgcn_mfma_f64_4x4x4f64(a.y, -bv[st].y, are[st][I], 0, 0, 0);
+            aim[st][I] = __builtin_amdgcn_mfma_f64_4x4x4f64(a.y, bv[st].x, aim[st][I], 0, 0, 0);
+          }
         };
         static_for<NI>(rows);
       }
@@ -1633,45 +1741,56 @@ __global__ __launch_bounds__(64 * NW_) void k_contract1(cx_t<R>* __restrict__ S,
     ci_pass(integral_constant<int, 2>{});
     __syncthreads();
 #pragma unroll
-    for (int I = 0; I < NI; I++) {
-      red[((wave * NI + I) * 2 + 0) * 64 + lane] = are[I];
-      red[((wave * NI + I) * 2 + 1) * 64 + lane] = aim[I];
-    }
+    for (int st = 0; st < NSET; st++)
+#pragma unroll
+      for (int I = 0; I < NI; I++) {
+        red[(((st * NW + wave) * NI + I) * 2 + 0) * 64 + lane] = are[st][I];
+        red[(((st * NW + wave) * NI + I) * 2 + 1) * 64 + lane] = aim[st][I];
+      }
   }
   __syncthreads();
-  // 5. sums over waves (and FP32 layer offsets) in a fixed order, output signs, store
-  for (int e = tid; e < 4 * nz3; e += NTH) {
-    const int p = e / nz3, f = e - p * nz3, co = f / nz, z = f - co * nz;
-    R sr = 0, si = 0;
-    if constexpr (F32) {
+  // 5. sums over waves in a fixed order, output signs, store: the rows as
+  //    in step 1 (wave-uniform row, lane-strided elements)
 #pragma unroll
-      for (int w = 0; w < NW; w++)
+  for (int rr = 0; rr < RPW; rr++) {
+    const int rho = wave * RPW + rr, st = rho >> 2, q = rho & 3;
+    if (st == 1 && !two) continue;
+    const bool live = VAL(q);
+    T2* row = S + ROW(st, q) * nz3;
 #pragma unroll
-        for (int gg = 0; gg < C::G; gg++) {
-          const int at = ((w * 3 + co) * 2) * 256 + (4 * (gg * (NZ / 4) + (z >> 2)) + p) * 4 + (z & 3);
+    for (int k = 0; k < EPL; k++) {
+      const int e = lane + 64 * k, co = e / NZ, z = e - co * NZ;
+      if (e >= 3 * NZ || z >= nz) continue;
+      R sr = 0, si = 0;
+      if constexpr (F32) {
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+          const int at = (((st * NW + w) * 3 + co) * 2) * (4 * NZ) + (4 * (z >> 2) + q) * 4 + (z & 3);
           sr += red[at];
-          si += red[at + 256];
+          si += red[at + 4 * NZ];
         }
-    } else {
-      const int rb = co * (NZ / 4) + (z >> 2), I = rb >> 2, bq = rb & 3;
+      } else {
+        const int rb = co * (NZ / 4) + (z >> 2), I = rb >> 2, bq = rb & 3;
 #pragma unroll
-      for (int w = 0; w < NW; w++) {
-        const int at = ((w * NI + I) * 2) * 64 + 16 * (z & 3) + 4 * bq + p;
-        sr += red[at];
-        si += red[at + 64];
+        for (int w = 0; w < NW; w++) {
+          const int at = (((st * NW + w) * NI + I) * 2) * 64 + 16 * (z & 3) + 4 * bq + q;
+          sr += red[at];
+          si += red[at + 64];
+        }
       }
+      // output sign as arithmetic on 0/1 flags: the short-circuit form
+      // (co == 0 && p & 1) || (co == 1 && p & 2) selecting -sum compiled, for
+      // FP32 with this toolchain, to code that kept +sum for co = 1, p & 2
+      const int flip = ((co == 0) & (q & 1)) | ((co == 1) & ((q >> 1) & 1));
+      const R so = R(1 - 2 * flip);
+      const int cs = (st == 0 || co == 2) ? co : 1 - co;  // mirrored set: x <-> y
+      row[cs * nz + z] = live ? mk<T2>(so * sr, so * si) : zero;
     }
-    // output sign as arithmetic on 0/1 flags: the short-circuit form
-    // (co == 0 && p & 1) || (co == 1 && p & 2) selecting -sum compiled, for
-    // FP32 with this toolchain, to code that kept +sum for co = 1, p & 2
-    const int flip = ((co == 0) & (p & 1)) | ((co == 1) & ((p >> 1) & 1));
-    const R so = R(1 - 2 * flip);
-    const T2 o = VAL(p) ? mk<T2>(so * sr, so * si) : zero;
-    S[((int64_t)PX(p) * Ny2 + PY(p)) * nz3 + f] = o;
   }
 #undef PX
 #undef PY
 #undef VAL
+#undef ROW
 }
 
 // waves per block at 32 layers (eight measured 1% slower in FP64 and 6%
@@ -1684,24 +1803,31 @@ template <typename R> constexpr int C1_W32 = 4;
 // 16x16x4 tiles were 10-20% faster, and at Nz = 17-23 the vector-ALU kernel
 // stays faster)
 template <typename R> bool c1_auto(int nz) { return nz >= (sizeof(R) == 8 ? 8 : 24) && nz <= 32; }
-// The small-block kernel for Nz from 8 to 32 (padded to 16 or 32 layers).
-// False for other Nz.
+// The small-block kernel for Nz from 8 to 32 (padded to 16 or 32 layers),
+// with the x-y mirror when mir is set (square grid, dx = dy, the library's
+// own tensor).  False for other Nz.
 template <typename R>
-static bool launch_contract1(const Operator& op, const cx_t<R>* G, cx_t<R>* S, const Geo& g, hipStream_t s) {
+static bool launch_contract1(const Operator& op, const cx_t<R>* G, cx_t<R>* S, const Geo& g, bool mir,
+                             hipStream_t s) {
   if (op.Nz < C3_MIN_NZ || op.Nz > 32) return false;
+  const unsigned nblk = mir ? (unsigned)(op.Nx * (op.Nx + 1) / 2) : (unsigned)(op.Nx * op.Ny);
   auto go = [&](auto kern, int sm, int nw) {
     if (sm > 65536)
       G2G_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    hipLaunchKernelGGL(kern, dim3(op.Nx * op.Ny), dim3(64 * nw), sm, s, S, G, g, op.BS);
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(64 * nw), sm, s, S, G, g, op.BS);
   };
   constexpr int W32 = C1_W32<R>;
-  if (op.Nz == 16) go(k_contract1<R, 16, false, 4>, C1Cfg<R, 16, 4>::smem, 4);
-  else if (op.Nz == 32) go(k_contract1<R, 32, false, W32>, C1Cfg<R, 32, W32>::smem, W32);
-  else if (op.Nz < 16) go(k_contract1<R, 16, true, 4>, C1Cfg<R, 16, 4>::smem, 4);
-  else go(k_contract1<R, 32, true, W32>, C1Cfg<R, 32, W32>::smem, W32);
+  auto run = [&](auto MC) {
+    constexpr bool M = decltype(MC)::value;
+    if (op.Nz == 16) go(k_contract1<R, 16, false, 4, M>, C1Cfg<R, 16, 4, M>::smem, 4);
+    else if (op.Nz == 32) go(k_contract1<R, 32, false, W32, M>, C1Cfg<R, 32, W32, M>::smem, W32);
+    else if (op.Nz < 16) go(k_contract1<R, 16, true, 4, M>, C1Cfg<R, 16, 4, M>::smem, 4);
+    else go(k_contract1<R, 32, true, W32, M>, C1Cfg<R, 32, W32, M>::smem, W32);
+  };
+  if (mir) run(std::true_type{});
+  else run(std::false_type{});
   return true;
 }
-
 // Automatic kernel for one vector: the matrix cores, except complex64 at
 // padded layer counts where the vector-ALU kernel measured faster (128x128,
 // contraction alone: Nz = 20 by 19%, Nz = 50 and 56 by 5-8%; the padded tiles
@@ -1832,7 +1958,7 @@ static void fft_inverse(const Operator& op, OpBuffers<R>& b, const cx_t<R>* S, c
 
 template <typename R>
 void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, hipStream_t s,
-              bool timed) {
+              bool timed, bool mirror) {
   G2G_CHECK(op.have_tensor && op.have_sigma, -4, "operator not ready: tensor or conductivity missing");
   const int Nz3 = 3 * op.Nz;
   Geo g = make_geo(op, b.NR);
@@ -1871,8 +1997,8 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
     // matrix cores for Nz from 8 to 64 (complex64: where c3_auto says so)
     // unless a vector-ALU kernel is requested (FP64 at 128x128x32:
     // 0.28-0.30 ms against 0.33 ms)
-    if ((op.contraction == 4 || (op.contraction == 0 && c1_auto<R>(op.Nz))) &&
-        launch_contract1<R>(op, b.G, b.S, g, s)) {
+    if ((op.contraction == 4 || op.contraction == 5 || (op.contraction == 0 && c1_auto<R>(op.Nz))) &&
+        launch_contract1<R>(op, b.G, b.S, g, mirror && op.contraction != 4 && op_mirror_ok(op), s)) {
     } else if ((op.contraction == 2 || (op.contraction == 0 && c3_auto<R>(op.Nz))) &&
                launch_contract3<R, 1>(op, b.G, b.S, 0, g, s)) {
     } else if (op.contraction == 3) {
@@ -1959,7 +2085,8 @@ static void op_apply_k(Operator& op, OpBuffers<R>& b, int k, const cx_t<R>* cons
   bool ok = false;
   switch (kk) {
     case 1:
-      ok = (op.contraction == 0 && c1_auto<R>(op.Nz)) ? launch_contract1<R>(op, b.G, b.SB, g, s)
+      // (no x-y mirror in the batched applies, as for their single vectors)
+      ok = (op.contraction == 0 && c1_auto<R>(op.Nz)) ? launch_contract1<R>(op, b.G, b.SB, g, false, s)
                                                         : launch_contract3<R, 1>(op, b.G, b.SB, nS, g, s);
       break;
     case 2: ok = launch_contract3<R, 2>(op, b.G, b.SB, nS, g, s); break;
@@ -2009,7 +2136,7 @@ void op_apply_batch(Operator& op, OpBuffers<R>& b, int nrhs, const cx_t<R>* vin,
   for (int r0 = 0; r0 < nrhs;) {
     const int k = std::min(kmax, nrhs - r0);
     if (k == 1) {
-      op_apply<R>(op, b, vin + nv * r0, vout + nv * r0, s, false);
+      op_apply<R>(op, b, vin + nv * r0, vout + nv * r0, s, false, false);
     } else {
       const cx_t<R>* vins[8];
       cx_t<R>* vouts[8];
@@ -2102,7 +2229,7 @@ void op_apply_batch_noalloc(Operator& op, OpBuffers<R>& b, int nrhs, const cx_t<
   for (int r0 = 0; r0 < nrhs;) {
     const int k = std::min(kcap, nrhs - r0);
     if (k == 1) {
-      op_apply<R>(op, bl, vin + nv * r0, vout + nv * r0, s, false);
+      op_apply<R>(op, bl, vin + nv * r0, vout + nv * r0, s, false, false);
     } else {
       const T2* vins[8];
       T2* vouts[8];
@@ -2530,6 +2657,7 @@ void op_set_geometry(Operator& op, double dx, double dy, const double* z) {
   }
   G2G_HIP(hipStreamSynchronize(op.stream));
   if (dx != op.dx || dy != op.dy) op.have_wts = false;
+  op.tensor_own = false;  // a tensor from the earlier geometry: no mirror until the next build
   op.dx = dx;
   op.dy = dy;
   op.z.assign(z, z + op.Nz + 1);
@@ -2560,8 +2688,8 @@ template void op_alloc_buffers<float>(Operator&, OpBuffers<float>&, bool);
 template void op_free_buffers<double>(OpBuffers<double>&);
 template void op_free_buffers<float>(OpBuffers<float>&);
 template void op_apply<double>(Operator&, OpBuffers<double>&, const zd*, zd*,
-                               hipStream_t, bool);
+                               hipStream_t, bool, bool);
 template void op_apply<float>(Operator&, OpBuffers<float>&, const zf*, zf*, hipStream_t,
-                              bool);
+                              bool, bool);
 
 }  // namespace g2g

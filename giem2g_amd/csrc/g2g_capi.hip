@@ -163,6 +163,7 @@ void set_background(Operator& op, int Nl, const double* csigb, const double* thi
   op.have_bkg = true;
   op.bkg_version++;
   op.have_tensor = false;
+  op.tensor_own = false;
   op.have_g32 = false;
   // the per-cell scalings depend on the background: derive them again from
   // the conductivity last set
@@ -174,6 +175,7 @@ void build_tensor(Operator& op, int stage) {
     op_build_tensor(op);
     op.have_tensor = false;
     op.have_g32 = false;
+    op.tensor_own = true;
   }
   if (stage != 0) {
     hipEvent_t a, b;
@@ -497,6 +499,7 @@ int giem2g_op_put_tensor(void* p, const double* G, int spectral) {
     Operator& op = H(p)->op;
     G2G_HIP(hipMemcpy(op.d64.G, G, sizeof(zd) * op.Nx * op.Ny * op.BS, hipMemcpyHostToDevice));
     op.have_tensor = spectral != 0;
+    op.tensor_own = false;
     op.have_g32 = false;
     if (op.have_tensor && op.d32.G) op_make_g32(op);
   });
@@ -804,11 +807,13 @@ int giem2g_last_history(void* p, double* out, int maxlen, int* len) {
 int giem2g_op_set_contraction(void* p, int mode) {
   return guard([&] {
     Handle* h = H(p);
-    G2G_CHECK(mode >= 0 && mode <= 4, -3, "contraction mode must be 0, 1, 2, 3 or 4");
+    G2G_CHECK(mode >= 0 && mode <= 5, -3, "contraction mode must be 0 to 5");
     G2G_CHECK(mode != 2 || op_pair_supported(h->op), -3,
               "the matrix-core contraction needs Nz from 8 to 64");
-    G2G_CHECK(mode != 4 || (h->op.Nz >= 8 && h->op.Nz <= 32), -3,
+    G2G_CHECK((mode != 4 && mode != 5) || (h->op.Nz >= 8 && h->op.Nz <= 32), -3,
               "the small-block matrix-core contraction needs Nz from 8 to 32");
+    G2G_CHECK(mode != 5 || op_mirror_ok(h->op), -3,
+              "the x-y mirror needs Nx = Ny, dx = dy and a tensor built by the library");
     h->op.contraction = mode;
   });
 }

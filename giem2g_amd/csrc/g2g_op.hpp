@@ -179,7 +179,8 @@ struct Operator {
   OpBuffers<double> d64;
   OpBuffers<float> d32;
   hipStream_t stream = nullptr;
-  int contraction = 0;  // 0 automatic (2 where it exists, else 1), 1 vector ALU two-deep, 2 matrix cores, 3 vector ALU one-deep, 4 small-block matrix cores
+  int contraction = 0;  // 0 automatic, 1 vector ALU two-deep, 2 matrix cores, 3 vector ALU one-deep, 4 small-block matrix cores, 5 the same with the x-y mirror
+  bool tensor_own = false;  // the spectral tensor was built here (not put by the caller)
   Counters cnt;
   // tensor-build scratch (frequency independent weights are cached)
   double* wts = nullptr;  // 6 * 501 * Nx*Ny masked, lambda-scaled weights
@@ -240,15 +241,22 @@ inline int op_layer(const Operator& op, double z) {
 // apply kernels (g2g_apply.hip)
 template <typename R> void op_alloc_buffers(Operator& op, OpBuffers<R>& b, bool with_tensor);
 template <typename R> void op_free_buffers(OpBuffers<R>& b);
+// mirror: the one-vector contraction may take the x-y mirror where it
+// applies (op_mirror_ok); the batched applies pass false so that their
+// products do not depend on how the vectors are grouped
 template <typename R>
 void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, hipStream_t s,
-              bool timed);
+              bool timed, bool mirror = true);
 // two vectors through one tensor read; needs op_pair_supported(op)
 template <typename R>
 void op_apply2(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin0, const cx_t<R>* vin1, cx_t<R>* vout0,
                cx_t<R>* vout1, hipStream_t s);
 // Nz for which the matrix-core contraction exists (8 to 64; padded to a multiple of 16)
 bool op_pair_supported(const Operator& op);
+// The x-y mirror of the one-vector contraction applies: a square lateral grid
+// with dx = dy and a tensor the library built (a caller's tensor may lack the
+// symmetry that swapping the x and y axes gives a layered earth)
+inline bool op_mirror_ok(const Operator& op) { return op.tensor_own && op.Nx == op.Ny && op.dx == op.dy; }
 // right-hand sides one tensor read serves: 8 (Nz 8 to 32), 2 (Nz 33 to 64), else 1
 int op_batch_max(const Operator& op);
 // nrhs vectors stored one after another, batches of op_batch_max (FP64: at

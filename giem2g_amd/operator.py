@@ -291,7 +291,9 @@ class IEOperator:
         Nz = 17-23, 33-35, 49-63 uses 1), 1 vector ALU
         (two components in flight), 2 matrix cores (MFMA), 3 vector ALU (one
         component in flight), 4 small-block matrix cores (Nz from 8 to 32:
-        every operand column a point)."""
+        every operand column a point), 5 the same with the x-y mirror (one
+        tensor read for the quads (kx, ky) and (ky, kx); Nx = Ny, dx = dy
+        and a tensor built by the library; automatic where it applies)."""
         L.check(self.lib.giem2g_op_set_contraction(self.h, mode))
 
     def set_learned(self, net, accept=0.9):

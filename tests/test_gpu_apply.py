@@ -119,6 +119,54 @@ def test_small_block_contraction_matches_oracle(oracle, Nx, Ny, Nz):
     big.close()
 
 
+@pytest.mark.parametrize("N,Nz", [(16, 16), (32, 32), (24, 20), (16, 9), (12, 31)])
+def test_mirror_contraction_matches_oracle(oracle, N, Nz):
+    """The x-y mirror of the small-block contraction (mode 5; automatic on a
+    square grid with dx = dy and a tensor built by the library): one tensor
+    read serves the quads (kx, ky) and (ky, kx).  On the library's own
+    spectral tensor, mode 5 and mode 4 against the oracle's apply with that
+    tensor, FP64 at 1e-12 (the tensor's own x-y asymmetry, the weight-error
+    scale, is what separates the two: measured 1.3e-13 to 1.7e-13 on the
+    oracle's tensor) and complex64 within 2e-6.  The mirror is refused on a
+    rectangular grid and once the caller puts a tensor, after which the
+    automatic choice reproduces mode 4 exactly."""
+    torch = _torch()
+    from giem2g_amd import lib as L, models, operator as gop
+    print("case", N, Nz, flush=True)
+    m = models.small(N, N, Nz, seed=N + 7 * Nz, freq=1.5, dz=20.0)
+    om = oracle_model(oracle, m)
+    op = gop.operator_for(m)
+    Gs = op.get_tensor()
+    v = rand_vec(3 * m.ncell, 16)
+    ref = om.apply(Gs, m.csiga(), v)
+    op.set_contraction(4)
+    r4 = op.apply_host(v)
+    op.set_contraction(5)
+    r5 = op.apply_host(v)
+    op.set_contraction(0)
+    r0 = op.apply_host(v)
+    e4, e5 = rel(r4, ref), rel(r5, ref)
+    print("  mode 4 %.2e, mode 5 %.2e, 5 vs 4 %.2e" % (e4, e5, rel(r5, r4)), flush=True)
+    assert e4 < 1e-12 and e5 < 1e-12
+    assert rel(r0, r5) < 1e-15  # the automatic choice takes the mirror here
+    vt = torch.from_numpy(v).cuda().to(torch.complex64)
+    o32 = torch.empty_like(vt)
+    op.set_contraction(5)
+    op.apply_dev_f32(vt, o32)
+    torch.cuda.synchronize()
+    assert rel(o32.to(torch.complex128).cpu().numpy(), ref) < 2e-6
+    op.put_tensor(Gs, spectral=True)
+    with pytest.raises(L.G2GError, match="mirror"):
+        op.set_contraction(5)
+    op.set_contraction(0)
+    assert rel(op.apply_host(v), r4) < 1e-15
+    op.close()
+    rect = gop.operator_for(models.small(16, 8, Nz, seed=1, freq=1.5, dz=20.0))
+    with pytest.raises(L.G2GError, match="mirror"):
+        rect.set_contraction(5)
+    rect.close()
+
+
 @pytest.mark.parametrize("Nx,Ny,Nz", [(16, 8, 16), (8, 16, 32), (8, 8, 48), (8, 8, 64),
                                       (8, 8, 12), (8, 16, 50), (16, 8, 9)])
 def test_paired_apply_matches_oracle(oracle, Nx, Ny, Nz):
@@ -367,8 +415,12 @@ def test_error_paths():
     # matrix-core contraction and two-vector applies need Nz from 8 to 64
     with pytest.raises(L.G2GError, match="Nz from 8 to 64"):
         op.set_contraction(2)
+    # the small-block kernels (with or without the x-y mirror) need 8 to 32
+    for mode in (4, 5):
+        with pytest.raises(L.G2GError, match="Nz from 8 to 32"):
+            op.set_contraction(mode)
     with pytest.raises(L.G2GError, match="contraction mode"):
-        op.set_contraction(5)
+        op.set_contraction(6)
     op.close()
     import torch
     from giem2g_amd import models
