@@ -266,14 +266,20 @@ int giem2g_solve_dev(void *ie_op, const void *E_bkg_dev, const void *E0_dev,
                      const giem2g_solver_params *p, void *E_sol_dev,
                      giem2g_solve_info *info);
 /* Kernel of the per-(kx,ky) tensor contraction of one vector: 0 automatic
- * (4 for FP64 at Nz from 8 to 32 and complex64 at Nz from 24 to 32, else 2
+ * (5, else 4, for FP64 at Nz from 8 to 32 and complex64 at Nz from 24 to 32, else 2
  * for Nz from 8 to 64, otherwise 1; complex64 products at Nz = 17-23,
  * 33-35 and 49-63 use 1, measured faster there than the zero-padded
  * matrix-core tiles), 1 vector ALU with two tensor components in flight,
  * 2 matrix cores (MFMA; Nz from 8 to 64, padded to a multiple of 16), 3
  * vector ALU with one component in flight, 4 small-block matrix cores (Nz
  * from 8 to 32, padded to 16 or 32; the four points are the operand columns
- * of 4x4 MFMA blocks).  All compute the same product.
+ * of 4x4 MFMA blocks), 5 the same with the x-y mirror: one tensor read for
+ * the quads (kx, ky) and (ky, kx), exact for a layered earth on a square
+ * grid with dx = dy (needs Nx = Ny, dx = dy and a tensor built by the
+ * library; automatic under mode 0 where these hold; giem2g_op_put_tensor or
+ * a new geometry turns it off until the next build; batched applies read
+ * the full tensor).  All compute the same product, the mirror within the
+ * tensor's own x-y asymmetry (about 1e-13 relative).
  * Two-vector products (giem2g_op_apply_pair_dev, giem2g_solve_pair_dev) use
  * the matrix cores. */
 int giem2g_op_set_contraction(void *ie_op, int mode);
