@@ -10,7 +10,9 @@ them (any even Nx, Ny: CalcLocalKernelSize, Integral_Equation_Module.F90:
     oracle's spectrum, the GPU tensor and its spectrum, and FP64 and
     mixed-precision solves;
   - the paper's COMMEMI3D-3 lateral grid, 176x224 (352- and 448-point
-    transforms), with sampled real-space blocks and an apply on 4 layers.
+    transforms), with sampled real-space blocks and an apply on 4 layers;
+  - the largest grid side accepted, 512 (1024-point transforms in x and in
+    y): tensor, spectrum and apply at 512x16 and 16x512.
 """
 import numpy as np
 import pytest
@@ -36,7 +38,7 @@ def rand_vec(n, seed):
 
 
 @pytest.mark.parametrize("Nx,Ny,Nz", [(10, 14, 5), (24, 40, 8), (22, 12, 3), (12, 26, 6), (34, 16, 4),
-                                      (8, 8, 72), (12, 10, 100), (16, 8, 256)])
+                                      (8, 8, 72), (12, 10, 100), (16, 8, 256), (512, 16, 4), (16, 512, 4)])
 def test_apply_on_oracle_spectrum(oracle, Nx, Ny, Nz):
     torch = _torch()
     from giem2g_amd import models, operator as gop
@@ -61,7 +63,7 @@ def test_apply_on_oracle_spectrum(oracle, Nx, Ny, Nz):
     op.close()
 
 
-@pytest.mark.parametrize("Nx,Ny,Nz", [(24, 40, 8), (10, 14, 72)])
+@pytest.mark.parametrize("Nx,Ny,Nz", [(24, 40, 8), (10, 14, 72), (512, 16, 4), (16, 512, 4)])
 def test_tensor_and_spectrum(oracle, Nx, Ny, Nz):
     """Layers of 20 m, so that the layer borders meet the background's 1 km
     interface.  A cell that straddles an interface (15 m layers: 990-1005 m)
