@@ -1050,7 +1050,13 @@ struct C3Cfg {
   static constexpr size_t smem = vin_bytes + (stage_bytes > red_bytes ? stage_bytes : red_bytes);
 };
 
-template <typename R, int NZ, int NRHS, int DEEP, int NW, bool PAD>
+// MIR (NRHS = 2 only): the second right-hand side is the x-y mirror of the
+// first, as in k_contract1: blocks cover kx <= ky, and "rhs 1" is the quad
+// (ky, kx), read at the transposed coordinates (PY(p), PX(p)) with x <-> y
+// exchanged and stored the same way, with the signs of point p; a diagonal
+// block (kx = ky) leaves it out.  One-vector products then fill the tile
+// columns that would otherwise repeat, from one tensor read per pair.
+template <typename R, int NZ, int NRHS, int DEEP, int NW, bool PAD, bool MIR = false>
 __device__ __forceinline__ void contract3_body(cx_t<R>* __restrict__ S, int64_t sst,
                                                const cx_t<R>* __restrict__ G, const Geo& g, int64_t BS) {
   using T2 = cx_t<R>;
@@ -1076,13 +1082,35 @@ __device__ __forceinline__ void contract3_body(cx_t<R>* __restrict__ S, int64_t 
 #define C3_DIVNZ(x) (PAD ? (int)(((uint32_t)(x) * mnz) >> 20) : (x) / NZ)
 #define C3_DIVNZ3(x) (PAD ? (int)(((uint32_t)(x) * mnz3) >> 20) : (x) / NZ3)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  static_assert(!MIR || NRHS == 2, "the mirror takes the second operand group");
   // blocks walk ky fastest: concurrent blocks touch neighbouring spectrum
-  // rows (measured 2-3% faster at 256x256 than kx fastest)
-  const int kx = blockIdx.x / g.Ny, ky = blockIdx.x % g.Ny, b = kx + g.Nx * ky;
+  // rows (measured 2-3% faster at 256x256 than kx fastest); with MIR block
+  // t = ky (ky + 1) / 2 + kx, kx <= ky
+  int kx, ky;
+  if constexpr (MIR) {
+    const int t = blockIdx.x;
+    int y = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+    while ((y + 1) * (y + 2) / 2 <= t) y++;
+    while (y * (y + 1) / 2 > t) y--;
+    ky = y;
+    kx = t - y * (y + 1) / 2;
+  } else {
+    kx = blockIdx.x / g.Ny;
+    ky = blockIdx.x % g.Ny;
+  }
+  const int b = kx + g.Nx * ky;
+  const bool two = !MIR || kx != ky;  // the mirrored set exists off the diagonal
   const int Nx2 = 2 * g.Nx, Ny2 = 2 * g.Ny, gNx = g.Nx, gNy = g.Ny;
 #define PX(p) (((p) & 1) ? (kx == 0 ? gNx : Nx2 - kx) : kx)
 #define PY(p) (((p) & 2) ? (ky == 0 ? gNy : Ny2 - ky) : ky)
 #define VAL(p) ((!((p) & 1) || kx > 0) && (!((p) & 2) || ky > 0))
+  // rhs r of point p: its spectrum row and the component stored as ci
+#define C3_MSET(r) (MIR && (r) == 1)
+#define C3_ROWP(r, p)                                                              \
+  (C3_MSET(r) ? S + ((int64_t)PY(p) * Ny2 + PX(p)) * nz3                            \
+              : S + (MIR ? 0 : (r) * sst) + ((int64_t)PX(p) * Ny2 + PY(p)) * nz3)
+#define C3_CS(r, ci) ((C3_MSET(r) && (ci) < 2) ? 1 - (ci) : (ci))
+#define C3_LIVE(r, p) (VAL(p) && (!C3_MSET(r) || two))
   const T2* blk = G + (int64_t)b * BS;
   const T2 zero = mk<T2>(0, 0);
   // DEEP register sets F, G, H, I, J, K; component c travels in set c % DEEP
@@ -1149,13 +1177,13 @@ __device__ __forceinline__ void contract3_body(cx_t<R>* __restrict__ S, int64_t 
   if ((k) < NVQ) {                                                                         \
     const int i = min(tid + (k) * NTH, NV - 1);                                            \
     const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3, ci = e / NZ, z = min(e % NZ, nz - 1); \
-    vq##k = S[(rp >> 2) * sst + ((int64_t)PX(p) * Ny2 + PY(p)) * nz3 + ci * nz + z];       \
+    vq##k = C3_ROWP(rp >> 2, p)[C3_CS(rp >> 2, ci) * nz + z];                               \
   }
 #define C3_VQP(k)                                                                  \
   if ((k) < NVQ && tid + (k) * NTH < NV) {                                       \
     const int i = tid + (k) * NTH;                                               \
     const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3;                        \
-    T2 vv = (VAL(p) && e % NZ < nz) ? vq##k : zero;                              \
+    T2 vv = (C3_LIVE(rp >> 2, p) && e % NZ < nz) ? vq##k : zero;                 \
     /* D_p: x times s (-1 for -kx), y times m (-1 for -ky) */                    \
     if ((e < NZ && (p & 1)) || (e >= NZ && e < 2 * NZ && (p & 2))) vv = -vv;     \
     vin[i] = vv;                                                                 \
@@ -1174,9 +1202,8 @@ __device__ __forceinline__ void contract3_body(cx_t<R>* __restrict__ S, int64_t 
   } else {
     for (int i = tid; i < NV; i += NTH) {
       const int rp = i / NZ3, e = i - rp * NZ3, p = rp & 3, ci = e / NZ, z = e % NZ;
-      const T2* Sr = S + (rp >> 2) * sst;
       T2 vv = zero;
-      if (VAL(p) && z < nz) vv = Sr[((int64_t)PX(p) * Ny2 + PY(p)) * nz3 + ci * nz + z];
+      if (C3_LIVE(rp >> 2, p) && z < nz) vv = C3_ROWP(rp >> 2, p)[C3_CS(rp >> 2, ci) * nz + z];
       if ((e < NZ && (p & 1)) || (e >= NZ && e < 2 * NZ && (p & 2))) vv = -vv;
       vin[i] = vv;
     }
@@ -1320,10 +1347,10 @@ __device__ __forceinline__ void contract3_body(cx_t<R>* __restrict__ S, int64_t 
   __syncthreads();
   for (int i = tid; i < NRHS * 4 * nz3; i += NTH) {
     const int rp = C3_DIVNZ3(i), e = i - rp * nz3, p = rp & 3, ci = C3_DIVNZ(e), z = e - ci * nz;
-    T2* Sr = S + (rp >> 2) * sst;
+    if (C3_MSET(rp >> 2) && !two) continue;
     T2 o = vin[rp * NZ3 + ci * NZ + z];
     if (!VAL(p)) o = zero;
-    Sr[((int64_t)PX(p) * Ny2 + PY(p)) * nz3 + e] = o;
+    C3_ROWP(rp >> 2, p)[C3_CS(rp >> 2, ci) * nz + z] = o;
   }
 #undef C3_VQL
 #undef C3_VQP
@@ -1348,6 +1375,10 @@ __device__ __forceinline__ void contract3_body(cx_t<R>* __restrict__ S, int64_t 
 #undef PX
 #undef PY
 #undef VAL
+#undef C3_MSET
+#undef C3_ROWP
+#undef C3_CS
+#undef C3_LIVE
 }
 // Minimum waves per SIMD asked of the compiler for the unpadded kernel: four
 // for complex64 at NZ = 48 (one or two right-hand sides fit in 128 registers
@@ -1359,18 +1390,18 @@ template <typename R, int NZ, int NRHS> constexpr int c3_waves() {
   return (sizeof(R) == 4 && NZ == 48 && NRHS <= 2) ? 4 : 1;
 }
 // Nz a multiple of 16 (NZ = Nz)
-template <typename R, int NZ, int NRHS, int DEEP, int NW>
+template <typename R, int NZ, int NRHS, int DEEP, int NW, bool MIR = false>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(c3_waves<R, NZ, NRHS>()))) void k_contract3(cx_t<R>* __restrict__ S, int64_t sst,
                                                    const cx_t<R>* __restrict__ G, Geo g, int64_t BS) {
-  contract3_body<R, NZ, NRHS, DEEP, NW, false>(S, sst, G, g, BS);
+  contract3_body<R, NZ, NRHS, DEEP, NW, false, MIR>(S, sst, G, g, BS);
 }
 // other Nz from 8 to 64: the tiles of the next multiple of 16, zero-padded
 // (at least two waves per SIMD: the padded paired kernels for FP64 at NZ = 48
 // and complex64 at NZ = 64 otherwise take more than 256 registers a lane)
-template <typename R, int NZ, int NRHS, int DEEP, int NW>
+template <typename R, int NZ, int NRHS, int DEEP, int NW, bool MIR = false>
 __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2))) void k_contract3_pad(cx_t<R>* __restrict__ S, int64_t sst,
                                                        const cx_t<R>* __restrict__ G, Geo g, int64_t BS) {
-  contract3_body<R, NZ, NRHS, DEEP, NW, true>(S, sst, G, g, BS);
+  contract3_body<R, NZ, NRHS, DEEP, NW, true, MIR>(S, sst, G, g, BS);
 }
 
 // ------------------------------------------------------------ dispatch ---
@@ -1378,14 +1409,16 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2))) vo
 // through one tensor read: one or two for Nz <= 64, four or eight for
 // Nz <= 32 (register budget).  Nz that is not a multiple of 16 runs the next
 // multiple's kernel with zero-padded tiles.  False for other cases.
-template <typename R, int NRHS>
+// MIR (NRHS = 2): one vector and its x-y mirror (op_mirror_ok), blocks over kx <= ky.
+template <typename R, int NRHS, bool MIR = false>
 static bool launch_contract3(const Operator& op, const cx_t<R>* G, cx_t<R>* S, int64_t sst, const Geo& g,
                              hipStream_t s) {
+  const unsigned nblk = MIR ? (unsigned)(op.Nx * (op.Nx + 1) / 2) : (unsigned)(op.Nx * op.Ny);
   auto go = [&](auto kern, size_t sm, int nw) {
     G2G_CHECK(sm <= 160 * 1024, -3, "matrix-core contraction does not fit in LDS for this Nz");
     if (sm > 65536)
       G2G_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-    hipLaunchKernelGGL(kern, dim3(op.Nx * op.Ny), dim3(64 * nw), sm, s, S, sst, G, g, op.BS);
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(64 * nw), sm, s, S, sst, G, g, op.BS);
   };
   // measured choices: FP64 keeps one component in flight (Nz = 32: four
   // waves per SIMD instead of three, 1-3% faster) and at Nz = 64 uses eight
@@ -1400,8 +1433,8 @@ static bool launch_contract3(const Operator& op, const cx_t<R>* G, cx_t<R>* S, i
   const bool pad = nzp != op.Nz;
   auto run = [&](auto ZC, auto DC, auto WC) {
     constexpr int Z = decltype(ZC)::value, D = decltype(DC)::value, W = decltype(WC)::value;
-    if (pad) go(k_contract3_pad<R, Z, NRHS, D, W>, C3Cfg<R, Z, NRHS, W>::smem, W);
-    else go(k_contract3<R, Z, NRHS, D, W>, C3Cfg<R, Z, NRHS, W>::smem, W);
+    if (pad) go(k_contract3_pad<R, Z, NRHS, D, W, MIR>, C3Cfg<R, Z, NRHS, W>::smem, W);
+    else go(k_contract3<R, Z, NRHS, D, W, MIR>, C3Cfg<R, Z, NRHS, W>::smem, W);
   };
   using std::integral_constant;
   switch (nzp) {
@@ -1828,7 +1861,10 @@ static bool launch_contract1(const Operator& op, const cx_t<R>* G, cx_t<R>* S, c
   else run(std::false_type{});
   return true;
 }
-// Automatic kernel for one vector: the matrix cores, except complex64 at
+// Automatic kernel for one vector without the x-y mirror (with it, measured
+// at 128x128 for Nz = 8 to 64: the small-block kernel where c1_auto says so,
+// else the mirrored 16x16x4 tiles, which beat the vector-ALU kernel at every
+// Nz, 1.5-3.4x): the matrix cores, except complex64 at
 // padded layer counts where the vector-ALU kernel measured faster (128x128,
 // contraction alone: Nz = 20 by 19%, Nz = 50 and 56 by 5-8%; the padded tiles
 // won at Nz = 8, 12, 28, 40, tied at 24, and won in FP64 at every Nz
@@ -1997,10 +2033,17 @@ void op_apply(Operator& op, OpBuffers<R>& b, const cx_t<R>* vin, cx_t<R>* vout, 
     // matrix cores for Nz from 8 to 64 (complex64: where c3_auto says so)
     // unless a vector-ALU kernel is requested (FP64 at 128x128x32:
     // 0.28-0.30 ms against 0.33 ms)
-    if ((op.contraction == 4 || op.contraction == 5 || (op.contraction == 0 && c1_auto<R>(op.Nz))) &&
-        launch_contract1<R>(op, b.G, b.S, g, mirror && op.contraction != 4 && op_mirror_ok(op), s)) {
-    } else if ((op.contraction == 2 || (op.contraction == 0 && c3_auto<R>(op.Nz))) &&
-               launch_contract3<R, 1>(op, b.G, b.S, 0, g, s)) {
+    // the x-y mirror where it applies (modes 0, 5 and 6); small-block
+    // kernel, then the mirrored 16x16x4 tiles, then the plain tiles
+    const bool mir = mirror && op_mirror_ok(op);
+    bool done = false;
+    if (op.contraction == 4 || op.contraction == 5 || (op.contraction == 0 && c1_auto<R>(op.Nz)))
+      done = launch_contract1<R>(op, b.G, b.S, g, op.contraction != 4 && mir, s);
+    if (!done && mir && (op.contraction == 6 || op.contraction == 0) && c3_supported(op.Nz, 2))
+      done = launch_contract3<R, 2, true>(op, b.G, b.S, 0, g, s);
+    if (!done && (op.contraction == 2 || op.contraction == 6 || (op.contraction == 0 && c3_auto<R>(op.Nz))))
+      done = launch_contract3<R, 1>(op, b.G, b.S, 0, g, s);
+    if (done) {
     } else if (op.contraction == 3) {
       if (op.Nz * op.Nz <= 4 * NT) launch(k_contract<R, 4>);
       else launch(k_contract<R, 16>);

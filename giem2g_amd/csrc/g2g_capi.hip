@@ -807,13 +807,15 @@ int giem2g_last_history(void* p, double* out, int maxlen, int* len) {
 int giem2g_op_set_contraction(void* p, int mode) {
   return guard([&] {
     Handle* h = H(p);
-    G2G_CHECK(mode >= 0 && mode <= 5, -3, "contraction mode must be 0 to 5");
+    G2G_CHECK(mode >= 0 && mode <= 6, -3, "contraction mode must be 0 to 6");
     G2G_CHECK(mode != 2 || op_pair_supported(h->op), -3,
               "the matrix-core contraction needs Nz from 8 to 64");
     G2G_CHECK((mode != 4 && mode != 5) || (h->op.Nz >= 8 && h->op.Nz <= 32), -3,
               "the small-block matrix-core contraction needs Nz from 8 to 32");
-    G2G_CHECK(mode != 5 || op_mirror_ok(h->op), -3,
+    G2G_CHECK((mode != 5 && mode != 6) || op_mirror_ok(h->op), -3,
               "the x-y mirror needs Nx = Ny, dx = dy and a tensor built by the library");
+    G2G_CHECK(mode != 6 || op_pair_supported(h->op), -3,
+              "the matrix-core contraction needs Nz from 8 to 64");
     h->op.contraction = mode;
   });
 }

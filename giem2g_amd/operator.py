@@ -293,7 +293,8 @@ class IEOperator:
         component in flight), 4 small-block matrix cores (Nz from 8 to 32:
         every operand column a point), 5 the same with the x-y mirror (one
         tensor read for the quads (kx, ky) and (ky, kx); Nx = Ny, dx = dy
-        and a tensor built by the library; automatic where it applies)."""
+        and a tensor built by the library; automatic where it applies), 6
+        the 16x16x4 tiles with the x-y mirror (Nz from 8 to 64)."""
         L.check(self.lib.giem2g_op_set_contraction(self.h, mode))
 
     def set_learned(self, net, accept=0.9):

@@ -266,7 +266,8 @@ int giem2g_solve_dev(void *ie_op, const void *E_bkg_dev, const void *E0_dev,
                      const giem2g_solver_params *p, void *E_sol_dev,
                      giem2g_solve_info *info);
 /* Kernel of the per-(kx,ky) tensor contraction of one vector: 0 automatic
- * (5, else 4, for FP64 at Nz from 8 to 32 and complex64 at Nz from 24 to 32, else 2
+ * (with the x-y mirror 5 or 6, as below; without it 4 for FP64 at Nz from 8
+ * to 32 and complex64 at Nz from 24 to 32, else 2
  * for Nz from 8 to 64, otherwise 1; complex64 products at Nz = 17-23,
  * 33-35 and 49-63 use 1, measured faster there than the zero-padded
  * matrix-core tiles), 1 vector ALU with two tensor components in flight,
@@ -279,7 +280,10 @@ int giem2g_solve_dev(void *ie_op, const void *E_bkg_dev, const void *E0_dev,
  * library; automatic under mode 0 where these hold; giem2g_op_put_tensor or
  * a new geometry turns it off until the next build; batched applies read
  * the full tensor).  All compute the same product, the mirror within the
- * tensor's own x-y asymmetry (about 1e-13 relative).
+ * tensor's own x-y asymmetry (about 1e-13 relative); 6 the 16x16x4 tiles
+ * with the x-y mirror (Nz from 8 to 64; the mirrored quad fills the operand
+ * columns a one-vector tile would repeat).  Under mode 0 with the mirror:
+ * mode 5 for FP64 at Nz 8-32 and complex64 at Nz 24-32, mode 6 otherwise.
  * Two-vector products (giem2g_op_apply_pair_dev, giem2g_solve_pair_dev) use
  * the matrix cores. */
 int giem2g_op_set_contraction(void *ie_op, int mode);

@@ -167,6 +167,34 @@ def test_mirror_contraction_matches_oracle(oracle, N, Nz):
     rect.close()
 
 
+@pytest.mark.parametrize("N,Nz", [(16, 48), (12, 50), (16, 64), (8, 40), (10, 33)])
+def test_mirror_tiles_match_oracle(oracle, N, Nz):
+    """The x-y mirror on the 16x16x4 tiles (Nz from 33 to 64, one vector):
+    the mirrored quad fills the operand columns that a one-vector tile would
+    repeat.  The automatic choice (mirror) and mode 2 (the full tensor read)
+    against the oracle's apply on the library's own tensor, FP64 at 1e-12;
+    once the caller puts a tensor the automatic choice equals mode 2."""
+    from giem2g_amd import models, operator as gop
+    print("case", N, Nz, flush=True)
+    m = models.small(N, N, Nz, seed=3 * N + Nz, freq=1.5, dz=20.0)
+    om = oracle_model(oracle, m)
+    op = gop.operator_for(m)
+    Gs = op.get_tensor()
+    v = rand_vec(3 * m.ncell, 18)
+    ref = om.apply(Gs, m.csiga(), v)
+    op.set_contraction(2)
+    r2 = op.apply_host(v)
+    op.set_contraction(0)
+    r0 = op.apply_host(v)
+    e2, e0 = rel(r2, ref), rel(r0, ref)
+    print("  mode 2 %.2e, automatic (mirror) %.2e, apart %.2e" % (e2, e0, rel(r0, r2)), flush=True)
+    assert e2 < 1e-12 and e0 < 1e-12
+    assert rel(r0, r2) > 0  # the mirror did run (a different summation)
+    op.put_tensor(Gs, spectral=True)
+    assert rel(op.apply_host(v), r2) < 1e-15
+    op.close()
+
+
 @pytest.mark.parametrize("Nx,Ny,Nz", [(16, 8, 16), (8, 16, 32), (8, 8, 48), (8, 8, 64),
                                       (8, 8, 12), (8, 16, 50), (16, 8, 9)])
 def test_paired_apply_matches_oracle(oracle, Nx, Ny, Nz):
@@ -419,8 +447,11 @@ def test_error_paths():
     for mode in (4, 5):
         with pytest.raises(L.G2GError, match="Nz from 8 to 32"):
             op.set_contraction(mode)
-    with pytest.raises(L.G2GError, match="contraction mode"):
+    # the mirrored tiles need a tensor built here (none yet)
+    with pytest.raises(L.G2GError, match="mirror"):
         op.set_contraction(6)
+    with pytest.raises(L.G2GError, match="contraction mode"):
+        op.set_contraction(7)
     op.close()
     import torch
     from giem2g_amd import models
