@@ -12,6 +12,7 @@
 //     S (full 2Nx x 2Ny spectrum, [kx][ky][c*Nz+z]).
 #pragma once
 #include <cmath>
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -123,6 +124,22 @@ struct Arena {
     }
     return x.p;
   }
+  // pinned host buffers (coherent, mapped: kernels write them directly and
+  // the host reads them without a copy), zeroed when allocated
+  std::vector<Buf> h;
+  void* get_pinned(int slot, size_t bytes) {
+    if ((size_t)slot >= h.size()) h.resize(slot + 1);
+    Buf& x = h[slot];
+    if (x.n < bytes) {
+      if (x.p) G2G_HIP(hipHostFree(x.p));
+      x.p = nullptr;
+      x.n = 0;
+      G2G_HIP(hipHostMalloc(&x.p, bytes, hipHostMallocCoherent | hipHostMallocMapped));
+      std::memset(x.p, 0, bytes);
+      x.n = bytes;
+    }
+    return x.p;
+  }
   size_t bytes() const {
     size_t t = 0;
     for (const Buf& x : b) t += x.n;
@@ -132,6 +149,9 @@ struct Arena {
     for (Buf& x : b)
       if (x.p) (void)hipFree(x.p);
     b.clear();
+    for (Buf& x : h)
+      if (x.p) (void)hipHostFree(x.p);
+    h.clear();
   }
 };
 

@@ -12,6 +12,7 @@
 // rotations stay on the host.  The inner solver can run on complex64
 // vectors with the complex64 operator (mixed precision; the outer stays FP64
 // and acts as iterative refinement).
+#include <chrono>
 #include <complex>
 #include <functional>
 #include <vector>
@@ -196,13 +197,48 @@ static inline dim3 gridN(int64_t N) {
   return dim3((unsigned)(b < 4096 ? b : 4096));
 }
 
+// Results the host reads (dot products, a Gram-Schmidt pass's coefficients
+// and norm) reach it through a mailbox in pinned host memory: one wave
+// copies them there and then raises a sequence number, which the host polls.
+// The host sees them a few microseconds after the kernel ends; a copy and a
+// blocking stream synchronisation took about 50 us per Gram-Schmidt pass.
+struct Mailbox {
+  unsigned seq;
+  unsigned pad[63];
+  double v[2 * MAXK + 2 * KC];
+};
+__global__ __launch_bounds__(64) void k_publish(const double* __restrict__ src, int n, Mailbox* mb,
+                                                unsigned seq) {
+  for (int i = threadIdx.x; i < n; i += 64) mb->v[i] = src[i];
+  __threadfence_system();  // the values are visible to the host before the number
+  if (threadIdx.x == 0) __hip_atomic_store(&mb->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+// the host polls this long before it falls back to a blocking synchronisation
+constexpr int MAIL_SPIN_MS = 200;
+
+// Waits until the work queued on s so far has ended, polling an event
+static void poll_event(hipEvent_t ev, hipStream_t s) {
+  G2G_HIP(hipEventRecord(ev, s));
+  const auto t0 = std::chrono::steady_clock::now();
+  for (unsigned i = 1;; i++) {
+    const hipError_t e = hipEventQuery(ev);
+    if (e == hipSuccess) return;
+    if (e != hipErrorNotReady) G2G_HIP(e);
+    if ((i & 255) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(MAIL_SPIN_MS)) break;
+    __builtin_ia32_pause();
+  }
+  G2G_HIP(hipEventSynchronize(ev));
+}
+
 // ---------------------------------------------------------------- BLAS ---
 struct DevBlas {
   hipStream_t s;
   void* part = nullptr;  // (MAXK/KC) * NBLK * 2KC doubles
   double* res = nullptr; // 2*MAXK dot products (complex pairs) + 2KC for a norm
   zd* hbuf = nullptr; // 64
-  double hres[2 * MAXK + 2 * KC];
+  Mailbox* mail = nullptr;   // pinned host memory
+  Mailbox* dmail = nullptr;  // its device address
+  const double* hres = nullptr;  // mail->v
   // counts of this solver (merged into the operator's counters by the caller)
   int64_t dots = 0, passes = 0;
   double bytes = 0;  // bytes the Krylov kernels read and write
@@ -216,6 +252,9 @@ struct DevBlas {
     part = ws.get(WS_PART, sizeof(double) * (MAXK / KC) * NBLK * 2 * KC);
     res = (double*)ws.get(WS_RES, sizeof(double) * (2 * MAXK + 2 * KC));
     hbuf = (zd*)ws.get(WS_HBUF, sizeof(zd) * 64);
+    mail = (Mailbox*)ws.get_pinned(0, sizeof(Mailbox));
+    G2G_HIP(hipHostGetDevicePointer((void**)&dmail, mail, 0));
+    hres = mail->v;
   }
   static int nblocks(int64_t N) { return (int)std::min<int64_t>(NBLK, (N + BT - 1) / BT); }
   // res[0 .. 2K) = V[:,k]^H w for k < K, on the device (no host round trip)
@@ -226,9 +265,23 @@ struct DevBlas {
     hipLaunchKernelGGL((k_mdot<T2, A>), dim3(nb, nrow), dim3(BT), 0, s, V, N, K, w, (A*)part);
     hipLaunchKernelGGL((k_reduce<A>), dim3(nrow), dim3(BT), 0, s, (const A*)part, nb, res);
   }
+  // res[0 .. n) to hres; everything queued on the stream before has ended
   void fetch(int n) {
-    G2G_HIP(hipMemcpyAsync(hres, res, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    // the last number published has been seen, so it cannot change meanwhile
+    const unsigned seq = __atomic_load_n(&mail->seq, __ATOMIC_ACQUIRE) + 1;
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, s, (const double*)res, n, dmail, seq);
+    G2G_HIP(hipGetLastError());
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned i = 1;; i++) {
+      if (__atomic_load_n(&mail->seq, __ATOMIC_ACQUIRE) == seq) return;
+      if ((i & 4095) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(MAIL_SPIN_MS))
+        break;
+      __builtin_ia32_pause();
+    }
+    // a long queue (or an error, which the synchronisation reports)
     G2G_HIP(hipStreamSynchronize(s));
+    G2G_CHECK(__atomic_load_n(&mail->seq, __ATOMIC_ACQUIRE) == seq, -1,
+              "the reduction results did not reach the host");
   }
   // out[k] = V[:,k]^H w for k < K
   template <typename T2, typename A>
@@ -697,7 +750,11 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
 // wait for a product of the same precision, one two-vector apply serves both
 // (op_apply2: the tensor is read once for two right-hand sides on the matrix
 // cores); a thread whose partner has finished or waits for the other
-// precision goes alone.
+// precision goes alone.  A requester's queued work ends before its request
+// is posted, and the issuer waits for the product before either thread goes
+// on; both waits poll an event (measured: event waits between the streams
+// instead of host waits made the paired solves 5% slower, and blocking
+// synchronisations wake the host about 50 us after the work ends).
 namespace {
 struct PairBatcher {
   Operator* op = nullptr;
@@ -708,7 +765,10 @@ struct PairBatcher {
     void* out = nullptr;
     int prec = 0;
     bool pending = false, done = false;
+    hipStream_t s = nullptr;     // the requester's stream
+    hipEvent_t ready = nullptr;  // polled before the request is posted
   } r[2];
+  hipEvent_t served = nullptr;   // polled after each product
   bool active[2] = {true, true};
   long long paired = 0, alone = 0;
 
@@ -717,6 +777,7 @@ struct PairBatcher {
     else op_apply<float>(*op, op->d32, (const zf*)q.in, (zf*)q.out, op->stream, false);
   }
   void apply(int me, const void* in, void* out, int prec) {
+    poll_event(r[me].ready, r[me].s);  // s and ready do not change while the threads run
     std::unique_lock<std::mutex> lk(mu);
     Req& m = r[me];
     m.in = in;
@@ -747,7 +808,7 @@ struct PairBatcher {
           run_one(m);
           alone++;
         }
-        G2G_HIP(hipStreamSynchronize(op->stream));
+        poll_event(served, op->stream);
         if (r[ot].pending) {
           r[ot].pending = false;
           r[ot].done = true;
@@ -780,33 +841,50 @@ PairResult nested_solve_pair(Operator& op, const zd* const b[2], const zd* const
   pb.op = &op;
   PairResult out;
   std::exception_ptr err[2];
+  // streams and events exist before either thread starts (a request reads its
+  // partner's); they are released after both threads have drained
+  auto release = [&]() {
+    for (auto& q : pb.r) {
+      if (q.s) (void)hipStreamSynchronize(q.s);
+      if (q.ready) (void)hipEventDestroy(q.ready);
+      if (q.s) (void)hipStreamDestroy(q.s);
+      q.s = nullptr;
+      q.ready = nullptr;
+    }
+    if (pb.served) (void)hipEventDestroy(pb.served);
+    pb.served = nullptr;
+  };
+  try {
+    for (auto& q : pb.r) {
+      G2G_HIP(hipStreamCreate(&q.s));
+      G2G_HIP(hipEventCreateWithFlags(&q.ready, hipEventDisableTiming));
+    }
+    G2G_HIP(hipEventCreateWithFlags(&pb.served, hipEventDisableTiming));
+  } catch (...) {
+    release();
+    throw;
+  }
   auto worker = [&](int me) {
-    hipStream_t s = nullptr;
+    const hipStream_t s = pb.r[me].s;
     try {
-      G2G_HIP(hipStreamCreate(&s));
       SolveEnv env;
       env.stream = s;
       env.ws = &op.ws[me];
       env.mv64 = &out.mv64[me];
       env.mv32 = &out.mv32[me];
-      env.A64 = [&, me, s](const zd* in, zd* o) {
-        G2G_HIP(hipStreamSynchronize(s));
-        pb.apply(me, in, o, 0);
-      };
-      env.A32 = [&, me, s](const zf* in, zf* o) {
-        G2G_HIP(hipStreamSynchronize(s));
-        pb.apply(me, in, o, 1);
-      };
+      env.A64 = [&, me](const zd* in, zd* o) { pb.apply(me, in, o, 0); };
+      env.A32 = [&, me](const zf* in, zf* o) { pb.apply(me, in, o, 1); };
       out.r[me] = nested_solve(op, b[me], x0[me], xout[me], p, &out.hist[me], nullptr, &env);
     } catch (...) {
       err[me] = std::current_exception();
     }
     pb.finish(me);
-    if (s) (void)hipStreamDestroy(s);
   };
   std::thread t0(worker, 0), t1(worker, 1);
   t0.join();
   t1.join();
+  (void)hipStreamSynchronize(op.stream);
+  release();
   for (auto& e : err)
     if (e) std::rethrow_exception(e);
   out.paired = pb.paired;
