@@ -216,6 +216,48 @@ __global__ __launch_bounds__(64) void k_publish(const double* __restrict__ src, 
 // the host polls this long before it falls back to a blocking synchronisation
 constexpr int MAIL_SPIN_MS = 200;
 
+// The last reduction of a Gram-Schmidt pass (one block): |w|^2 from the
+// update's partial sums, the pass's test 2|w| > |h| (accepted also when
+// `force`, the fourth pass), both kept in res[2 MAXK], res[2 MAXK + 1] for
+// the conditional scaling, and published with the pass's coefficients h
+// (res[0 .. 2K), written by the previous reduction) in the host mailbox.
+template <typename A>
+__global__ __launch_bounds__(BT) void k_reduce_pass(const A* __restrict__ part, int nb, double* res, int K,
+                                                    int force, Mailbox* mb, unsigned seq) {
+  __shared__ double red[BT / 64];
+  double acc = 0;
+  for (int b = threadIdx.x; b < nb; b += BT) acc += (double)part[b * 2 * KC];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    for (int i = threadIdx.x; i < 2 * K; i += 64) mb->v[i] = res[i];
+    if (threadIdx.x == 0) {
+      double w2 = 0;
+      for (int q = 0; q < BT / 64; q++) w2 += red[q];
+      double h2 = 0;
+      for (int i = 0; i < K; i++) h2 += res[2 * i] * res[2 * i] + res[2 * i + 1] * res[2 * i + 1];
+      const double ok = (force || 2.0 * sqrt(w2) > sqrt(h2)) ? 1.0 : 0.0;
+      res[2 * MAXK] = w2;
+      res[2 * MAXK + 1] = ok;
+      mb->v[2 * MAXK] = w2;
+      mb->v[2 * MAXK + 1] = ok;
+    }
+    __threadfence_system();  // the values are visible to the host before the number
+    if (threadIdx.x == 0) __hip_atomic_store(&mb->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+// dst = src / |w| when the pass was accepted (the next basis vector, queued
+// before the host has read the pass's results); nothing otherwise
+template <typename T2, typename R>
+__global__ void k_scale_if(T2* __restrict__ dst, const T2* __restrict__ src, int64_t N,
+                           const double* __restrict__ res) {
+  if (res[2 * MAXK + 1] == 0.0) return;
+  const R f = (R)(1.0 / sqrt(res[2 * MAXK]));
+  for (int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x; i < N; i += (int64_t)gridDim.x * BT)
+    dst[i] = src[i] * f;
+}
+
 // Waits until the work queued on s so far has ended, polling an event
 static void poll_event(hipEvent_t ev, hipStream_t s) {
   G2G_HIP(hipEventRecord(ev, s));
@@ -271,6 +313,10 @@ struct DevBlas {
     const unsigned seq = __atomic_load_n(&mail->seq, __ATOMIC_ACQUIRE) + 1;
     hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, s, (const double*)res, n, dmail, seq);
     G2G_HIP(hipGetLastError());
+    wait_mail(seq);
+  }
+  // until the mailbox holds number seq
+  void wait_mail(unsigned seq) {
     const auto t0 = std::chrono::steady_clock::now();
     for (unsigned i = 1;; i++) {
       if (__atomic_load_n(&mail->seq, __ATOMIC_ACQUIRE) == seq) return;
@@ -295,18 +341,27 @@ struct DevBlas {
   }
   // One classical Gram-Schmidt pass: h = V^H w, w -= V h, returns |w|; h goes
   // from the reduction straight to the update kernel, one host round trip.
+  // The device also makes the pass's test 2|w| > |h| (accepted when `force`)
+  // and, when it passes, writes vnext = w/|w| before the host has read the
+  // results; *accepted tells the host which it was.
   template <typename T2, typename A>
-  double ortho_pass(T2* w, const T2* V, int64_t N, int K, zc* h) {
+  double ortho_pass(T2* w, const T2* V, int64_t N, int K, zc* h, T2* vnext, bool force, bool* accepted) {
     const int pa = prof_start();
     mdot_dev<T2, A>(V, N, K, w);
     const int nb = nblocks(N);
     hipLaunchKernelGGL((k_update_norm<T2, A>), dim3(nb), dim3(BT), 0, s, w, V, N, K, (const zd*)res,
                        (A*)part);
-    hipLaunchKernelGGL((k_reduce<A>), dim3(1), dim3(BT), 0, s, (const A*)part, nb, res + 2 * MAXK);
+    const unsigned seq = __atomic_load_n(&mail->seq, __ATOMIC_ACQUIRE) + 1;
+    hipLaunchKernelGGL((k_reduce_pass<A>), dim3(1), dim3(BT), 0, s, (const A*)part, nb, res, K, force ? 1 : 0,
+                       dmail, seq);
+    hipLaunchKernelGGL((k_scale_if<T2, A>), gridN(N), dim3(BT), 0, s, vnext, (const T2*)w, N, (const double*)res);
+    G2G_HIP(hipGetLastError());
     prof_end(pa);
     bytes += (double)sizeof(T2) * N * (K + 2);
-    fetch(2 * MAXK + 2 * KC);
+    wait_mail(seq);
     for (int k = 0; k < K; k++) h[k] = zc(hres[2 * k], hres[2 * k + 1]);
+    *accepted = hres[2 * MAXK + 1] != 0.0;
+    if (*accepted) bytes += 2.0 * sizeof(T2) * N;
     dots += K + 1;
     passes++;
     return std::sqrt(hres[2 * MAXK]);
@@ -474,19 +529,19 @@ template <typename R> struct Gmres {
       }
       std::swap(w, r0);
       for (int i = 0; i < jH; i++) HH(i, j) = 0;
+      // iterated classical Gram-Schmidt: passes until 2|w| > |h| of the
+      // last pass, at most four; the device makes the test and writes the
+      // next basis vector w/|w| after the accepted pass
       int northo = 0;
       double dnormw = 0;
       for (;;) {
         northo++;
-        dnormw = blas->ortho_pass<T2, R>(w, V, N, jH, dp.data());
+        bool accepted = false;
+        dnormw = blas->ortho_pass<T2, R>(w, V, N, jH, dp.data(), Vc(jH), northo > 3, &accepted);
         for (int i = 0; i < jH; i++) HH(i, j) += dp[i];
-        double dloo = 0;
-        for (int i = 0; i < jH; i++) dloo += std::norm(dp[i]);
-        dloo = std::sqrt(dloo);
-        if ((2.0 * dnormw > dloo) || (northo > 3)) break;
+        if (accepted) break;
       }
       HH(jH, j) = dnormw;
-      blas->scale(Vc(jH), w, N, (R)(1.0 / dnormw));
       for (int i = 0; i < j; i++) zrot(HH(i, j), HH(i + 1, j), gc[i], gs[i]);
       zc h1 = HH(j, j), h2 = HH(j + 1, j);
       zrotg(h1, h2, gc[j], gs[j]);
