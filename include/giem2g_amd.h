@@ -159,8 +159,11 @@ int giem2g_op_apply_dev(void *ie_op, const void *in_dev, void *out_dev);
 int giem2g_op_apply_dev_f32(void *ie_op, const void *in_dev, void *out_dev);
 /* out0 = A*in0 and out1 = A*in1 with one read of the tensor (two-column
  * matrix-core contraction; needs Nz from 8 to 64); precision 0 complex128,
- * 1 complex64.  No reference counterpart: the reference applies the operator
- * to one vector at a time (apply_ie_operator_module.F90:39-96). */
+ * 1 complex64.  Under mode 0 with the x-y mirror at Nz where mode 5 serves
+ * one vector, two complex128 vectors take one mirrored contraction each
+ * (each reads half the tensor; together faster than the two-column kernel).  No
+ * reference counterpart: the reference applies the operator to one vector
+ * at a time (apply_ie_operator_module.F90:39-96). */
 int giem2g_op_apply_pair_dev(void *ie_op, int precision, const void *in0_dev, const void *in1_dev,
                              void *out0_dev, void *out1_dev);
 /* out[r] = A*in[r] for nrhs vectors stored one after another (3*Nx*Ny*Nz
@@ -285,7 +288,8 @@ int giem2g_solve_dev(void *ie_op, const void *E_bkg_dev, const void *E0_dev,
  * columns a one-vector tile would repeat).  Under mode 0 with the mirror:
  * mode 5 for FP64 at Nz 8-32 and complex64 at Nz 24-32, mode 6 otherwise.
  * Two-vector products (giem2g_op_apply_pair_dev, giem2g_solve_pair_dev) use
- * the matrix cores. */
+ * the matrix cores: in complex128 two mode-5 contractions under mode 0 where
+ * mode 5 would serve one vector, else the two-column tiles. */
 int giem2g_op_set_contraction(void *ie_op, int mode);
 
 /* Scalings between the field and the scaled system (ie_solver_module.F90:
