@@ -13,6 +13,7 @@
 // vectors with the complex64 operator (mixed precision; the outer stays FP64
 // and acts as iterative refinement).
 #include <chrono>
+#include <cstdlib>
 #include <complex>
 #include <functional>
 #include <vector>
@@ -213,8 +214,14 @@ __global__ __launch_bounds__(64) void k_publish(const double* __restrict__ src, 
   __threadfence_system();  // the values are visible to the host before the number
   if (threadIdx.x == 0) __hip_atomic_store(&mb->seq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
-// the host polls this long before it falls back to a blocking synchronisation
-constexpr int MAIL_SPIN_MS = 200;
+// How long the host polls before it falls back to a blocking
+// synchronisation: GIEM2G_POLL_MS milliseconds (0: block at once), else 200
+static int poll_limit_ms() {
+  const char* e = std::getenv("GIEM2G_POLL_MS");
+  if (!e || !*e) return 200;
+  const int v = std::atoi(e);
+  return v < 0 ? 0 : v;
+}
 
 // The last reduction of a Gram-Schmidt pass (one block): |w|^2 from the
 // update's partial sums, the pass's test 2|w| > |h| (accepted also when
@@ -258,15 +265,16 @@ __global__ void k_scale_if(T2* __restrict__ dst, const T2* __restrict__ src, int
     dst[i] = src[i] * f;
 }
 
-// Waits until the work queued on s so far has ended, polling an event
-static void poll_event(hipEvent_t ev, hipStream_t s) {
+// Waits until the work queued on s so far has ended, polling an event for
+// at most ms milliseconds
+static void poll_event(hipEvent_t ev, hipStream_t s, int ms) {
   G2G_HIP(hipEventRecord(ev, s));
   const auto t0 = std::chrono::steady_clock::now();
-  for (unsigned i = 1;; i++) {
+  for (unsigned i = 1; ms > 0; i++) {
     const hipError_t e = hipEventQuery(ev);
     if (e == hipSuccess) return;
     if (e != hipErrorNotReady) G2G_HIP(e);
-    if ((i & 255) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(MAIL_SPIN_MS)) break;
+    if ((i & 255) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(ms)) break;
     __builtin_ia32_pause();
   }
   G2G_HIP(hipEventSynchronize(ev));
@@ -281,6 +289,7 @@ struct DevBlas {
   Mailbox* mail = nullptr;   // pinned host memory
   Mailbox* dmail = nullptr;  // its device address
   const double* hres = nullptr;  // mail->v
+  int poll_ms = 200;             // poll_limit_ms() when the solve began
   // counts of this solver (merged into the operator's counters by the caller)
   int64_t dots = 0, passes = 0;
   double bytes = 0;  // bytes the Krylov kernels read and write
@@ -297,6 +306,7 @@ struct DevBlas {
     mail = (Mailbox*)ws.get_pinned(0, sizeof(Mailbox));
     G2G_HIP(hipHostGetDevicePointer((void**)&dmail, mail, 0));
     hres = mail->v;
+    poll_ms = poll_limit_ms();
   }
   static int nblocks(int64_t N) { return (int)std::min<int64_t>(NBLK, (N + BT - 1) / BT); }
   // res[0 .. 2K) = V[:,k]^H w for k < K, on the device (no host round trip)
@@ -318,9 +328,9 @@ struct DevBlas {
   // until the mailbox holds number seq
   void wait_mail(unsigned seq) {
     const auto t0 = std::chrono::steady_clock::now();
-    for (unsigned i = 1;; i++) {
+    for (unsigned i = 1; poll_ms > 0; i++) {
       if (__atomic_load_n(&mail->seq, __ATOMIC_ACQUIRE) == seq) return;
-      if ((i & 4095) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(MAIL_SPIN_MS))
+      if ((i & 4095) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(poll_ms))
         break;
       __builtin_ia32_pause();
     }
@@ -824,6 +834,7 @@ struct PairBatcher {
     hipEvent_t ready = nullptr;  // polled before the request is posted
   } r[2];
   hipEvent_t served = nullptr;   // polled after each product
+  int poll_ms = poll_limit_ms();
   bool active[2] = {true, true};
   long long paired = 0, alone = 0;
 
@@ -832,7 +843,7 @@ struct PairBatcher {
     else op_apply<float>(*op, op->d32, (const zf*)q.in, (zf*)q.out, op->stream, false);
   }
   void apply(int me, const void* in, void* out, int prec) {
-    poll_event(r[me].ready, r[me].s);  // s and ready do not change while the threads run
+    poll_event(r[me].ready, r[me].s, poll_ms);  // s and ready do not change while the threads run
     std::unique_lock<std::mutex> lk(mu);
     Req& m = r[me];
     m.in = in;
@@ -863,7 +874,7 @@ struct PairBatcher {
           run_one(m);
           alone++;
         }
-        poll_event(served, op->stream);
+        poll_event(served, op->stream, poll_ms);
         if (r[ot].pending) {
           r[ot].pending = false;
           r[ot].done = true;
