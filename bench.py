@@ -492,6 +492,8 @@ def main():
     ap.add_argument("--no-c3", action="store_true", help="skip the 128x128x64 multi-frequency run")
     ap.add_argument("--c3-freqs", type=int, default=16, help="number of C3 frequencies")
     ap.add_argument("--headline-only", action="store_true", help="only the C2 solves (profiling runs)")
+    ap.add_argument("--c3-only", action="store_true",
+                    help="only the 128x128x64 multi-frequency paired solves (profiling runs)")
     ap.add_argument("--no-solve", action="store_true",
                     help="skip the C2 solves: only the C1 apply, continuation and batched records "
                          "(profiling runs of those kernels)")
@@ -514,6 +516,11 @@ def main():
         dst.copy_(src)  # reads and writes exactly 1 GiB
         torch.cuda.synchronize()
         del src, dst
+    if a.c3_only:
+        line = {"metric": "C3 multi-frequency paired solves (profiling run)",
+                "c3": c3_bench(torch, gop, models, nfreq=a.c3_freqs)}
+        print(json.dumps(line), flush=True)
+        return 0
     if a.no_solve:
         line = {"metric": "IE operator apply (profiling run, no solves)"}
         m1, op1, v1, st64, st32, c1rec = c1_apply(torch, gop, models, 200, not a.no_cpu)
