@@ -261,7 +261,10 @@ typedef struct {
 } giem2g_solve_info;
 
 /* E_bkg, E0 (nullable warm start), E_sol: complex (Nx,Ny,Nz,3).  _dev takes
- * HBM pointers, the plain form host pointers. */
+ * HBM pointers, the plain form host pointers.  The calling thread polls for
+ * each Gram-Schmidt pass's results in pinned host memory (busy, for at most
+ * GIEM2G_POLL_MS milliseconds, default 200, before a blocking wait;
+ * GIEM2G_POLL_MS=0 blocks at once). */
 int giem2g_solve(void *ie_op, const double *E_bkg, const double *E0,
                  const giem2g_solver_params *p, double *E_sol,
                  giem2g_solve_info *info);
