@@ -376,10 +376,12 @@ SolveParams params_from(const giem2g_solver_params* p) {
     s.inner_precision = p->inner_precision;
     s.precond = p->precond;
     s.learned_patience = p->learned_patience == 0 ? 2 : (p->learned_patience < 0 ? 0 : p->learned_patience);
+    s.inner_tol = p->inner_tol;
   }
   G2G_CHECK(s.outer_m >= 1 && s.outer_m <= 63 && s.inner_m >= 1 && s.inner_m <= 63, -3,
             "buffer sizes must be in [1,63]");
   G2G_CHECK(s.max_restart >= 1 && s.tol > 0, -3, "bad max_restart or tol");
+  G2G_CHECK(s.inner_tol >= 0 && s.inner_tol < 1, -3, "inner_tol must be in [0,1)");
   return s;
 }
 

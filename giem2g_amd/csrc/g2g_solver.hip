@@ -674,7 +674,7 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
     if (p.inner_precision == 0) {
       in64.blas = &blas;
       in64.maxit = 1;
-      in64.tol = p.tol;
+      in64.tol = p.inner_tol > 0 ? p.inner_tol : p.tol;
       in64.alloc(N, p.inner_m, false, ws, WS_IN64);
       in64.matvecs = env ? env->mv64 : &op.cnt.matvecs;
       in64.A = outer.A;
@@ -682,7 +682,7 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
       if (!op.have_g32) op_make_g32(op);
       in32.blas = &blas;
       in32.maxit = 1;
-      in32.tol = p.tol;
+      in32.tol = p.inner_tol > 0 ? p.inner_tol : p.tol;
       in32.alloc(N, p.inner_m, false, ws, WS_IN32);
       in32.matvecs = env ? env->mv32 : &op.cnt.matvecs32;
       if (env) in32.A = env->A32;

@@ -28,6 +28,9 @@ struct SolveParams {
   // learned modes: after this many rejected learned steps in a row the
   // network is not called again in the solve (0: never stop)
   int learned_patience = 2;
+  // inner solves stop below this residual relative to their right-hand
+  // side; 0: tol (the reference's inner and outer share one tolerance)
+  double inner_tol = 0;
 };
 
 struct SolveResult {

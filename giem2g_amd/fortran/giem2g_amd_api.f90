@@ -13,6 +13,7 @@ module giem2g_amd_api
      integer(c_int) :: outer_m, inner_m, max_restart
      real(c_double) :: tol
      integer(c_int) :: inner_precision, precond, verbose, learned_patience
+     real(c_double) :: inner_tol
   end type giem2g_solver_params
 
   type, bind(c) :: giem2g_solve_info

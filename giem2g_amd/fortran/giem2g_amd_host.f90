@@ -108,6 +108,7 @@ program giem2g_amd_host
   prm%precond = json_get_int(deck, 'Solver.Precond', 0)
   prm%verbose = 0
   prm%learned_patience = 0
+  prm%inner_tol = json_get_real(deck, 'Solver.InnerTolerance', 0d0)
   accept = json_get_real(deck, 'Solver.Accept', 1d0)
   save_solution = json_get_logical(deck, 'Save Solution', .true.)
   solve_equation = json_get_logical(deck, 'Solve Equation', .true.)

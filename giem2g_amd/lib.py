@@ -59,7 +59,7 @@ class Data(C.Structure):
 class SolverParams(C.Structure):
     _fields_ = [("outer_m", C.c_int), ("inner_m", C.c_int), ("max_restart", C.c_int),
                 ("tol", C.c_double), ("inner_precision", C.c_int), ("precond", C.c_int),
-                ("verbose", C.c_int), ("learned_patience", C.c_int)]
+                ("verbose", C.c_int), ("learned_patience", C.c_int), ("inner_tol", C.c_double)]
 
 
 class SolveInfo(C.Structure):

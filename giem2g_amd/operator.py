@@ -255,13 +255,15 @@ class IEOperator:
         return E
 
     def solve(self, Ebkg, E0=None, outer_m=30, inner_m=30, max_restart=100, tol=1e-8,
-              inner_precision=0, precond=0, verbose=0, learned_patience=0):
+              inner_precision=0, precond=0, verbose=0, learned_patience=0, inner_tol=0.0):
         """Nested FGMRES solve of the scaled system for the background field
         Ebkg (numpy: on the host; torch: on the GPU).  learned_patience:
         rejected learned steps in a row after which the network is not
-        called again in this solve (0: 2, negative: never stop)."""
+        called again in this solve (0: 2, negative: never stop).  inner_tol:
+        relative residual at which an inner solve stops (0: tol, as in the
+        reference)."""
         p = L.SolverParams(outer_m, inner_m, max_restart, tol, inner_precision, precond, verbose,
-                           learned_patience)
+                           learned_patience, inner_tol)
         info = L.SolveInfo()
         if isinstance(Ebkg, np.ndarray):
             Ebkg = np.ascontiguousarray(Ebkg, np.complex128)
@@ -323,11 +325,11 @@ class IEOperator:
         L.check(self.lib.giem2g_op_learned_apply(self.h, a, b))
 
     def solve_pair(self, Eb0, Eb1, E00=None, E01=None, outer_m=30, inner_m=30, max_restart=100,
-                   tol=1e-8, inner_precision=0):
+                   tol=1e-8, inner_precision=0, inner_tol=0.0):
         """Two solves (torch complex128 tensors on the GPU) with paired
         operator products; returns (E0, E1, info0, info1, [paired, alone])."""
         import torch
-        p = L.SolverParams(outer_m, inner_m, max_restart, tol, inner_precision, 0, 0, 0)
+        p = L.SolverParams(outer_m, inner_m, max_restart, tol, inner_precision, 0, 0, 0, inner_tol)
         infos = (L.SolveInfo * 2)()
         cnt = (C.c_longlong * 2)()
         o0, o1 = torch.empty_like(Eb0), torch.empty_like(Eb1)

@@ -246,6 +246,10 @@ typedef struct {
   int learned_patience; /* learned modes: after this many rejected learned
                            steps in a row the network is not called again in
                            the solve; 0 = 2 (default), < 0 = never stop */
+  double inner_tol;     /* inner solves stop when their residual falls below
+                           inner_tol times their right-hand side's norm;
+                           0 = tol (default: the reference's inner and outer
+                           share one tolerance, seq_fgmres.F90:48-49, 62-65) */
 } giem2g_solver_params;
 
 typedef struct {
