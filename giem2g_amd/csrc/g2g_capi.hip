@@ -381,7 +381,7 @@ SolveParams params_from(const giem2g_solver_params* p) {
   G2G_CHECK(s.outer_m >= 1 && s.outer_m <= 63 && s.inner_m >= 1 && s.inner_m <= 63, -3,
             "buffer sizes must be in [1,63]");
   G2G_CHECK(s.max_restart >= 1 && s.tol > 0, -3, "bad max_restart or tol");
-  G2G_CHECK(s.inner_tol >= 0 && s.inner_tol < 1, -3, "inner_tol must be in [0,1)");
+  G2G_CHECK(s.inner_tol >= -1e6 && s.inner_tol < 1, -3, "inner_tol must be below 1 (negative: relaxed)");
   return s;
 }
 

@@ -29,7 +29,9 @@ struct SolveParams {
   // network is not called again in the solve (0: never stop)
   int learned_patience = 2;
   // inner solves stop below this residual relative to their right-hand
-  // side; 0: tol (the reference's inner and outer share one tolerance)
+  // side; < 0: relaxed, |inner_tol| tol divided by the outer's current
+  // relative residual, between tol and 0.1; 0: tol for FP64 inner solves
+  // (the reference's inner and outer share one tolerance), -1 for complex64
   double inner_tol = 0;
 };
 

@@ -260,8 +260,9 @@ class IEOperator:
         Ebkg (numpy: on the host; torch: on the GPU).  learned_patience:
         rejected learned steps in a row after which the network is not
         called again in this solve (0: 2, negative: never stop).  inner_tol:
-        relative residual at which an inner solve stops (0: tol, as in the
-        reference)."""
+        relative residual at which an inner solve stops; negative: relaxed,
+        |inner_tol| tol / (the outer's current relative residual); 0: tol
+        for FP64 inner solves (as in the reference), -1 for complex64 ones."""
         p = L.SolverParams(outer_m, inner_m, max_restart, tol, inner_precision, precond, verbose,
                            learned_patience, inner_tol)
         info = L.SolveInfo()

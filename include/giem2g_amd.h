@@ -248,8 +248,13 @@ typedef struct {
                            the solve; 0 = 2 (default), < 0 = never stop */
   double inner_tol;     /* inner solves stop when their residual falls below
                            inner_tol times their right-hand side's norm;
-                           0 = tol (default: the reference's inner and outer
-                           share one tolerance, seq_fgmres.F90:48-49, 62-65) */
+                           < 0 = relaxed: |inner_tol| * tol / (the outer's
+                           current relative residual), between tol and 0.1
+                           (inexact Krylov, Simoncini and Szyld 2003);
+                           0 = default: tol for FP64 inner solves (the
+                           reference's inner and outer share one tolerance,
+                           seq_fgmres.F90:48-49, 62-65), -1 for complex64
+                           inner solves (inner_precision 1) */
 } giem2g_solver_params;
 
 typedef struct {

@@ -154,12 +154,13 @@ def test_blocking_waits_give_the_same_solves(monkeypatch):
 
 @pytest.mark.parametrize("prec", [0, 1])
 def test_inner_tolerance(prec):
-    """inner_tol: the inner solves stop at their own relative residual
-    instead of the outer tolerance.  With inner_tol = 1e-2 the solve still
-    reaches the FP64 1e-8 true residual and the default solve's field within
-    1e-6, with at most the default's matvecs plus one outer cycle; 0 is the
-    default (the reference's shared tolerance); values outside [0, 1) are
-    rejected."""
+    """inner_tol: the inner solves stop at their own relative residual.
+    0 is the default (FP64 inner: the reference's shared tolerance;
+    complex64 inner: the relaxed rule, factor 1).  A fixed 1e-2, the shared
+    tolerance given explicitly and the relaxed rule with factor 10 all reach
+    the FP64 1e-8 true residual and the default solve's field within 1e-6,
+    with at most twice the default's matvecs plus one outer cycle; values from 1
+    up are rejected."""
     import torch
     from giem2g_amd import lib as L, models, operator as gop
     print("inner tolerance, precision", prec, flush=True)
@@ -169,14 +170,15 @@ def test_inner_tolerance(prec):
     kw = dict(outer_m=10, inner_m=10, max_restart=50, tol=1e-8, inner_precision=prec)
     s0, i0 = op.solve(E, **kw)
     s1, i1 = op.solve(E, inner_tol=0.0, **kw)
-    s2, i2 = op.solve(E, inner_tol=1e-2, **kw)
-    print(i0, i2, flush=True)
-    s0, s1, s2 = (t.cpu().numpy() for t in (s0, s1, s2))
+    s0, s1 = s0.cpu().numpy(), s1.cpu().numpy()
     assert np.array_equal(s0, s1) and i0["outer_iterations"] == i1["outer_iterations"]
-    assert i2["status"] == 1 and i2["true_residual"] < 1e-8
-    assert rel(s2, s0) < 1e-6
     mv = lambda i: i["matvecs_fp64"] + i["matvecs_fp32"]
-    assert mv(i2) <= mv(i0) + 11
+    for itol in (1e-2, 1e-8, -10.0):
+        s2, i2 = op.solve(E, inner_tol=itol, **kw)
+        print(itol, i0, i2, flush=True)
+        assert i2["status"] == 1 and i2["true_residual"] < 1e-8
+        assert rel(s2.cpu().numpy(), s0) < 1e-6
+        assert mv(i2) <= 2 * mv(i0) + 11
     with pytest.raises(L.G2GError, match="inner_tol"):
         op.solve(E, inner_tol=1.5, **kw)
     op.close()
