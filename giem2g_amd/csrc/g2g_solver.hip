@@ -448,6 +448,9 @@ template <typename R> struct Gmres {
   std::vector<double> gc;
   int iternum = 0;
   double cur_res = 1;  // relative residual after the last Arnoldi step
+  // FinalActions recomputes the true residual (one more product); an inner
+  // solve whose caller only takes the solution can skip it
+  bool true_residual = true;
   Apply A;
   Apply precond;  // right preconditioner (flex only); identity when empty
   // optional check of a variable preconditioner: accept_step(v, A z) false
@@ -511,8 +514,12 @@ template <typename R> struct Gmres {
     blas->copy(x, solution, N);
   }
   // FinalActions (:382-421)
-  double final_actions(int jH, double sPb, const T2* rhs, T2* solution, int* stat) {
+  double final_actions(int jH, double sPb, const T2* rhs, T2* solution, int* stat, double bea) {
     construct(jH, solution);
+    if (!true_residual) {
+      *stat = bea >= tol ? 2 : 1;
+      return bea;
+    }
     blas->copy(w, solution, N);
     apply(w, r0);
     blas->bminus(r0, rhs, N);
@@ -617,7 +624,7 @@ template <typename R> struct Gmres {
       jH = 1;
       bea = krylov(bn, &jH);
       if (bea < tol) {
-        r.be = final_actions(jH, sPb, b, xo, &stat);
+        r.be = final_actions(jH, sPb, b, xo, &stat, bea);
         r.stat = stat;
         r.iterations = iternum;
         r.bea = bea;
@@ -627,7 +634,7 @@ template <typename R> struct Gmres {
       if (iter >= maxit) break;
       beta = restart(xo, jH);
     }
-    r.be = final_actions(jH, sPb, b, xo, &stat);
+    r.be = final_actions(jH, sPb, b, xo, &stat, bea);
     r.bea = bea;
     r.iterations = iternum;
     r.stat = -1;
@@ -693,6 +700,9 @@ SolveResult nested_solve(Operator& op, const zd* b, const zd* x0, zd* xout,
       in32.blas = &blas;
       in32.maxit = 1;
       in32.tol = itol > 0 ? itol : p.tol;
+      // the complex64 inner solve's own true residual is never used (the
+      // FP64 outer judges the step); the FP64 inner keeps the reference's
+      in32.true_residual = false;
       in32.alloc(N, p.inner_m, false, ws, WS_IN32);
       in32.matvecs = env ? env->mv32 : &op.cnt.matvecs32;
       if (env) in32.A = env->A32;
