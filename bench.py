@@ -546,7 +546,9 @@ def main():
         "data": "synthetic (seeded PCG64 random-box model; trained network weights committed)",
         "config": {"workload": "C2 %s solve to 1e-8: 256x256x32 cells of 100x100x25 m, 20 random boxes "
                                "1e-3..10 S/m in 0.01 S/m, background 0.01/0.1/1 S/m, f=0.1 Hz, EY; nested "
-                               "FGMRES(30) outer FP64, inner GMRES(30) complex64" % top_name,
+                               "FGMRES(30) outer FP64, inner GMRES(30) complex64 stopping at the relaxed "
+                               "tolerance tol/r (r: the outer's current relative residual); the FP64 solve "
+                               "keeps the reference's shared tolerance" % top_name,
                    "grid": [m2.Nx, m2.Ny, m2.Nz], "solver": SOLVER, "headline_solve": top_name},
         "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ach / HBM_PEAK_GBS if ach else None,
